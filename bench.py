@@ -1,0 +1,179 @@
+"""Throughput of the D3Q19 MRT channel step on one MI355X.
+
+A step is one iteration of the reference's main loop body
+(Channel-Flow/main.f90:157-161: collision_MRT with its periodic exchange,
+then macrovar) over the whole lattice, run by liblbm's fused kernel.
+
+Workload (default): turbulent channel Re_tau = 180, nu = 0.0036, MRTtype 1,
+768 streamwise x 384 spanwise x 192 wall-normal nodes (reference axes
+lx=192, ly=768, lz=384; 56.6 M nodes), log-law mean profile plus a seeded
+divergence-free random-phase perturbation of 10% amplitude, populations
+resident in HBM before timing.  `value` is FP64 MLUPS; FP32 and FP16-storage
+variants of the same workload are reported under "variants".
+
+Roofline: the step kernel is HBM-bound; algorithmic traffic is one read and
+one write of the 19 populations per node (304 B FP64, 152 B FP32, 76 B FP16).
+`achieved` = algorithmic bytes per launch / mean kernel duration from HIP
+events recorded on the library's stream around every launch.
+
+cpu_baseline: the CPU oracle's out-of-place schedule (same arithmetic as the
+reference, OpenMP over z planes) on the host cores, on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "d3q19-single-phase_amd"), os.path.join(ROOT, "oracle")]
+
+import numpy as np  # noqa: E402
+
+import d3q19  # noqa: E402
+from d3q19 import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_NODE = {"fp64": 304, "fp32": 152, "fp16": 76}
+WORKLOADS = {
+    # name: (lx, ly, lz, description)
+    "c2": (192, 768, 384, "turbulent channel Re_tau=180, 768x384x192 (streamwise x spanwise x wall-normal)"),
+    "c1": (32, 64, 32, "channel 64x32x32 (streamwise x spanwise x wall-normal)"),
+    "c5": (96, 384, 192, "coarse channel 384x192x96 (streamwise x spanwise x wall-normal)"),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(prec, workload):
+    """HBM bytes per launch measured by rocprofv3 --pmc (committed summary), or None."""
+    path = os.path.join(ROOT, "profiles", f"step_{prec}_{workload}_traffic.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh).get("hbm_bytes_per_launch")
+    return None
+
+
+def init_channel(ch, lx, ly, lz):
+    ux, uy, uz = workloads.loglaw_divfree(lx, ly, lz, ch.p["ustar"], ch.p["ystar"], seed=2)
+    ch.set_macros(ux=ux, uy=uy, uz=uz)
+    ch.FORCING()
+    ch.initpop()
+    ch.macrovar()
+    return ux, uy, uz
+
+
+def measure(prec, workload, steps, warmup):
+    lx, ly, lz, _ = WORKLOADS[workload]
+    n = lx * ly * lz
+    with d3q19.Channel(lx, ly, lz, prec) as ch:
+        init_channel(ch, lx, ly, lz)
+        m0 = ch.mass()
+        ch.run(warmup)
+        ch.sync()
+        t0 = time.perf_counter()
+        avg_ms, total_ms = ch.time_steps(steps)
+        ch.sync()
+        wall = time.perf_counter() - t0
+        m1 = ch.mass()
+    mlups = n * steps / wall / 1e6
+    algo = n * BYTES_PER_NODE[prec]
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles(prec, workload)
+    return {
+        "mlups": mlups,
+        "ms_per_step": wall / steps * 1e3,
+        "kernel_ms": avg_ms,
+        "event_total_ms": total_ms,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": algo},
+        "mass_drift": (m1 - m0) / max(abs(m0), 1e-300),
+    }
+
+
+def cpu_baseline(sample_steps, threads):
+    """Oracle out-of-place step (collision + macrovar) on the full default grid."""
+    import oracle
+    lx, ly, lz, _ = WORKLOADS["c2"]
+    o = oracle.OracleChannel(lx, ly, lz)
+    o.FORCING()
+    ux, uy, uz = workloads.loglaw_divfree(lx, ly, lz, o.p.ustar, o.p.ystar, seed=2)
+    o.ux[:], o.uy[:], o.uz[:] = ux, uy, uz
+    del ux, uy, uz
+    o.initpop()
+    o.macrovar()
+    o.run(1, ab=True, nthreads=threads)  # warm the pages
+    t0 = time.perf_counter()
+    o.run(sample_steps, ab=True, nthreads=threads)
+    dt = time.perf_counter() - t0
+    n = lx * ly * lz
+    return {"value": round(n * sample_steps / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{sample_steps} steps (collision_MRT + macrovar) of the full {lx}x{ly}x{lz} FP64 grid, "
+                      f"oracle out-of-place schedule, OpenMP {threads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--variants", default="fp32,fp16", help="comma list of extra storage precisions ('' = none)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU slab decomposition is a design, not part of this run", file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    log(f"bench: workload {args.workload} fp64 steps={args.steps} warmup={args.warmup}")
+    main_m = measure("fp64", args.workload, args.steps, args.warmup)
+    log(f"bench: fp64 {main_m['mlups']:.1f} MLUPS, kernel {main_m['kernel_ms']:.3f} ms, "
+        f"{main_m['roofline']['achieved']:.0f} GB/s")
+    variants = {}
+    for prec in [v for v in args.variants.split(",") if v]:
+        log(f"bench: variant {prec}")
+        v = measure(prec, args.workload, args.steps, args.warmup)
+        variants[prec] = {"value": round(v["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(v["ms_per_step"], 4),
+                          "kernel_ms": round(v["kernel_ms"], 4), "roofline": v["roofline"],
+                          "speedup_vs_fp64": round(v["mlups"] / main_m["mlups"], 3),
+                          "mass_drift": v["mass_drift"]}
+        log(f"bench: {prec} {v['mlups']:.1f} MLUPS, {v['roofline']['achieved']:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu and rank == 0:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"bench: cpu baseline, {threads} threads")
+        cpu = cpu_baseline(args.cpu_steps, threads)
+
+    lx, ly, lz, desc = WORKLOADS[args.workload]
+    out = {
+        "metric": "MLUPS (FP64 D3Q19 MRT channel step) + % of HBM peak",
+        "value": round(main_m["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_m["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded log-law + divergence-free perturbation)",
+        "config": {"workload": desc, "lx": lx, "ly": ly, "lz": lz, "nodes": lx * ly * lz,
+                   "kernel_ms": round(main_m["kernel_ms"], 4), "mass_drift": main_m["mass_drift"]},
+        "roofline": main_m["roofline"],
+        "cpu_baseline": cpu,
+        "variants": variants,
+    }
+    if rank == 0:
+        print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

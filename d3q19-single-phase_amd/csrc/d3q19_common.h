@@ -1,0 +1,408 @@
+// D3Q19 MRT node arithmetic for gfx950, shared by the FP64 and FP32/FP16
+// kernel translation units.
+//
+// mrt_collide() follows Channel-Flow/collision.f90:60-220 operation by
+// operation: every sum is evaluated left to right as written there and every
+// product keeps the reference's operand grouping, so with FMA contraction off
+// (the FP64 translation unit is built with -ffp-contract=off) the FP64 result
+// is the reference's bit for bit.  Two value-preserving simplifications are
+// made: products of a population or force with the lattice integers
+// cix/ciy/ciz in {-1,0,1} keep only the nonzero terms (x*1 = x, x*(-1) = -x and
+// adding an exact zero leave every nonzero value unchanged), and products of
+// two structural constants of para.f90:143-174 (e.g. coef2*val2i) are folded
+// at compile time, where IEEE round-to-nearest gives the same double the
+// reference forms at run time.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace d3q19 {
+
+// Velocity set in d'Humieres et al. (2002) order, Channel-Flow/para.f90:201-206.
+constexpr int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+
+// Unsigned division by a run-time constant d via a multiply-high, valid for
+// numerators below 2^31 (every node index here is below 2^31).
+struct FastDiv {
+    uint32_t d, m, s;
+};
+
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    uint32_t s = 0;
+    while (s < 32 && (1ull << s) < d)
+        s++;
+    uint64_t m = ((1ull << 32) * ((1ull << s) - d)) / d + 1;
+    return FastDiv{d, (uint32_t)m, s};
+}
+
+__host__ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f)
+{
+    const uint32_t t = (uint32_t)(((uint64_t)n * f.m) >> 32);
+    return (t + n) >> f.s;
+}
+
+// Grid geometry of one rank's block: x wall-normal (fastest in memory),
+// y streamwise, z spanwise.  Populations are stored structure-of-arrays:
+// element (i, n) at i*N + n with n = ix + lx*(iy + ly*iz).
+struct Geom {
+    uint32_t lx, ly, lz, plane; // plane = lx*ly
+    uint64_t N;
+    FastDiv dlx, dly;
+    uint32_t remap; // 1: map consecutive node blocks to the same XCD
+};
+
+// Block index -> logical block index so that blocks dispatched to the same
+// XCD (round-robin, blockIdx % 8) cover one contiguous range of nodes.
+__host__ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
+{
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// Per-node neighbour offsets (mod 2^32) and wall flags.
+struct Nbr {
+    uint32_t n;
+    uint32_t ym, yp, zm, zp; // offsets to the iy-1, iy+1, iz-1, iz+1 neighbours (wrapped)
+    bool wlo, whi;           // ix == 0, ix == lx-1
+};
+
+__host__ __device__ __forceinline__ Nbr make_nbr(uint32_t n, const Geom &g)
+{
+    Nbr b;
+    b.n = n;
+    const uint32_t q = fdiv(n, g.dlx);
+    const uint32_t ix = n - q * g.lx;
+    const uint32_t iz = fdiv(q, g.dly);
+    const uint32_t iy = q - iz * g.ly;
+    b.ym = (iy == 0) ? (g.ly - 1) * g.lx : (uint32_t)(-(int32_t)g.lx);
+    b.yp = (iy == g.ly - 1) ? (uint32_t)(-(int32_t)((g.ly - 1) * g.lx)) : g.lx;
+    b.zm = (iz == 0) ? (g.lz - 1) * g.plane : (uint32_t)(-(int32_t)g.plane);
+    b.zp = (iz == g.lz - 1) ? (uint32_t)(-(int32_t)((g.lz - 1) * g.plane)) : g.plane;
+    b.wlo = (ix == 0);
+    b.whi = (ix == g.lx - 1);
+    return b;
+}
+
+// Index of x - c_I (the pull source of direction I), ignoring walls.
+template <int I>
+__host__ __device__ __forceinline__ uint32_t src_of(const Nbr &b)
+{
+    uint32_t s = b.n;
+    if constexpr (CX[I] == 1)
+        s -= 1u;
+    else if constexpr (CX[I] == -1)
+        s += 1u;
+    if constexpr (CY[I] == 1)
+        s += b.ym;
+    else if constexpr (CY[I] == -1)
+        s += b.yp;
+    if constexpr (CZ[I] == 1)
+        s += b.zm;
+    else if constexpr (CZ[I] == -1)
+        s += b.zp;
+    return s;
+}
+
+// Is x - c_I beyond a wall?
+template <int I>
+__host__ __device__ __forceinline__ bool walled(const Nbr &b)
+{
+    if constexpr (CX[I] == 1)
+        return b.wlo;
+    else if constexpr (CX[I] == -1)
+        return b.whi;
+    else
+        return false;
+}
+
+// Relaxation rates and body force: the run-time constants of
+// Channel-Flow/para.f90:106-141 and collision.f90:522-524.
+template <typename T>
+struct Rates {
+    T s1, s2, s4, s9, s10, s13, s16;
+    T omegepsl, omegepslj, omegxx;
+    T fx, fy, fz;
+};
+
+// Storage precision traits.  FP16 keeps f_i * 2^k with 2^k close to 1/w_i
+// (4, 16, 32), so all directions use the same part of the half range;
+// scaling by a power of two is exact.
+template <int P>
+struct Prec;
+
+template <>
+struct Prec<0> {
+    using S = double;
+    using T = double;
+    template <int I>
+    static __device__ __forceinline__ T ld(const S *p) { return __builtin_nontemporal_load(p); }
+    template <int I>
+    static __device__ __forceinline__ void st(S *p, T v) { __builtin_nontemporal_store(v, p); }
+    template <int I>
+    static __device__ __forceinline__ T to_T(S v) { return v; }
+    template <int I>
+    static __device__ __forceinline__ S to_S(T v) { return v; }
+};
+
+template <>
+struct Prec<1> {
+    using S = float;
+    using T = float;
+    template <int I>
+    static __device__ __forceinline__ T ld(const S *p) { return __builtin_nontemporal_load(p); }
+    template <int I>
+    static __device__ __forceinline__ void st(S *p, T v) { __builtin_nontemporal_store(v, p); }
+    template <int I>
+    static __device__ __forceinline__ T to_T(S v) { return v; }
+    template <int I>
+    static __device__ __forceinline__ S to_S(T v) { return v; }
+};
+
+template <int I>
+constexpr float fp16_scale() { return I == 0 ? 4.0f : (I <= 6 ? 16.0f : 32.0f); }
+
+template <>
+struct Prec<2> {
+    using S = _Float16;
+    using T = float;
+    template <int I>
+    static __device__ __forceinline__ T to_T(S v) { return (float)v * (1.0f / fp16_scale<I>()); }
+    template <int I>
+    static __device__ __forceinline__ S to_S(T v) { return (_Float16)(v * fp16_scale<I>()); }
+    template <int I>
+    static __device__ __forceinline__ T ld(const S *p) { return to_T<I>(__builtin_nontemporal_load(p)); }
+    template <int I>
+    static __device__ __forceinline__ void st(S *p, T v) { __builtin_nontemporal_store(to_S<I>(v), p); }
+};
+
+// Lattice velocity component times a value: c*a for c in {-1,0,1}.
+template <int C, typename T>
+__device__ __forceinline__ T cmul(T a) { return C > 0 ? a : -a; }
+
+// cix*a + ciy*b + ciz*c (collision.f90:73-74, 79-80) keeping only the
+// nonzero terms, summed in x, y, z order.
+template <int I, typename T>
+__device__ __forceinline__ T cdot(T a, T b, T c)
+{
+    constexpr int cx = CX[I], cy = CY[I], cz = CZ[I];
+    if constexpr (cx != 0 && cy != 0)
+        return cmul<cx>(a) + cmul<cy>(b);
+    else if constexpr (cx != 0 && cz != 0)
+        return cmul<cx>(a) + cmul<cz>(c);
+    else if constexpr (cy != 0 && cz != 0)
+        return cmul<cy>(b) + cmul<cz>(c);
+    else if constexpr (cx != 0)
+        return cmul<cx>(a);
+    else if constexpr (cy != 0)
+        return cmul<cy>(b);
+    else if constexpr (cz != 0)
+        return cmul<cz>(c);
+    else
+        return T(0);
+}
+
+// Guo forcing term Fbar_i, collision.f90:70-82.
+template <int I, typename T>
+__device__ __forceinline__ T fbar(T ux9, T uy9, T uz9, T fx9, T fy9, T fz9, T G3)
+{
+    if constexpr (I == 0) {
+        return -G3;
+    } else {
+        constexpr T W = I <= 6 ? T(1.0 / 18.0) : T(1.0 / 36.0); // ww1, ww2: para.f90:173-174
+        T G1 = cdot<I>(fx9, fy9, fz9);
+        T G2 = cdot<I>(ux9, uy9, uz9);
+        return W * (T(3.) * G1 + T(9.) * G1 * G2 - T(3.) * G3);
+    }
+}
+
+// macrovar, collision.f90:398-413: momentum sums and density in the
+// reference's order (before adding F/2).
+template <typename T>
+__device__ __forceinline__ void macro_sums(const T (&f9)[19], T &rho9, T &ux9, T &uy9, T &uz9)
+{
+    T sum1 = f9[7] - f9[10];
+    T sum2 = f9[9] - f9[8];
+    T sum3 = f9[11] - f9[14];
+    T sum4 = f9[13] - f9[12];
+    T sum5 = f9[15] - f9[18];
+    T sum6 = f9[17] - f9[16];
+    ux9 = f9[1] - f9[2] + sum1 + sum2 + sum3 + sum4;
+    uy9 = f9[3] - f9[4] + sum1 - sum2 + sum5 + sum6;
+    uz9 = f9[5] - f9[6] + sum3 - sum4 + sum5 - sum6;
+    rho9 = f9[0] + f9[1] + f9[2] + f9[3] + f9[4] + f9[5] + f9[6] + f9[7] + f9[8] + f9[9] + f9[10] + f9[11] +
+           f9[12] + f9[13] + f9[14] + f9[15] + f9[16] + f9[17] + f9[18];
+}
+
+// One MRT collision with Guo forcing, collision.f90:60-220.  On entry f holds
+// the node's populations; on exit the stored post-collision values
+// f9(ip) + 0.5*Fbar(ip) (the values the swap loop writes, :230-262).
+template <typename T>
+__device__ __forceinline__ void mrt_collide(T (&f)[19], T rho9, T ux9, T uy9, T uz9, const Rates<T> &r)
+{
+    const T fx9 = r.fx, fy9 = r.fy, fz9 = r.fz;
+    const T ux9s = ux9 * ux9; // :60-62
+    const T uy9s = uy9 * uy9;
+    const T uz9s = uz9 * uz9;
+    const T G3 = ux9 * fx9 + uy9 * fy9 + uz9 * fz9; // :68
+
+    T Fb[19];
+    Fb[0] = fbar<0>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[1] = fbar<1>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[2] = fbar<2>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[3] = fbar<3>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[4] = fbar<4>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[5] = fbar<5>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[6] = fbar<6>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[7] = fbar<7>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[8] = fbar<8>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[9] = fbar<9>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[10] = fbar<10>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[11] = fbar<11>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[12] = fbar<12>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[13] = fbar<13>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[14] = fbar<14>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[15] = fbar<15>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[16] = fbar<16>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[17] = fbar<17>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+    Fb[18] = fbar<18>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+
+    T f9[19];
+#pragma unroll
+    for (int i = 0; i < 19; i++) // :84
+        f9[i] = f[i] + T(0.5) * Fb[i];
+
+    // :86-101 equilibrium moments (incompressible; no rho divisor)
+    const T t1 = ux9s + uy9s + uz9s;
+    const T eqm1 = -(T(11.0) * rho9) + T(19.0) * t1;
+    const T eqm2 = r.omegepsl * rho9 + r.omegepslj * t1;
+    constexpr T coef1 = T(-(2.0 / 3.0));
+    const T eqm3 = coef1 * ux9;
+    const T eqm4 = coef1 * uy9;
+    const T eqm5 = coef1 * uz9;
+    const T eqm6 = T(2.0) * ux9s - uy9s - uz9s;
+    const T eqm7 = r.omegxx * eqm6;
+    const T eqm8 = uy9s - uz9s;
+    const T eqm9 = r.omegxx * eqm8;
+    const T eqm10 = ux9 * uy9;
+    const T eqm11 = uy9 * uz9;
+    const T eqm12 = ux9 * uz9;
+
+    // :103-119
+    const T sum1 = f9[1] + f9[2] + f9[3] + f9[4] + f9[5] + f9[6];
+    const T sum2 = f9[7] + f9[8] + f9[9] + f9[10] + f9[11] + f9[12] + f9[13] + f9[14] + f9[15] + f9[16] +
+                   f9[17] + f9[18];
+    const T sum3 = f9[7] - f9[8] + f9[9] - f9[10] + f9[11] - f9[12] + f9[13] - f9[14];
+    const T sum4 = f9[7] + f9[8] - f9[9] - f9[10] + f9[15] - f9[16] + f9[17] - f9[18];
+    const T sum5 = f9[11] + f9[12] - f9[13] - f9[14] + f9[15] + f9[16] - f9[17] - f9[18];
+    const T sum6 = f9[1] + f9[2];
+    const T sum7 = f9[3] + f9[4] + f9[5] + f9[6];
+    const T sum8 = f9[7] + f9[8] + f9[9] + f9[10] + f9[11] + f9[12] + f9[13] + f9[14];
+    const T sum9 = f9[15] + f9[16] + f9[17] + f9[18];
+    const T sum10 = f9[3] + f9[4] - f9[5] - f9[6];
+    const T sum11 = f9[7] + f9[8] + f9[9] + f9[10] - f9[11] - f9[12] - f9[13] - f9[14];
+
+    // :121-138 (coef2 = -11, coef3 = 8, coef4 = -4, coef5 = 2)
+    const T evlm1 = -(T(30.0) * f9[0]) + T(-11.0) * sum1 + T(8.0) * sum2;
+    const T evlm2 = T(12.0) * f9[0] + T(-4.0) * sum1 + sum2;
+    const T evlm3 = T(-4.0) * (f9[1] - f9[2]) + sum3;
+    const T evlm4 = T(-4.0) * (f9[3] - f9[4]) + sum4;
+    const T evlm5 = T(-4.0) * (f9[5] - f9[6]) + sum5;
+    const T evlm6 = T(2.0) * sum6 - sum7 + sum8 - T(2.0) * sum9;
+    const T evlm7 = T(-4.0) * sum6 + T(2.0) * sum7 + sum8 - T(2.0) * sum9;
+    const T evlm8 = sum10 + sum11;
+    const T evlm9 = -(T(2.0) * sum10) + sum11;
+    const T evlm10 = f9[7] - f9[8] - f9[9] + f9[10];
+    const T evlm11 = f9[15] - f9[16] - f9[17] + f9[18];
+    const T evlm12 = f9[11] - f9[12] - f9[13] + f9[14];
+    const T evlm13 = f9[7] - f9[8] + f9[9] - f9[10] - f9[11] + f9[12] - f9[13] + f9[14];
+    const T evlm14 = -f9[7] - f9[8] + f9[9] + f9[10] + f9[15] - f9[16] + f9[17] - f9[18];
+    const T evlm15 = f9[11] + f9[12] - f9[13] - f9[14] - f9[15] - f9[16] + f9[17] + f9[18];
+
+    // :140-154 relaxation (eqm13..15 = 0)
+    const T eqmc1 = evlm1 - r.s1 * (evlm1 - eqm1);
+    const T eqmc2 = evlm2 - r.s2 * (evlm2 - eqm2);
+    const T eqmc3 = evlm3 - r.s4 * (evlm3 - eqm3);
+    const T eqmc4 = evlm4 - r.s4 * (evlm4 - eqm4);
+    const T eqmc5 = evlm5 - r.s4 * (evlm5 - eqm5);
+    const T eqmc6 = evlm6 - r.s9 * (evlm6 - eqm6);
+    const T eqmc7 = evlm7 - r.s10 * (evlm7 - eqm7);
+    const T eqmc8 = evlm8 - r.s9 * (evlm8 - eqm8);
+    const T eqmc9 = evlm9 - r.s10 * (evlm9 - eqm9);
+    const T eqmc10 = evlm10 - r.s13 * (evlm10 - eqm10);
+    const T eqmc11 = evlm11 - r.s13 * (evlm11 - eqm11);
+    const T eqmc12 = evlm12 - r.s13 * (evlm12 - eqm12);
+    const T eqmc13 = evlm13 - r.s16 * (evlm13 - T(0.0));
+    const T eqmc14 = evlm14 - r.s16 * (evlm14 - T(0.0));
+    const T eqmc15 = evlm15 - r.s16 * (evlm15 - T(0.0));
+
+    // :157-177 inverse transform; val*i = 1/val*, coef3i = 1/8, coef4i = -1/4
+    const T tl1 = T(1.0 / 19.0) * rho9;
+    const T tl2 = T(-11.0 * (1.0 / 2394.0)) * eqmc1;
+    const T tl3 = T(8.0 * (1.0 / 2394.0)) * eqmc1;
+    const T tl4 = T(-4.0 * (1.0 / 252.0)) * eqmc2;
+    const T tl5 = T(1.0 / 252.0) * eqmc2;
+    const T tl6 = T(1.0 / 10.0) * ux9;
+    const T tl7 = T(1.0 / 40.0) * eqmc3;
+    const T tl8 = T(1.0 / 10.0) * uy9;
+    const T tl9 = T(1.0 / 40.0) * eqmc4;
+    const T tl10 = T(1.0 / 10.0) * uz9;
+    const T tl11 = T(1.0 / 40.0) * eqmc5;
+    const T tl12 = T(1.0 / 36.0) * eqmc6;
+    const T tl13 = T(1.0 / 72.0) * eqmc7;
+    const T tl14 = T(1.0 / 12.0) * eqmc8;
+    const T tl15 = T(1.0 / 24.0) * eqmc9;
+    const T tl16 = -(T(-0.25) * eqmc10);
+    const T tl17 = -(T(-0.25) * eqmc11);
+    const T tl18 = -(T(-0.25) * eqmc12);
+    const T tl19 = T(0.125) * eqmc13;
+    const T tl20 = T(0.125) * eqmc14;
+    const T tl21 = T(0.125) * eqmc15;
+
+    // :180-198
+    f9[0] = tl1 - T(30.0 * (1.0 / 2394.0)) * eqmc1 + T(12.0 * (1.0 / 252.0)) * eqmc2;
+    const T suma = tl1 + tl2 + tl4;
+    const T sumb = tl1 + tl3 + tl5;
+    const T sumc = tl6 + T(-4.0) * tl7;
+    const T sumd = T(2.0) * tl12 + T(-4.0) * tl13;
+    const T sume = tl8 + T(-4.0) * tl9;
+    const T sumf = -tl12 + T(2.0) * tl13 + tl14 - T(2.0) * tl15;
+    const T sumg = tl10 + T(-4.0) * tl11;
+    const T sumh = -tl12 + T(2.0) * tl13 - tl14 + T(2.0) * tl15;
+    const T sumi = tl12 + tl13 + tl14 + tl15;
+    const T sumk = tl12 + tl13 - tl14 - tl15;
+    const T sump = -(T(2.0) * tl12) - T(2.0) * tl13;
+    const T sum67 = tl6 + tl7;
+    const T sum89 = tl8 + tl9;
+    const T sum1011 = tl10 + tl11;
+
+    // :200-220
+    f9[1] = suma + sumc + sumd;
+    f9[2] = suma - sumc + sumd;
+    f9[3] = suma + sume + sumf;
+    f9[4] = suma - sume + sumf;
+    f9[5] = suma + sumg + sumh;
+    f9[6] = suma - sumg + sumh;
+    f9[7] = sumb + sum67 + sum89 + sumi + tl16 + tl19 - tl20;
+    f9[8] = sumb - sum67 + sum89 + sumi - tl16 - tl19 - tl20;
+    f9[9] = sumb + sum67 - sum89 + sumi - tl16 + tl19 + tl20;
+    f9[10] = sumb - sum67 - sum89 + sumi + tl16 - tl19 + tl20;
+    f9[11] = sumb + sum67 + sum1011 + sumk + tl18 - tl19 + tl21;
+    f9[12] = sumb - sum67 + sum1011 + sumk - tl18 + tl19 + tl21;
+    f9[13] = sumb + sum67 - sum1011 + sumk - tl18 - tl19 - tl21;
+    f9[14] = sumb - sum67 - sum1011 + sumk + tl18 + tl19 - tl21;
+    f9[15] = sumb + sum89 + sum1011 + sump + tl17 + tl20 - tl21;
+    f9[16] = sumb - sum89 + sum1011 + sump - tl17 - tl20 - tl21;
+    f9[17] = sumb + sum89 - sum1011 + sump - tl17 + tl20 + tl21;
+    f9[18] = sumb - sum89 - sum1011 + sump + tl17 - tl20 + tl21;
+
+#pragma unroll
+    for (int i = 0; i < 19; i++) // the stored value, :230-262
+        f[i] = f9[i] + T(0.5) * Fb[i];
+}
+
+} // namespace d3q19

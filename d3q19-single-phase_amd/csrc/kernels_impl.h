@@ -1,0 +1,480 @@
+// Lattice kernels for gfx950, templated on the storage precision P
+// (0 = FP64, 1 = FP32, 2 = FP16 storage / FP32 compute).  Included by one
+// translation unit per precision, which instantiates Launch<P>.
+//
+// Streaming scheme: two lattices (A-B), pull.  The device lattice holds the
+// post-collision populations P_k (the values collision.f90:230-262 stores);
+// the reference's post-streaming array f is F_k(x)_i = P_k(x - c_i)_i, or
+// P_k(x)_opp(i) when x - c_i lies beyond a wall (halfway bounce-back,
+// collision.f90:229-230,250-251), with periodic wraps in y and z
+// (collisionExchnge on one rank, collision.f90:281-372).  One step reads
+// F_k by that gather, forms rho and u as macrovar does (collision.f90:396-418)
+// unless the fields come from arrays, collides, and writes P_{k+1} with
+// aligned stores.  Every read and write is a permutation of whole values, so
+// the result equals the reference's in-place swap algorithm bit for bit.
+#pragma once
+
+#include <utility>
+
+#include "d3q19_common.h"
+#include "kernels.h"
+
+namespace d3q19 {
+
+constexpr int BLOCK = 256;
+
+// F_I(x): streamed population of direction I at node x.
+template <int P, int I>
+__device__ __forceinline__ typename Prec<P>::T pull(const typename Prec<P>::S *__restrict__ L, uint64_t N,
+                                                    const Nbr &b)
+{
+    using S = typename Prec<P>::S;
+    if constexpr (CX[I] != 0) {
+        // bounce-back reads the node's own opposite slot (stored with the
+        // opposite direction's scale)
+        const bool w = walled<I>(b);
+        const S *p = w ? (L + (uint64_t)OPP[I] * N + b.n) : (L + (uint64_t)I * N + src_of<I>(b));
+        const S v = __builtin_nontemporal_load(p);
+        return w ? Prec<P>::template to_T<OPP[I]>(v) : Prec<P>::template to_T<I>(v);
+    } else {
+        return Prec<P>::template ld<I>(L + (uint64_t)I * N + src_of<I>(b));
+    }
+}
+
+template <int P, int... Is>
+__device__ __forceinline__ void pull_all_impl(const typename Prec<P>::S *__restrict__ L, uint64_t N, const Nbr &b,
+                                              typename Prec<P>::T (&f)[19], std::integer_sequence<int, Is...>)
+{
+    ((f[Is] = pull<P, Is>(L, N, b)), ...);
+}
+
+template <int P>
+__device__ __forceinline__ void pull_all(const typename Prec<P>::S *__restrict__ L, uint64_t N, const Nbr &b,
+                                         typename Prec<P>::T (&f)[19])
+{
+    pull_all_impl<P>(L, N, b, f, std::make_integer_sequence<int, 19>{});
+}
+
+template <int P, int... Is>
+__device__ __forceinline__ void store_all_impl(typename Prec<P>::S *__restrict__ L, uint64_t N, uint32_t n,
+                                               const typename Prec<P>::T (&f)[19], std::integer_sequence<int, Is...>)
+{
+    ((Prec<P>::template st<Is>(L + (uint64_t)Is * N + n, f[Is])), ...);
+}
+
+template <int P>
+__device__ __forceinline__ void store_all(typename Prec<P>::S *__restrict__ L, uint64_t N, uint32_t n,
+                                          const typename Prec<P>::T (&f)[19])
+{
+    store_all_impl<P>(L, N, n, f, std::make_integer_sequence<int, 19>{});
+}
+
+// Scatter the streamed value F_I(x) back to the post-collision slot it came
+// from (the inverse of pull).
+template <int P, int... Is>
+__device__ __forceinline__ void scatter_all_impl(typename Prec<P>::S *__restrict__ L, uint64_t N, const Nbr &b,
+                                                 const typename Prec<P>::T (&f)[19], std::integer_sequence<int, Is...>)
+{
+    ((walled<Is>(b) ? Prec<P>::template st<OPP[Is]>(L + (uint64_t)OPP[Is] * N + b.n, f[Is])
+                    : Prec<P>::template st<Is>(L + (uint64_t)Is * N + src_of<Is>(b), f[Is])),
+     ...);
+}
+
+template <int P>
+__device__ __forceinline__ void scatter_all(typename Prec<P>::S *__restrict__ L, uint64_t N, const Nbr &b,
+                                            const typename Prec<P>::T (&f)[19])
+{
+    scatter_all_impl<P>(L, N, b, f, std::make_integer_sequence<int, 19>{});
+}
+
+__device__ __forceinline__ uint32_t node_index(const Geom &g)
+{
+    uint32_t blk = blockIdx.x;
+    if (g.remap)
+        blk = xcd_remap(blk, gridDim.x);
+    return blk * BLOCK + threadIdx.x;
+}
+
+// ---------------------------------------------------------------------------
+// One time step: gather F_k, macroscopic fields, MRT collision, store P_{k+1}.
+// MODE 0: rho, u from F_k (macrovar fused); MODE 1: same with rho - rho_off
+// (avedensity's correction); MODE 2: rho, u read from the field arrays.
+template <int P, int MODE>
+__global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
+{
+    using S = typename Prec<P>::S;
+    using T = typename Prec<P>::T;
+    const uint32_t n = node_index(a.g);
+    if (n >= a.g.N)
+        return;
+    const Nbr b = make_nbr(n, a.g);
+    const S *__restrict__ src = (const S *)a.src;
+    S *__restrict__ dst = (S *)a.dst;
+
+    T f[19];
+    pull_all<P>(src, a.g.N, b, f);
+
+    Rates<T> r;
+    r.s1 = (T)a.r.s1;
+    r.s2 = (T)a.r.s2;
+    r.s4 = (T)a.r.s4;
+    r.s9 = (T)a.r.s9;
+    r.s10 = (T)a.r.s10;
+    r.s13 = (T)a.r.s13;
+    r.s16 = (T)a.r.s16;
+    r.omegepsl = (T)a.r.omegepsl;
+    r.omegepslj = (T)a.r.omegepslj;
+    r.omegxx = (T)a.r.omegxx;
+    r.fx = (T)a.r.fx;
+    r.fy = (T)a.r.fy;
+    r.fz = (T)a.r.fz;
+
+    T rho9, ux9, uy9, uz9;
+    if constexpr (MODE == 2) {
+        rho9 = (T)a.rho[n];
+        ux9 = (T)a.ux[n];
+        uy9 = (T)a.uy[n];
+        uz9 = (T)a.uz[n];
+    } else {
+        macro_sums(f, rho9, ux9, uy9, uz9);
+        ux9 = ux9 + r.fx / T(2.); // collision.f90:415-417
+        uy9 = uy9 + r.fy / T(2.);
+        uz9 = uz9 + r.fz / T(2.);
+        if constexpr (MODE == 1)
+            rho9 = rho9 - (T)a.rho_off; // collision.f90:508
+    }
+    mrt_collide(f, rho9, ux9, uy9, uz9, r);
+    store_all<P>(dst, a.g.N, n, f);
+}
+
+// macrovar / rhoupdat / materialisation of the fields of lattice L.
+// WHAT 0: rho, ux, uy, uz = macrovar(F) with rho - rho_off.
+// WHAT 1: rho = sum F (rhoupdat); max |rho_new - rho_old| into *errbits.
+template <int P, int WHAT>
+__global__ void __launch_bounds__(BLOCK) k_macro(MacroArgs a)
+{
+    using S = typename Prec<P>::S;
+    using T = typename Prec<P>::T;
+    const uint32_t n = blockIdx.x * BLOCK + threadIdx.x;
+    double err = 0.0;
+    if (n < a.g.N) {
+        const Nbr b = make_nbr(n, a.g);
+        T f[19];
+        pull_all<P>((const S *)a.L, a.g.N, b, f);
+        T rho9, ux9, uy9, uz9;
+        macro_sums(f, rho9, ux9, uy9, uz9);
+        if constexpr (WHAT == 0) {
+            a.ux[n] = (double)(ux9 + (T)a.fx / T(2.));
+            a.uy[n] = (double)(uy9 + (T)a.fy / T(2.));
+            a.uz[n] = (double)(uz9 + (T)a.fz / T(2.));
+            a.rho[n] = (double)(rho9 - (T)a.rho_off);
+        } else {
+            const double rnew = (double)rho9;
+            err = fabs(rnew - a.rho[n]);
+            a.rho[n] = rnew;
+        }
+    }
+    if constexpr (WHAT == 1) {
+        // max over the block, then one atomic per block on the bit pattern
+        // (non-negative doubles order like their unsigned bit patterns)
+        for (int o = 32; o > 0; o >>= 1)
+            err = fmax(err, __shfl_xor(err, o));
+        __shared__ double wmax[BLOCK / 64];
+        if ((threadIdx.x & 63) == 0)
+            wmax[threadIdx.x >> 6] = err;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = wmax[0];
+            for (int w = 1; w < BLOCK / 64; w++)
+                m = fmax(m, wmax[w]);
+            atomicMax(a.errbits, (unsigned long long)__double_as_longlong(m));
+        }
+    }
+}
+
+// Download: F_k of nodes [n0, n0+count) as reference-layout FP64 rows of 19.
+template <int P>
+__global__ void __launch_bounds__(BLOCK) k_gather(CopyArgs a)
+{
+    using S = typename Prec<P>::S;
+    using T = typename Prec<P>::T;
+    __shared__ double tile[19][BLOCK];
+    const uint64_t base = (uint64_t)blockIdx.x * BLOCK;
+    const uint64_t n = a.n0 + base + threadIdx.x;
+    if (base + threadIdx.x < a.count) {
+        const Nbr b = make_nbr((uint32_t)n, a.g);
+        T f[19];
+        pull_all<P>((const S *)a.L, a.g.N, b, f);
+#pragma unroll
+        for (int i = 0; i < 19; i++)
+            tile[i][threadIdx.x] = (double)f[i];
+    }
+    __syncthreads();
+    const uint64_t nodes = (a.count - base < BLOCK) ? (a.count - base) : BLOCK;
+    double *out = a.aos + base * 19;
+    for (uint32_t e = threadIdx.x; e < nodes * 19; e += BLOCK)
+        out[e] = tile[e % 19][e / 19];
+}
+
+// Upload: reference-layout FP64 rows of 19 (F values) for nodes
+// [n0, n0+count), scattered to the post-collision slots they stream from.
+template <int P>
+__global__ void __launch_bounds__(BLOCK) k_scatter(CopyArgs a)
+{
+    using S = typename Prec<P>::S;
+    using T = typename Prec<P>::T;
+    __shared__ double tile[19][BLOCK];
+    const uint64_t base = (uint64_t)blockIdx.x * BLOCK;
+    const uint64_t nodes = (a.count - base < BLOCK) ? (a.count - base) : BLOCK;
+    const double *in = a.aos + base * 19;
+    for (uint32_t e = threadIdx.x; e < nodes * 19; e += BLOCK)
+        tile[e % 19][e / 19] = in[e];
+    __syncthreads();
+    if (base + threadIdx.x < a.count) {
+        const uint64_t n = a.n0 + base + threadIdx.x;
+        const Nbr b = make_nbr((uint32_t)n, a.g);
+        T f[19];
+#pragma unroll
+        for (int i = 0; i < 19; i++)
+            f[i] = (T)tile[i][threadIdx.x];
+        scatter_all<P>((S *)a.L, a.g.N, b, f);
+    }
+}
+
+// initpop, Channel-Flow/initial.f90:26-44, with rho = 0: F = equilibrium of
+// the velocity arrays, scattered like an upload; rho array set to 0.
+template <int P>
+__global__ void __launch_bounds__(BLOCK) k_initpop(InitArgs a)
+{
+    using S = typename Prec<P>::S;
+    using T = typename Prec<P>::T;
+    const uint32_t n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= a.g.N)
+        return;
+    const T ux = (T)a.ux[n], uy = (T)a.uy[n], uz = (T)a.uz[n];
+    const T rho = T(0.0);
+    T usqr = ux * ux + uy * uy + uz * uz;
+    usqr = T(1.5) * usqr;
+    T f[19];
+    f[0] = T(1.0 / 3.0) * (rho - usqr);
+#define D3Q19_EQ(I)                                                                     \
+    {                                                                                   \
+        constexpr T W = (I) <= 6 ? T(1.0 / 18.0) : T(1.0 / 36.0);                       \
+        const T G = cdot<(I)>(ux, uy, uz);                                              \
+        f[I] = W * (rho + T(3.0) * G + T(4.5) * G * G - usqr);                          \
+    }
+    D3Q19_EQ(1) D3Q19_EQ(2) D3Q19_EQ(3) D3Q19_EQ(4) D3Q19_EQ(5) D3Q19_EQ(6) D3Q19_EQ(7) D3Q19_EQ(8)
+    D3Q19_EQ(9) D3Q19_EQ(10) D3Q19_EQ(11) D3Q19_EQ(12) D3Q19_EQ(13) D3Q19_EQ(14) D3Q19_EQ(15)
+    D3Q19_EQ(16) D3Q19_EQ(17) D3Q19_EQ(18)
+#undef D3Q19_EQ
+    a.rho[n] = 0.0;
+    const Nbr b = make_nbr(n, a.g);
+    scatter_all<P>((S *)a.L, a.g.N, b, f);
+}
+
+// Deterministic block sum: each thread's partial, reduced in a fixed tree.
+__device__ __forceinline__ double block_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    __shared__ double ws[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0)
+        ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        s = ws[0];
+        for (int w = 1; w < BLOCK / 64; w++)
+            s += ws[w];
+    }
+    return s;
+}
+
+// Per-block partial sums of rho over nodes.  SRC 0: the rho array;
+// SRC 1: rho = macrovar(F) computed from lattice L.  Grid-stride, fixed
+// assignment of nodes to threads, so the result is reproducible.
+template <int P, int SRC>
+__global__ void __launch_bounds__(BLOCK) k_rho_partials(SumArgs a)
+{
+    using S = typename Prec<P>::S;
+    using T = typename Prec<P>::T;
+    double acc = 0.0;
+    for (uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; n < a.g.N; n += (uint64_t)gridDim.x * BLOCK) {
+        if constexpr (SRC == 0) {
+            acc += a.rho[n];
+        } else {
+            const Nbr b = make_nbr((uint32_t)n, a.g);
+            T f[19];
+            pull_all<P>((const S *)a.L, a.g.N, b, f);
+            T rho9, ux9, uy9, uz9;
+            macro_sums(f, rho9, ux9, uy9, uz9);
+            acc += (double)rho9;
+        }
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0)
+        a.partials[blockIdx.x] = s;
+}
+
+// Per-block partial sums of every stored population (total mass).
+template <int P, int... Is>
+__device__ __forceinline__ double node_mass(const typename Prec<P>::S *__restrict__ L, uint64_t N, uint64_t n,
+                                            std::integer_sequence<int, Is...>)
+{
+    double m = 0.0;
+    ((m += (double)Prec<P>::template to_T<Is>(L[(uint64_t)Is * N + n])), ...);
+    return m;
+}
+
+template <int P>
+__global__ void __launch_bounds__(BLOCK) k_mass_partials(SumArgs a)
+{
+    using S = typename Prec<P>::S;
+    double acc = 0.0;
+    for (uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; n < a.g.N; n += (uint64_t)gridDim.x * BLOCK)
+        acc += node_mass<P>((const S *)a.L, a.g.N, n, std::make_integer_sequence<int, 19>{});
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0)
+        a.partials[blockIdx.x] = s;
+}
+
+// Plane sums for the statistics of Channel-Flow/saveload.f90:1382-1409:
+// per wall-normal index ix, the sums over (iy, iz) of
+// ux, uy, uz, ux*uz, ux*uy, uy*uz, ux^2, uy^2, uz^2, rho, rho^2.
+// Block (blockIdx.x, blockIdx.y) handles ix = blockIdx.y*blockDim.x + tid and
+// rows blockIdx.x, blockIdx.x + gridDim.x, ...; partials[(c*11 + q)*lx + ix].
+// SRC 0: field arrays; SRC 1: macrovar(F) of lattice L with rho - rho_off.
+template <int P, int SRC>
+__global__ void __launch_bounds__(BLOCK) k_plane_partials(PlaneArgs a)
+{
+    using S = typename Prec<P>::S;
+    using T = typename Prec<P>::T;
+    const uint32_t ix = blockIdx.y * blockDim.x + threadIdx.x;
+    if (ix >= a.g.lx)
+        return;
+    double s[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t rows = a.g.ly * a.g.lz;
+    for (uint32_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const uint32_t n = row * a.g.lx + ix;
+        double rho, ux, uy, uz;
+        if constexpr (SRC == 0) {
+            rho = a.rho[n];
+            ux = a.ux[n];
+            uy = a.uy[n];
+            uz = a.uz[n];
+        } else {
+            const Nbr b = make_nbr(n, a.g);
+            T f[19];
+            pull_all<P>((const S *)a.L, a.g.N, b, f);
+            T rho9, ux9, uy9, uz9;
+            macro_sums(f, rho9, ux9, uy9, uz9);
+            ux = (double)(ux9 + (T)a.fx / T(2.));
+            uy = (double)(uy9 + (T)a.fy / T(2.));
+            uz = (double)(uz9 + (T)a.fz / T(2.));
+            rho = (double)(rho9 - (T)a.rho_off);
+        }
+        s[0] += ux;
+        s[1] += uy;
+        s[2] += uz;
+        s[3] += ux * uz;
+        s[4] += ux * uy;
+        s[5] += uy * uz;
+        s[6] += ux * ux;
+        s[7] += uy * uy;
+        s[8] += uz * uz;
+        s[9] += rho;
+        s[10] += rho * rho;
+    }
+    double *out = a.partials + (uint64_t)blockIdx.x * 11 * a.g.lx;
+#pragma unroll
+    for (int q = 0; q < 11; q++)
+        out[(uint64_t)q * a.g.lx + ix] = s[q];
+}
+
+// ---------------------------------------------------------------------------
+template <int P>
+static inline uint32_t nblocks(uint64_t n)
+{
+    return (uint32_t)((n + BLOCK - 1) / BLOCK);
+}
+
+template <int P>
+hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
+{
+    const dim3 grid(nblocks<P>(a.g.N)), blk(BLOCK);
+    switch (mode) {
+    case 0:
+        hipLaunchKernelGGL((k_step<P, 0>), grid, blk, 0, s, a);
+        break;
+    case 1:
+        hipLaunchKernelGGL((k_step<P, 1>), grid, blk, 0, s, a);
+        break;
+    default:
+        hipLaunchKernelGGL((k_step<P, 2>), grid, blk, 0, s, a);
+        break;
+    }
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t Launch<P>::macro(const MacroArgs &a, int what, hipStream_t s)
+{
+    const dim3 grid(nblocks<P>(a.g.N)), blk(BLOCK);
+    if (what == 0)
+        hipLaunchKernelGGL((k_macro<P, 0>), grid, blk, 0, s, a);
+    else
+        hipLaunchKernelGGL((k_macro<P, 1>), grid, blk, 0, s, a);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t Launch<P>::gather(const CopyArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_gather<P>), dim3(nblocks<P>(a.count)), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t Launch<P>::scatter(const CopyArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_scatter<P>), dim3(nblocks<P>(a.count)), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t Launch<P>::initpop(const InitArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_initpop<P>), dim3(nblocks<P>(a.g.N)), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t Launch<P>::rho_partials(const SumArgs &a, int src, uint32_t grid, hipStream_t s)
+{
+    if (src == 0)
+        hipLaunchKernelGGL((k_rho_partials<P, 0>), dim3(grid), dim3(BLOCK), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_rho_partials<P, 1>), dim3(grid), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t Launch<P>::mass_partials(const SumArgs &a, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_mass_partials<P>), dim3(grid), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t Launch<P>::plane_partials(const PlaneArgs &a, int src, uint32_t chunks, hipStream_t s)
+{
+    const uint32_t bx = a.g.lx >= BLOCK ? BLOCK : ((a.g.lx + 63) / 64) * 64;
+    const dim3 grid(chunks, (a.g.lx + bx - 1) / bx), blk(bx);
+    if (src == 0)
+        hipLaunchKernelGGL((k_plane_partials<P, 0>), grid, blk, 0, s, a);
+    else
+        hipLaunchKernelGGL((k_plane_partials<P, 1>), grid, blk, 0, s, a);
+    return hipGetLastError();
+}
+
+} // namespace d3q19

@@ -1,0 +1,267 @@
+"""MI355X D3Q19 MRT channel-flow core: Python host over liblbm's C ABI (include/lbm.h).
+
+`Channel` mirrors the reference's channel solver one rank at a time: its
+methods carry the names of the Fortran subroutines they replace
+(Channel-Flow/collision.f90: collision_MRT, macrovar, rhoupdat, avedensity,
+FORCING; Channel-Flow/initial.f90: initvel, initpop) and act on a lattice
+that lives on the GPU.  Host arrays use the reference's layout: f(0:18,lx,ly,lz)
+held as a C-ordered numpy array of shape (lz, ly, lx, 19), fields as
+(lz, ly, lx).
+
+There is no CPU fallback: importing this package requires the built
+liblbm.so, and every call runs on the GPU.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .params import para, loglaw_profile, CIX, CIY, CIZ, IPOPP  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblbm.so")
+
+FP64, FP32, FP16 = 0, 1, 2
+_PREC = {"fp64": FP64, "f64": FP64, "fp32": FP32, "f32": FP32, "fp16": FP16, "f16": FP16}
+STORAGE_BYTES = {FP64: 8, FP32: 4, FP16: 2}
+
+
+class LbmError(RuntimeError):
+    """A liblbm call returned an error status."""
+
+    def __init__(self, code, func, msg):
+        super().__init__(f"{func} failed (status {code}): {msg}")
+        self.code = code
+
+
+class LbmConfig(ctypes.Structure):
+    """Mirror of struct lbm_config in include/lbm.h."""
+    _fields_ = [(n, ctypes.c_int) for n in ("lx", "ly", "lz", "precision", "device", "reserved")] + [
+        (n, ctypes.c_double) for n in (
+            "s1", "s2", "s4", "s9", "s10", "s13", "s16",
+            "omegepsl", "omegepslj", "omegxx",
+            "coef1", "coef2", "coef3", "coef4", "coef5", "coef3i", "coef4i",
+            "val8",
+            "val1i", "val2i", "val3i", "val4i", "val5i", "val6i", "val7i", "val8i", "val9i",
+            "ww0", "ww1", "ww2")]
+
+
+# Every function the C ABI exports, with its argument types.
+_D = ctypes.c_void_p
+_H = ctypes.c_void_p
+_SIGS = {
+    "lbm_create": [ctypes.POINTER(LbmConfig), ctypes.POINTER(ctypes.c_void_p)],
+    "lbm_destroy": [_H],
+    "lbm_upload_f": [_H, _D],
+    "lbm_download_f": [_H, _D],
+    "lbm_set_force": [_H, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+    "lbm_set_macros": [_H, _D, _D, _D, _D],
+    "lbm_initpop": [_H],
+    "lbm_collision_mrt": [_H],
+    "lbm_macrovar": [_H, _D, _D, _D, _D],
+    "lbm_rhoupdat": [_H, ctypes.POINTER(ctypes.c_double)],
+    "lbm_avedensity": [_H, ctypes.POINTER(ctypes.c_double)],
+    "lbm_run": [_H, ctypes.c_int],
+    "lbm_mass_sum": [_H, ctypes.POINTER(ctypes.c_double)],
+    "lbm_statistc2": [_H, ctypes.c_double, ctypes.c_double, _D],
+    "lbm_stats_accumulate": [_H],
+    "lbm_stats_read": [_H, ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_longlong)],
+    "lbm_stats_reset": [_H],
+    "lbm_sync": [_H],
+    "lbm_time_steps": [_H, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+    "lbm_device_bytes": [_H, ctypes.POINTER(ctypes.c_size_t)],
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load liblbm.so; raise if it has not been built (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"liblbm.so not found at {path}: build it first "
+                          "(python -c 'import __graft_entry__ as g; g.build()' or make -C d3q19-single-phase_amd)")
+    L = ctypes.CDLL(path)
+    for name, argtypes in _SIGS.items():
+        fn = getattr(L, name)
+        fn.argtypes = argtypes
+        fn.restype = ctypes.c_int
+    L.lbm_last_error.argtypes = []
+    L.lbm_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    return sorted(list(_SIGS) + ["lbm_last_error"])
+
+
+def _call(name, *args):
+    L = load_library()
+    rc = getattr(L, name)(*args)
+    if rc != 0:
+        raise LbmError(rc, name, L.lbm_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous, "arrays must be C-contiguous float64"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Channel:
+    """One rank's channel block on one GPU (reference axes: x wall-normal, y streamwise, z spanwise)."""
+
+    def __init__(self, lx, ly, lz, precision="fp64", laminar=False, mrttype=None, device=0, params=None,
+                 visc=None, rstar=None):
+        self.lx, self.ly, self.lz = int(lx), int(ly), int(lz)
+        self.N = self.lx * self.ly * self.lz
+        self.prec = _PREC[precision] if isinstance(precision, str) else int(precision)
+        self.p = dict(params) if params is not None else para(lx, laminar, mrttype, visc=visc, rstar=rstar)
+        cfg = LbmConfig()
+        cfg.lx, cfg.ly, cfg.lz = self.lx, self.ly, self.lz
+        cfg.precision, cfg.device, cfg.reserved = self.prec, int(device), 0
+        for name, _ in LbmConfig._fields_[6:]:
+            setattr(cfg, name, float(self.p[name]))
+        self._cfg = cfg
+        h = ctypes.c_void_p()
+        _call("lbm_create", ctypes.byref(cfg), ctypes.byref(h))
+        self._h = h
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            _call("lbm_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- reference subroutines --------------------------------------------------
+    def FORCING(self):
+        """Channel-Flow/collision.f90:515-527: uniform force (0, force_in_y*force_mag, 0)."""
+        self.set_force(0.0, self.p["force_in_y"] * self.p["force_mag"], 0.0)
+
+    def set_force(self, fx, fy, fz):
+        _call("lbm_set_force", self._h, float(fx), float(fy), float(fz))
+
+    def initvel(self):
+        """Channel-Flow/initial.f90:94-115: log-law streamwise profile, u_x = u_z = 0."""
+        prof = np.asarray(loglaw_profile(self.lx, self.p["ustar"], self.p["ystar"]))
+        uy = np.ascontiguousarray(np.broadcast_to(prof, (self.lz, self.ly, self.lx)))
+        zero = np.zeros((self.lz, self.ly, self.lx))
+        self.set_macros(ux=zero, uy=uy, uz=zero)
+
+    def set_macros(self, rho=None, ux=None, uy=None, uz=None):
+        """Assign the macroscopic field arrays the next collision reads."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (rho, ux, uy, uz)]
+        for a in arrs:
+            if a is not None and a.size != self.N:
+                raise ValueError(f"field of {a.size} values for {self.N} nodes")
+        _call("lbm_set_macros", self._h, *[_ptr(a) for a in arrs])
+
+    def initpop(self):
+        """Channel-Flow/initial.f90:19-46: rho = 0, f = equilibrium of the current u."""
+        _call("lbm_initpop", self._h)
+
+    def collision_MRT(self):
+        """Channel-Flow/collision.f90:24-273 (with collisionExchnge as the periodic wrap)."""
+        _call("lbm_collision_mrt", self._h)
+
+    def macrovar(self, download=False):
+        """Channel-Flow/collision.f90:378-463.  With download=True returns (rho, ux, uy, uz)."""
+        if not download:
+            _call("lbm_macrovar", self._h, None, None, None, None)
+            return None
+        out = [np.empty((self.lz, self.ly, self.lx)) for _ in range(4)]
+        _call("lbm_macrovar", self._h, *[_ptr(a) for a in out])
+        return tuple(out)
+
+    def rhoupdat(self):
+        """Channel-Flow/collision.f90:469-480; returns max|rho_new - rho_old| (main.f90:79)."""
+        v = ctypes.c_double()
+        _call("lbm_rhoupdat", self._h, ctypes.byref(v))
+        return v.value
+
+    def avedensity(self):
+        """Channel-Flow/collision.f90:487-513; returns the subtracted mean."""
+        v = ctypes.c_double()
+        _call("lbm_avedensity", self._h, ctypes.byref(v))
+        return v.value
+
+    # -- data movement ---------------------------------------------------------
+    def upload_f(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        if f.size != 19 * self.N:
+            raise ValueError(f"f has {f.size} values, expected 19*{self.N}")
+        _call("lbm_upload_f", self._h, _ptr(f))
+
+    def download_f(self):
+        f = np.empty((self.lz, self.ly, self.lx, 19))
+        _call("lbm_download_f", self._h, _ptr(f))
+        return f
+
+    # -- driver helpers ----------------------------------------------------------
+    def run(self, nsteps):
+        """nsteps of the main loop body (main.f90:157-161: collision_MRT, macrovar)."""
+        _call("lbm_run", self._h, int(nsteps))
+
+    def prerelax(self, rhoepsl=1.0e-5, maxiter=15000):
+        """Pre-relaxation loop of Channel-Flow/main.f90:70-90 (u held fixed).
+        Returns (iterations, last max|drho|)."""
+        istep = 0
+        while True:
+            err = self.rhoupdat()
+            self.collision_MRT()
+            if err <= rhoepsl or istep > maxiter:
+                return istep, err
+            istep += 1
+
+    def mass(self):
+        v = ctypes.c_double()
+        _call("lbm_mass_sum", self._h, ctypes.byref(v))
+        return v.value
+
+    def statistc2(self):
+        """Plane profiles of Channel-Flow/saveload.f90:1348-1502; array (lx, 15)."""
+        out = np.empty((self.lx, 15))
+        _call("lbm_statistc2", self._h, self.p["ustar"], self.p["ystar"], _ptr(out))
+        return out
+
+    def stats_accumulate(self):
+        _call("lbm_stats_accumulate", self._h)
+
+    def stats_read(self):
+        out = np.empty((self.lx, 15))
+        n = ctypes.c_longlong()
+        _call("lbm_stats_read", self._h, self.p["ustar"], self.p["ystar"], _ptr(out), ctypes.byref(n))
+        return out, n.value
+
+    def stats_reset(self):
+        _call("lbm_stats_reset", self._h)
+
+    def sync(self):
+        _call("lbm_sync", self._h)
+
+    def time_steps(self, nsteps):
+        """Run nsteps, returning (mean kernel ms, total event ms) from HIP events."""
+        a, t = ctypes.c_double(), ctypes.c_double()
+        _call("lbm_time_steps", self._h, int(nsteps), ctypes.byref(a), ctypes.byref(t))
+        return a.value, t.value
+
+    def device_bytes(self):
+        v = ctypes.c_size_t()
+        _call("lbm_device_bytes", self._h, ctypes.byref(v))
+        return v.value

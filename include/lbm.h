@@ -1,0 +1,154 @@
+/*
+ * liblbm -- MI355X (gfx950) D3Q19 MRT channel-flow core, C ABI.
+ *
+ * Drop-in boundary for the Channel-Flow hot path of UDel-CFD/D3Q19-Single-Phase.
+ * The reference's entry points are argument-free Fortran subroutines acting
+ * on module globals (Channel-Flow/var_inc.f90).  Here the lattice lives on the
+ * GPU inside an opaque handle; every reference subroutine on the path maps to
+ * one function below that takes the handle plus plain pointers and sizes.
+ * A Fortran ISO_C_BINDING shim keeps the original subroutine names.
+ *
+ * Axes follow the reference: x = wall-normal (lx nodes, halfway bounce-back
+ * walls beyond both ends), y = streamwise (ly, periodic), z = spanwise (lz,
+ * periodic).  Host arrays use the reference layout: f(0:18,lx,ly,lz) and
+ * macroscopic fields (lx,ly,lz), Fortran column-major.
+ *
+ * Every function returns 0 (LBM_OK) on success or a positive LBM_ERR_* code;
+ * lbm_last_error() then describes the failure.  Nothing aborts inside the
+ * library.  One host thread per handle; calls on a handle are enqueued on the
+ * handle's own HIP stream, and functions that return host data synchronise.
+ */
+#ifndef LBM_H
+#define LBM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_ERR_ARG 1     /* invalid argument or configuration */
+#define LBM_ERR_HIP 2     /* HIP runtime error (message names the call) */
+#define LBM_ERR_NOMEM 3   /* device or host allocation failed */
+#define LBM_ERR_STATE 4   /* call not valid in the handle's current state */
+
+/* Storage precision of the populations.  The reference's f is already the
+ * shifted population f - w_i*rho0 (Channel-Flow/initial.f90:29-44: rho holds
+ * the density fluctuation only); FP32/FP16 keep that same quantity at lower
+ * precision.  LBM_FP64 computes in FP64; LBM_FP32 and LBM_FP16 compute in FP32
+ * (FP16 is storage only, with per-direction power-of-two scaling). */
+#define LBM_FP64 0
+#define LBM_FP32 1
+#define LBM_FP16 2
+
+/* Run-time constants.  Field names and meanings follow
+ * Channel-Flow/var_inc.f90:85-93 as filled by Channel-Flow/para.f90:106-174;
+ * the driver computes them exactly as para.f90 does and passes them here, so
+ * nothing is recomputed differently.  The structural constants (coef*, val*,
+ * ww*) must equal para.f90's values: lbm_create rejects others. */
+typedef struct lbm_config {
+    int lx, ly, lz;     /* node counts: var_inc.f90:51-52 (lx), para.f90:234-245 (ly, lz) */
+    int precision;      /* LBM_FP64 / LBM_FP32 / LBM_FP16 */
+    int device;         /* HIP device ordinal */
+    int reserved;       /* must be 0 */
+    double s1, s2, s4, s9, s10, s13, s16;             /* para.f90:106-141 */
+    double omegepsl, omegepslj, omegxx;               /* para.f90:118-140 */
+    double coef1, coef2, coef3, coef4, coef5, coef3i, coef4i; /* para.f90:143-150 */
+    double val8;                                      /* para.f90:159 */
+    double val1i, val2i, val3i, val4i, val5i, val6i, val7i, val8i, val9i; /* para.f90:162-170 */
+    double ww0, ww1, ww2;                             /* para.f90:172-174 */
+} lbm_config;
+
+typedef struct lbm_channel lbm_channel;
+
+/* Message for the last failing call on this host thread ("" if none). */
+const char *lbm_last_error(void);
+
+/* Allocates the device lattices for one rank's (lx,ly,lz) block -- replaces
+ * Channel-Flow/para.f90:418-447 (allocarray) for f, rho, ux, uy, uz, force.
+ * The populations start at zero; the macroscopic fields read as macrovar of
+ * the lattice until a call sets them. */
+int lbm_create(const lbm_config *cfg, lbm_channel **out);
+int lbm_destroy(lbm_channel *h);
+
+/* Host <-> device copies of f(0:18,lx,ly,lz) in FP64, converting to and from
+ * the storage precision.  Upload replaces Channel-Flow/saveload.f90:152-156
+ * (loadinitflow) and :327-330 (loadcntdflow) and initial.f90:34-44 (initpop
+ * from host data); download feeds saveinitflow/savecntdflow
+ * (saveload.f90:119-120, :226-227). */
+int lbm_upload_f(lbm_channel *h, const double *f);
+int lbm_download_f(lbm_channel *h, double *f);
+
+/* FORCING, Channel-Flow/collision.f90:515-527: a uniform body force
+ * (force_realx, force_realy, force_realz). */
+int lbm_set_force(lbm_channel *h, double fx, double fy, double fz);
+
+/* Sets the macroscopic fields the next collision reads, as the driver does by
+ * assigning the module arrays (initial.f90:94-115 initvel, :32 rho = 0).
+ * Any pointer may be NULL to keep that field. */
+int lbm_set_macros(lbm_channel *h, const double *rho, const double *ux, const double *uy,
+                   const double *uz);
+
+/* initpop, Channel-Flow/initial.f90:19-46: rho = 0 and f = equilibrium of
+ * (rho = 0, u) from the current velocity fields, on the device. */
+int lbm_initpop(lbm_channel *h);
+
+/* collision_MRT, Channel-Flow/collision.f90:24-273, including
+ * collisionExchnge (:281-372), which on one GPU is the periodic y/z wrap.
+ * Reads the macroscopic fields as the reference does: the values left by the
+ * last macrovar / rhoupdat / avedensity / lbm_set_macros. */
+int lbm_collision_mrt(lbm_channel *h);
+
+/* macrovar, Channel-Flow/collision.f90:378-463: rho, u of the current lattice
+ * (u includes F/2).  Pointers may be NULL: the fields then stay on the device
+ * (computed lazily, fused into the next collision). */
+int lbm_macrovar(lbm_channel *h, double *rho, double *ux, double *uy, double *uz);
+
+/* rhoupdat, Channel-Flow/collision.f90:469-480, plus the convergence measure
+ * of the pre-relaxation loop, Channel-Flow/main.f90:71-81:
+ * *rhoerrmax = max |rho_new - rho_old| (may be NULL). */
+int lbm_rhoupdat(lbm_channel *h, double *rhoerrmax);
+
+/* avedensity, Channel-Flow/collision.f90:487-513: subtracts the mean density
+ * fluctuation (FP64 reduction) from rho; *rhomean receives it (may be NULL). */
+int lbm_avedensity(lbm_channel *h, double *rhomean);
+
+/* nsteps iterations of the main loop body, Channel-Flow/main.f90:157-161
+ * (collision_MRT then macrovar), without host I/O. */
+int lbm_run(lbm_channel *h, int nsteps);
+
+/* Sum of all populations in FP64 (mass monitor; the sum is invariant under
+ * streaming).  Synchronises. */
+int lbm_mass_sum(lbm_channel *h, double *mass);
+
+/* Plane statistics of Channel-Flow/saveload.f90:1382-1470 (statistc2) for the
+ * current macroscopic fields: out is (15, lx) column-major, the 15 columns
+ * written at saveload.f90:1492.  ustar and ystar are the driver's values
+ * (para.f90:64-66). */
+int lbm_statistc2(lbm_channel *h, double ustar, double ystar, double *out);
+
+/* Time-averaged statistics: lbm_stats_accumulate adds the current plane sums
+ * of u_i, u_i u_j and rho, rho^2 (FP64) to running device sums;
+ * lbm_stats_read returns the statistc2 columns of the running average
+ * (fluctuations about the time-and-plane mean) and the sample count;
+ * lbm_stats_reset clears them. */
+int lbm_stats_accumulate(lbm_channel *h);
+int lbm_stats_read(lbm_channel *h, double ustar, double ystar, double *out, long long *nsamples);
+int lbm_stats_reset(lbm_channel *h);
+
+/* Blocks until all work on the handle's stream has finished. */
+int lbm_sync(lbm_channel *h);
+
+/* Measurement: runs nsteps of lbm_run, timing each step kernel with HIP
+ * events on the handle's stream; *avg_ms is the mean kernel duration and
+ * *total_ms the event time from first launch to last completion. */
+int lbm_time_steps(lbm_channel *h, int nsteps, double *avg_ms, double *total_ms);
+
+/* Device bytes held by the handle (lattices, fields, scratch). */
+int lbm_device_bytes(lbm_channel *h, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_H */

@@ -1,0 +1,189 @@
+"""CPU oracle for the D3Q19 MRT channel-flow hot path (ctypes wrapper).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product (liblbm.so and its
+Python/Fortran hosts) never imports this module.
+
+`OracleChannel` holds the same module state as Channel-Flow/var_inc.f90:107-139
+(f, rho, ux, uy, uz, force_realx/y/z) and exposes the reference subroutines
+under their own names, each calling the C restatement in lbm_oracle.c:
+
+    collision_MRT   Channel-Flow/collision.f90:24-273
+    macrovar        Channel-Flow/collision.f90:378-463
+    rhoupdat        Channel-Flow/collision.f90:469-480
+    avedensity      Channel-Flow/collision.f90:487-513
+    FORCING         Channel-Flow/collision.f90:515-527
+    initpop         Channel-Flow/initial.f90:19-46
+    initvel         Channel-Flow/initial.f90:75-147
+    statistc2       Channel-Flow/saveload.f90:1348-1502
+    diag            Channel-Flow/saveload.f90:1507-1676
+
+Array layout is the reference's: f(0:18,lx,ly,lz) column-major, held here as a
+C-ordered numpy array of shape (lz, ly, lx, 19); macroscopic fields (lz, ly, lx).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblbm_oracle.so")
+
+_DBL = ctypes.c_double
+_INT = ctypes.c_int
+
+
+class OrcParams(ctypes.Structure):
+    """Mirror of orc_params (lbm_oracle.h)."""
+    _fields_ = [(n, _INT) for n in ("lx", "ly", "lz", "nx", "mrttype")] + [
+        (n, _DBL) for n in (
+            "visc", "rstar", "ustar", "force_in_y", "ystar", "force_mag", "tau",
+            "s1", "s2", "s4", "s9", "s10", "s13", "s16",
+            "omegepsl", "omegepslj", "omegxx",
+            "coef1", "coef2", "coef3", "coef4", "coef5", "coef3i", "coef4i",
+            "val8",
+            "val1i", "val2i", "val3i", "val4i", "val5i", "val6i", "val7i", "val8i", "val9i",
+            "ww0", "ww1", "ww2")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_lib = None
+
+
+def build():
+    """Compile liblbm_oracle.so with the committed Makefile."""
+    subprocess.run(["make", "-B", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.POINTER(OrcParams)
+        D = ctypes.c_void_p
+        L.orc_para.argtypes = [P, _INT, _INT, _INT, _INT, _INT]
+        L.orc_collide_node.argtypes = [P, D] + [_DBL] * 7 + [D]
+        L.orc_collision_mrt.argtypes = [P] + [D] * 8
+        L.orc_collision_mrt_ab.argtypes = [P] + [D] * 9 + [_INT]
+        L.orc_macrovar.argtypes = [P] + [D] * 8
+        L.orc_rhoupdat.argtypes = [P, D, D]
+        L.orc_avedensity.argtypes = [P, D]
+        L.orc_avedensity.restype = _DBL
+        L.orc_forcing.argtypes = [P, D, D, D]
+        L.orc_initpop.argtypes = [P] + [D] * 5
+        L.orc_initvel.argtypes = [P, D, D, D]
+        L.orc_statistc2.argtypes = [P] + [D] * 5
+        L.orc_diag.argtypes = [P] + [D] * 5
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def para(lx, ly, lz, laminar=False, mrttype=-1):
+    """Constants of Channel-Flow/para.f90:59-174 for a (lx, ly, lz) grid."""
+    p = OrcParams()
+    lib().orc_para(ctypes.byref(p), lx, ly, lz, 1 if laminar else 0, mrttype)
+    return p
+
+
+def collide_node(p, fin, rho, ux, uy, uz, fx, fy, fz):
+    """Post-collision values f9(ip)+Fbar(ip)/2 of one node (collision.f90:56-220)."""
+    fin = np.ascontiguousarray(fin, dtype=np.float64)
+    out = np.empty(19)
+    lib().orc_collide_node(ctypes.byref(p), _ptr(fin), rho, ux, uy, uz, fx, fy, fz, _ptr(out))
+    return out
+
+
+class OracleChannel:
+    """Module state of the reference channel solver, one MPI rank (periodic y/z)."""
+
+    def __init__(self, lx, ly, lz, laminar=False, mrttype=-1, params=None):
+        self.p = params if params is not None else para(lx, ly, lz, laminar, mrttype)
+        self.lx, self.ly, self.lz = lx, ly, lz
+        shp = (lz, ly, lx)
+        self.f = np.zeros(shp + (19,))
+        self.rho = np.zeros(shp)
+        self.ux = np.zeros(shp)
+        self.uy = np.zeros(shp)
+        self.uz = np.zeros(shp)
+        self.fx = np.zeros(shp)
+        self.fy = np.zeros(shp)
+        self.fz = np.zeros(shp)
+
+    # --- reference subroutines -------------------------------------------
+    def FORCING(self):
+        lib().orc_forcing(ctypes.byref(self.p), _ptr(self.fx), _ptr(self.fy), _ptr(self.fz))
+
+    def initvel(self):
+        lib().orc_initvel(ctypes.byref(self.p), _ptr(self.ux), _ptr(self.uy), _ptr(self.uz))
+
+    def initpop(self):
+        lib().orc_initpop(ctypes.byref(self.p), _ptr(self.f), _ptr(self.rho), _ptr(self.ux),
+                          _ptr(self.uy), _ptr(self.uz))
+
+    def collision_MRT(self):
+        lib().orc_collision_mrt(ctypes.byref(self.p), _ptr(self.f), _ptr(self.rho), _ptr(self.ux),
+                                _ptr(self.uy), _ptr(self.uz), _ptr(self.fx), _ptr(self.fy),
+                                _ptr(self.fz))
+
+    def collision_MRT_ab(self, nthreads=0):
+        """Same step, computed out of place and in parallel (bitwise equal)."""
+        out = np.empty_like(self.f)
+        lib().orc_collision_mrt_ab(ctypes.byref(self.p), _ptr(self.f), _ptr(out), _ptr(self.rho),
+                                   _ptr(self.ux), _ptr(self.uy), _ptr(self.uz), _ptr(self.fx),
+                                   _ptr(self.fy), _ptr(self.fz), nthreads)
+        self.f = out
+
+    def macrovar(self):
+        lib().orc_macrovar(ctypes.byref(self.p), _ptr(self.f), _ptr(self.rho), _ptr(self.ux),
+                           _ptr(self.uy), _ptr(self.uz), _ptr(self.fx), _ptr(self.fy), _ptr(self.fz))
+
+    def rhoupdat(self):
+        lib().orc_rhoupdat(ctypes.byref(self.p), _ptr(self.f), _ptr(self.rho))
+
+    def avedensity(self):
+        return lib().orc_avedensity(ctypes.byref(self.p), _ptr(self.rho))
+
+    def statistc2(self):
+        out = np.empty((self.lx, 15))
+        lib().orc_statistc2(ctypes.byref(self.p), _ptr(self.rho), _ptr(self.ux), _ptr(self.uy),
+                            _ptr(self.uz), _ptr(out))
+        return out
+
+    def diag(self):
+        out = np.empty(13)
+        lib().orc_diag(ctypes.byref(self.p), _ptr(self.rho), _ptr(self.ux), _ptr(self.uy),
+                       _ptr(self.uz), _ptr(out))
+        return out
+
+    # --- driver sequences (Channel-Flow/main.f90) --------------------------
+    def prerelax(self, rhoepsl=1.0e-5, maxiter=15000):
+        """Pre-relaxation loop of main.f90:70-90 with u held at its current value.
+        Returns (iterations, last max|drho|)."""
+        istep = 0
+        while True:
+            rhop = self.rho.copy()
+            self.rhoupdat()
+            self.collision_MRT()
+            rhoerrmax = float(np.max(np.abs(self.rho - rhop)))
+            if rhoerrmax <= rhoepsl or istep > maxiter:
+                return istep, rhoerrmax
+            istep += 1
+
+    def run(self, nsteps, ab=False, nthreads=0):
+        """Main loop body of main.f90:142-167 without I/O: collision_MRT, macrovar."""
+        for _ in range(nsteps):
+            if ab:
+                self.collision_MRT_ab(nthreads)
+            else:
+                self.collision_MRT()
+            self.macrovar()
