@@ -53,7 +53,8 @@ struct Geom {
     uint32_t lx, ly, lz, plane; // plane = lx*ly
     uint64_t N;
     FastDiv dlx, dly;
-    uint32_t remap; // 1: map consecutive node blocks to the same XCD
+    uint32_t remap; // 1: map consecutive node blocks to the same XCD (measured
+                    // slower for the step kernel: off by default)
 };
 
 // Block index -> logical block index so that blocks dispatched to the same
@@ -129,7 +130,8 @@ struct Rates {
     T fx, fy, fz;
 };
 
-// Storage precision traits.  FP16 keeps f_i * 2^k with 2^k close to 1/w_i
+// Storage precision traits.  Stores are non-temporal (the next read of a
+// written value is one full lattice sweep later).  FP16 keeps f_i * 2^k with 2^k close to 1/w_i
 // (4, 16, 32), so all directions use the same part of the half range;
 // scaling by a power of two is exact.
 template <int P>
@@ -140,7 +142,7 @@ struct Prec<0> {
     using S = double;
     using T = double;
     template <int I>
-    static __device__ __forceinline__ T ld(const S *p) { return __builtin_nontemporal_load(p); }
+    static constexpr int scale_class() { return 0; }
     template <int I>
     static __device__ __forceinline__ void st(S *p, T v) { __builtin_nontemporal_store(v, p); }
     template <int I>
@@ -154,7 +156,7 @@ struct Prec<1> {
     using S = float;
     using T = float;
     template <int I>
-    static __device__ __forceinline__ T ld(const S *p) { return __builtin_nontemporal_load(p); }
+    static constexpr int scale_class() { return 0; }
     template <int I>
     static __device__ __forceinline__ void st(S *p, T v) { __builtin_nontemporal_store(v, p); }
     template <int I>
@@ -175,7 +177,7 @@ struct Prec<2> {
     template <int I>
     static __device__ __forceinline__ S to_S(T v) { return (_Float16)(v * fp16_scale<I>()); }
     template <int I>
-    static __device__ __forceinline__ T ld(const S *p) { return to_T<I>(__builtin_nontemporal_load(p)); }
+    static constexpr int scale_class() { return I == 0 ? 0 : (I <= 6 ? 1 : 2); }
     template <int I>
     static __device__ __forceinline__ void st(S *p, T v) { __builtin_nontemporal_store(to_S<I>(v), p); }
 };

@@ -23,21 +23,24 @@ namespace d3q19 {
 
 constexpr int BLOCK = 256;
 
-// F_I(x): streamed population of direction I at node x.
+// F_I(x): streamed population of direction I at node x.  The bounce-back
+// choice is made with masks, not branches, so every load issues back to back.
+// Loads are plain (cached): the +-x shifted reads of neighbouring waves share
+// cache lines.
 template <int P, int I>
 __device__ __forceinline__ typename Prec<P>::T pull(const typename Prec<P>::S *__restrict__ L, uint64_t N,
                                                     const Nbr &b)
 {
-    using S = typename Prec<P>::S;
     if constexpr (CX[I] != 0) {
-        // bounce-back reads the node's own opposite slot (stored with the
-        // opposite direction's scale)
-        const bool w = walled<I>(b);
-        const S *p = w ? (L + (uint64_t)OPP[I] * N + b.n) : (L + (uint64_t)I * N + src_of<I>(b));
-        const S v = __builtin_nontemporal_load(p);
-        return w ? Prec<P>::template to_T<OPP[I]>(v) : Prec<P>::template to_T<I>(v);
+        // a walled pull reads the node's own opposite slot (stored with the
+        // opposite direction's scale, which equals this direction's)
+        static_assert(Prec<P>::template scale_class<I>() == Prec<P>::template scale_class<OPP[I]>());
+        const uint32_t w = walled<I>(b) ? 0xffffffffu : 0u;
+        const uint32_t node = (b.n & w) | (src_of<I>(b) & ~w);
+        const uint64_t row = w ? (uint64_t)OPP[I] * N : (uint64_t)I * N;
+        return Prec<P>::template to_T<I>(L[row + node]);
     } else {
-        return Prec<P>::template ld<I>(L + (uint64_t)I * N + src_of<I>(b));
+        return Prec<P>::template to_T<I>(L[(uint64_t)I * N + src_of<I>(b)]);
     }
 }
 

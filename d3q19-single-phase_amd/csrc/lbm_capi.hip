@@ -383,7 +383,7 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
     h->g.N = N;
     h->g.dlx = make_fastdiv(c.lx);
     h->g.dly = make_fastdiv(c.ly);
-    h->g.remap = 1;
+    h->g.remap = 0;
     h->ms = FROM_CUR;
     h->fresh = true;
 

@@ -1,0 +1,54 @@
+!=============================================================================
+! Hot-path entry points under the reference's names.
+!
+! Channel-Flow/collision.f90 defines argument-free subroutines acting on the
+! module arrays; these keep the names, the absence of arguments and the call
+! order, and forward to liblbm on the device lattice held in channel_vars.
+!   collision_MRT  -> lbm_collision_mrt   (collision.f90:24-273, exchange :281-372)
+!   macrovar       -> lbm_macrovar        (collision.f90:378-463)
+!   rhoupdat       -> lbm_rhoupdat        (collision.f90:469-480)
+!   avedensity     -> lbm_avedensity      (collision.f90:487-513)
+!   FORCING        -> lbm_set_force       (collision.f90:515-527)
+! macrovar leaves the fields on the device; fetch_fields copies them to the
+! host arrays when an I/O routine needs them.
+!=============================================================================
+      subroutine collision_MRT
+      use channel_vars
+      implicit none
+      call lbm_check(lbm_collision_mrt(lbm), 'collision_MRT')
+      end subroutine collision_MRT
+
+      subroutine macrovar
+      use channel_vars
+      implicit none
+      call lbm_check(lbm_macrovar(lbm, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), 'macrovar')
+      end subroutine macrovar
+
+      ! macrovar with the fields copied into the host arrays rho, ux, uy, uz.
+      subroutine fetch_fields
+      use channel_vars
+      implicit none
+      call ensure_host_fields
+      call lbm_check(lbm_macrovar(lbm, c_loc(rho), c_loc(ux), c_loc(uy), c_loc(uz)), 'macrovar')
+      end subroutine fetch_fields
+
+      ! rhoupdat; also leaves rhoerrmax = max|rho_new - rho_old|, the
+      ! convergence measure of the pre-relaxation loop (main.f90:79-81).
+      subroutine rhoupdat
+      use channel_vars
+      implicit none
+      call lbm_check(lbm_rhoupdat(lbm, rhoerrmax), 'rhoupdat')
+      end subroutine rhoupdat
+
+      subroutine avedensity
+      use channel_vars
+      implicit none
+      real(8) :: rhomean
+      call lbm_check(lbm_avedensity(lbm, rhomean), 'avedensity')
+      end subroutine avedensity
+
+      subroutine FORCING
+      use channel_vars
+      implicit none
+      call lbm_check(lbm_set_force(lbm, 0.0d0, force_in_y*force_mag, 0.0d0), 'FORCING')
+      end subroutine FORCING

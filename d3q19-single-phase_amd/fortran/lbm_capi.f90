@@ -1,0 +1,146 @@
+!=============================================================================
+! ISO_C_BINDING interface to liblbm (include/lbm.h).
+!
+! Each interface binds one C entry point; argument order and meaning are those
+! of the header.  Status returns are 0 on success.
+!=============================================================================
+      module lbm_capi
+      use iso_c_binding
+      implicit none
+
+      integer(c_int), parameter :: LBM_OK = 0
+      integer(c_int), parameter :: LBM_FP64 = 0, LBM_FP32 = 1, LBM_FP16 = 2
+
+      ! struct lbm_config
+      type, bind(C) :: lbm_config
+        integer(c_int) :: lx, ly, lz, precision, device, reserved
+        real(c_double) :: s1, s2, s4, s9, s10, s13, s16
+        real(c_double) :: omegepsl, omegepslj, omegxx
+        real(c_double) :: coef1, coef2, coef3, coef4, coef5, coef3i, coef4i
+        real(c_double) :: val8
+        real(c_double) :: val1i, val2i, val3i, val4i, val5i, val6i,      &
+                          val7i, val8i, val9i
+        real(c_double) :: ww0, ww1, ww2
+      end type lbm_config
+
+      interface
+        function lbm_last_error() bind(C, name="lbm_last_error")
+          import :: c_ptr
+          type(c_ptr) :: lbm_last_error
+        end function
+
+        integer(c_int) function lbm_create(cfg, h) bind(C, name="lbm_create")
+          import :: c_int, c_ptr, lbm_config
+          type(lbm_config), intent(in) :: cfg
+          type(c_ptr), intent(out) :: h
+        end function
+
+        integer(c_int) function lbm_destroy(h) bind(C, name="lbm_destroy")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+        end function
+
+        integer(c_int) function lbm_upload_f(h, f) bind(C, name="lbm_upload_f")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double), intent(in) :: f(*)
+        end function
+
+        integer(c_int) function lbm_download_f(h, f) bind(C, name="lbm_download_f")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double), intent(out) :: f(*)
+        end function
+
+        integer(c_int) function lbm_set_force(h, fx, fy, fz) bind(C, name="lbm_set_force")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double), value :: fx, fy, fz
+        end function
+
+        integer(c_int) function lbm_set_macros(h, rho, ux, uy, uz) bind(C, name="lbm_set_macros")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h, rho, ux, uy, uz
+        end function
+
+        integer(c_int) function lbm_initpop(h) bind(C, name="lbm_initpop")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+        end function
+
+        integer(c_int) function lbm_collision_mrt(h) bind(C, name="lbm_collision_mrt")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+        end function
+
+        integer(c_int) function lbm_macrovar(h, rho, ux, uy, uz) bind(C, name="lbm_macrovar")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h, rho, ux, uy, uz
+        end function
+
+        integer(c_int) function lbm_rhoupdat(h, rhoerrmax) bind(C, name="lbm_rhoupdat")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double), intent(out) :: rhoerrmax
+        end function
+
+        integer(c_int) function lbm_avedensity(h, rhomean) bind(C, name="lbm_avedensity")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double), intent(out) :: rhomean
+        end function
+
+        integer(c_int) function lbm_run(h, nsteps) bind(C, name="lbm_run")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+          integer(c_int), value :: nsteps
+        end function
+
+        integer(c_int) function lbm_mass_sum(h, mass) bind(C, name="lbm_mass_sum")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double), intent(out) :: mass
+        end function
+
+        integer(c_int) function lbm_statistc2(h, ustar, ystar, out) bind(C, name="lbm_statistc2")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double), value :: ustar, ystar
+          real(c_double), intent(out) :: out(*)
+        end function
+
+        integer(c_int) function lbm_sync(h) bind(C, name="lbm_sync")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+        end function
+      end interface
+
+      contains
+
+      ! Message of the last failing liblbm call as a Fortran string.
+      function lbm_error_string() result(msg)
+        character(len=512) :: msg
+        type(c_ptr) :: p
+        character(kind=c_char), pointer :: cs(:)
+        integer :: i
+        msg = ' '
+        p = lbm_last_error()
+        if (.not. c_associated(p)) return
+        call c_f_pointer(p, cs, [512])
+        do i = 1, 512
+          if (cs(i) == c_null_char) exit
+          msg(i:i) = cs(i)
+        end do
+      end function lbm_error_string
+
+      ! Stop the run with the library's message when a call failed.
+      subroutine lbm_check(rc, where)
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: where
+        if (rc /= LBM_OK) then
+          write(*,*) 'liblbm error in ', trim(where), ': ', trim(lbm_error_string())
+          error stop 1
+        end if
+      end subroutine lbm_check
+
+      end module lbm_capi

@@ -1,0 +1,88 @@
+!=============================================================================
+! Channel driver: the call sequence of Channel-Flow/main.f90:41-231 on one
+! GPU.  Run control comes from the namelist file named on the command line
+! (default channel.nml), e.g.
+!   &channel lx=32, ly=64, lz=32, nsteps=1000, newinitflow=.false. /
+! The throughput of the main loop is printed as MLUPS (time_max of
+! main.f90:214-231, measured around the same loop).
+!=============================================================================
+      program channel_main
+      use channel_vars
+      implicit none
+      character(len=300) :: nml
+      integer :: u, ios
+      integer(8) :: t0, t1, rate
+      real(8) :: time_max, mass
+
+      nml = 'channel.nml'
+      if (command_argument_count() >= 1) call get_command_argument(1, nml)
+      open(newunit=u, file=trim(nml), status='old', action='read', iostat=ios)
+      if (ios /= 0) then
+        write(*,*) 'cannot open namelist file ', trim(nml)
+        error stop 2
+      end if
+      read(u, nml=channel)
+      close(u)
+
+      call channel_para
+      call channel_alloc
+      istat = 0
+      imovie = 0
+      istep0 = 0
+
+      ! main.f90:52-121
+      if (newrun) then
+        if (newinitflow) then
+          call initvel
+          call FORCING
+          call initpop
+          istep = 0
+          do
+            call rhoupdat
+            call collision_MRT
+            write(*,*) istep, rhoerrmax
+            if (rhoerrmax <= rhoepsl .or. istep > 15000) then
+              write(*,*) 'final relaxation => ', istep, rhoerrmax
+              exit
+            end if
+            istep = istep + 1
+          end do
+          call saveinitflow
+          call macrovar
+        else
+          call loadinitflow
+        end if
+        istep0 = 0
+      else
+        call loadcntdflow
+      end if
+
+      ! main.f90:132-137
+      call FORCING
+      call macrovar
+      call lbm_check(lbm_sync(lbm), 'sync')
+      call system_clock(t0, rate)
+
+      ! main.f90:142-208
+      do istep = istep0 + 1, istep0 + nsteps
+        call collision_MRT
+        call macrovar
+        if (ndiag > 0) then
+          if (mod(istep, ndiag) == 0) call diag
+        end if
+        if (nstat > 0) then
+          if (mod(istep, nstat) == 0) call statistc2
+        end if
+      end do
+      istep = istep0 + nsteps
+
+      call lbm_check(lbm_sync(lbm), 'sync')
+      call system_clock(t1)
+      time_max = dble(t1 - t0)/dble(rate)
+      call lbm_check(lbm_mass_sum(lbm, mass), 'mass')
+      if (savecntd) call savecntdflow
+      write(*,*) 'time_max = ', time_max
+      write(*,'(a,f12.2)') ' MLUPS = ', dble(lx)*dble(ly)*dble(lz)*dble(nsteps)/time_max/1.0d6
+      write(*,*) 'mass = ', mass
+      call lbm_check(lbm_destroy(lbm), 'destroy')
+      end program channel_main

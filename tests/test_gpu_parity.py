@@ -30,8 +30,9 @@ def assert_bitwise(got, want, what):
                              f"max |diff| = {float(np.nanmax(d)):.3e}")
 
 
-def seeded_pair(lx, ly, lz, seed, laminar=False, noise=0.0, prec="fp64", vel=None):
-    """Oracle and device channel holding the same injected f (loadinitflow path)."""
+def seeded_pair(lx, ly, lz, seed, laminar=False, noise=0.0, prec="fp64", vel=None, force=None):
+    """Oracle and device channel holding the same injected f (loadinitflow path).
+    force=None uses FORCING (force_in_y of para.f90 for this lx); a value fixes F_y."""
     ref = O.OracleChannel(lx, ly, lz, laminar=laminar)
     ref.FORCING()
     if vel is None:
@@ -45,6 +46,9 @@ def seeded_pair(lx, ly, lz, seed, laminar=False, noise=0.0, prec="fp64", vel=Non
     ch.upload_f(f0)
     ch.FORCING()
     ref.FORCING()
+    if force is not None:
+        ref.fy[:] = force
+        ch.set_force(0.0, force, 0.0)
     return ref, ch, f0
 
 
@@ -79,13 +83,18 @@ def test_config1_fp64_1000_steps_bitwise():
 
 @pytest.mark.parametrize("shape", [(2, 1, 1), (3, 1, 4), (5, 7, 3), (64, 3, 2), (33, 17, 9), (130, 4, 3)])
 def test_edge_shapes_fp64_bitwise(shape):
+    """Ragged and degenerate blocks (one-node periodic directions, two-node
+    wall-to-wall channels).  The force is fixed at the 32-node channel's value:
+    force_in_y of para.f90 grows as 1/lx^3 and would be unstable at lx = 3."""
     lx, ly, lz = shape
-    ref, ch, _ = seeded_pair(lx, ly, lz, seed=5, noise=2e-4)
+    ref, ch, _ = seeded_pair(lx, ly, lz, seed=5, noise=2e-4, force=1.0252e-4)
+    assert np.isfinite(ref.f).all()
     with ch:
         ref.macrovar()
         ch.macrovar()
         ref.run(40)
         ch.run(40)
+        assert np.isfinite(ref.f).all()
         assert_bitwise(ch.download_f(), ref.f, f"f {shape}")
 
 
