@@ -21,6 +21,7 @@ struct StepArgs {
     const void *src; // P_k
     void *dst;       // P_{k+1}
     const double *rho, *ux, *uy, *uz;
+    const int *solid; // node flags (>= 0: inside a particle) for the particle modes
     double rho_off;
     RatesD r;
 };
@@ -51,16 +52,21 @@ struct SumArgs {
     Geom g;
     const void *L;
     const double *rho;
-    double *partials;
+    const int *solid; // nullable: when set, only fluid nodes (solid < 0) count
+    double *partials; // per block: sum, then count when solid is set
 };
 
 struct PlaneArgs {
     Geom g;
     const void *L;
     const double *rho, *ux, *uy, *uz;
+    const int *solid; // nullable: when set, only fluid nodes count
     double fx, fy, fz, rho_off;
     double *partials;
 };
+
+// Plane sums per wall-normal index: 11 moments and the fluid-node count.
+constexpr int NPLANE = 12;
 
 template <int P>
 struct Launch {

@@ -102,6 +102,9 @@ __device__ __forceinline__ uint32_t node_index(const Geom &g)
 // One time step: gather F_k, macroscopic fields, MRT collision, store P_{k+1}.
 // MODE 0: rho, u from F_k (macrovar fused); MODE 1: same with rho - rho_off
 // (avedensity's correction); MODE 2: rho, u read from the field arrays.
+// MODE + 4: nodes inside a particle are skipped (not collided, not stored);
+// fluid nodes pull the interpolated bounce-back values written into the
+// particle nodes' slots before the step.
 template <int P, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
 {
@@ -110,6 +113,10 @@ __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
     const uint32_t n = node_index(a.g);
     if (n >= a.g.N)
         return;
+    if constexpr ((MODE & 4) != 0) {
+        if (a.solid[n] >= 0)
+            return;
+    }
     const Nbr b = make_nbr(n, a.g);
     const S *__restrict__ src = (const S *)a.src;
     S *__restrict__ dst = (S *)a.dst;
@@ -133,7 +140,7 @@ __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
     r.fz = (T)a.r.fz;
 
     T rho9, ux9, uy9, uz9;
-    if constexpr (MODE == 2) {
+    if constexpr ((MODE & 3) == 2) {
         rho9 = (T)a.rho[n];
         ux9 = (T)a.ux[n];
         uy9 = (T)a.uy[n];
@@ -143,7 +150,7 @@ __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
         ux9 = ux9 + r.fx / T(2.); // collision.f90:415-417
         uy9 = uy9 + r.fy / T(2.);
         uz9 = uz9 + r.fz / T(2.);
-        if constexpr (MODE == 1)
+        if constexpr ((MODE & 3) == 1)
             rho9 = rho9 - (T)a.rho_off; // collision.f90:508
     }
     mrt_collide(f, rho9, ux9, uy9, uz9, r);
@@ -301,8 +308,11 @@ __global__ void __launch_bounds__(BLOCK) k_rho_partials(SumArgs a)
 {
     using S = typename Prec<P>::S;
     using T = typename Prec<P>::T;
-    double acc = 0.0;
+    double acc = 0.0, cnt = 0.0;
     for (uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; n < a.g.N; n += (uint64_t)gridDim.x * BLOCK) {
+        if (a.solid && a.solid[n] >= 0)
+            continue;
+        cnt += 1.0;
         if constexpr (SRC == 0) {
             acc += a.rho[n];
         } else {
@@ -315,8 +325,12 @@ __global__ void __launch_bounds__(BLOCK) k_rho_partials(SumArgs a)
         }
     }
     const double s = block_sum(acc);
-    if (threadIdx.x == 0)
+    __syncthreads();
+    const double c = block_sum(cnt);
+    if (threadIdx.x == 0) {
         a.partials[blockIdx.x] = s;
+        a.partials[gridDim.x + blockIdx.x] = c;
+    }
 }
 
 // Per-block partial sums of every stored population (total mass).
@@ -335,15 +349,16 @@ __global__ void __launch_bounds__(BLOCK) k_mass_partials(SumArgs a)
     using S = typename Prec<P>::S;
     double acc = 0.0;
     for (uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; n < a.g.N; n += (uint64_t)gridDim.x * BLOCK)
-        acc += node_mass<P>((const S *)a.L, a.g.N, n, std::make_integer_sequence<int, 19>{});
+        if (!a.solid || a.solid[n] < 0)
+            acc += node_mass<P>((const S *)a.L, a.g.N, n, std::make_integer_sequence<int, 19>{});
     const double s = block_sum(acc);
     if (threadIdx.x == 0)
         a.partials[blockIdx.x] = s;
 }
 
 // Plane sums for the statistics of Channel-Flow/saveload.f90:1382-1409:
-// per wall-normal index ix, the sums over (iy, iz) of
-// ux, uy, uz, ux*uz, ux*uy, uy*uz, ux^2, uy^2, uz^2, rho, rho^2.
+// per wall-normal index ix, the sums over the fluid nodes (iy, iz) of
+// ux, uy, uz, ux*uz, ux*uy, uy*uz, ux^2, uy^2, uz^2, rho, rho^2, and their count.
 // Block (blockIdx.x, blockIdx.y) handles ix = blockIdx.y*blockDim.x + tid and
 // rows blockIdx.x, blockIdx.x + gridDim.x, ...; partials[(c*11 + q)*lx + ix].
 // SRC 0: field arrays; SRC 1: macrovar(F) of lattice L with rho - rho_off.
@@ -355,10 +370,12 @@ __global__ void __launch_bounds__(BLOCK) k_plane_partials(PlaneArgs a)
     const uint32_t ix = blockIdx.y * blockDim.x + threadIdx.x;
     if (ix >= a.g.lx)
         return;
-    double s[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double s[NPLANE] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const uint32_t rows = a.g.ly * a.g.lz;
     for (uint32_t row = blockIdx.x; row < rows; row += gridDim.x) {
         const uint32_t n = row * a.g.lx + ix;
+        if (a.solid && a.solid[n] >= 0)
+            continue;
         double rho, ux, uy, uz;
         if constexpr (SRC == 0) {
             rho = a.rho[n];
@@ -387,10 +404,11 @@ __global__ void __launch_bounds__(BLOCK) k_plane_partials(PlaneArgs a)
         s[8] += uz * uz;
         s[9] += rho;
         s[10] += rho * rho;
+        s[11] += 1.0;
     }
-    double *out = a.partials + (uint64_t)blockIdx.x * 11 * a.g.lx;
+    double *out = a.partials + (uint64_t)blockIdx.x * NPLANE * a.g.lx;
 #pragma unroll
-    for (int q = 0; q < 11; q++)
+    for (int q = 0; q < NPLANE; q++)
         out[(uint64_t)q * a.g.lx + ix] = s[q];
 }
 
@@ -412,9 +430,20 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
     case 1:
         hipLaunchKernelGGL((k_step<P, 1>), grid, blk, 0, s, a);
         break;
-    default:
+    case 2:
         hipLaunchKernelGGL((k_step<P, 2>), grid, blk, 0, s, a);
         break;
+    case 4:
+        hipLaunchKernelGGL((k_step<P, 4>), grid, blk, 0, s, a);
+        break;
+    case 5:
+        hipLaunchKernelGGL((k_step<P, 5>), grid, blk, 0, s, a);
+        break;
+    case 6:
+        hipLaunchKernelGGL((k_step<P, 6>), grid, blk, 0, s, a);
+        break;
+    default:
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -24,6 +24,7 @@
 
 #include "kernels.h"
 #include "lbm.h"
+#include "particles.h"
 
 using namespace d3q19;
 
@@ -69,13 +70,21 @@ struct lbm_channel {
     double rho_off;
     double force[3];
     hipStream_t stream;
-    double *partials;   // RED_BLOCKS doubles, then PLANE_CHUNKS*11*lx doubles
-    double *red_out;    // 1 double + 11*lx doubles of final sums
+    double *partials;   // 2*RED_BLOCKS doubles, then PLANE_CHUNKS*NPLANE*lx doubles
+    double *red_out;    // 2 doubles + NPLANE*lx doubles of final sums
     unsigned long long *errbits;
-    double *stat_acc;   // 11*lx running sums
+    double *stat_acc;   // NPLANE*lx running sums
     long long stat_n;
     double *stage;      // COPY_NODES*19 doubles
     hipEvent_t ev[2];
+    struct Part *part;  // rigid particles (nullptr without)
+};
+
+// Particle set on the device (see particles.h).
+struct Part {
+    PartGeom pg;
+    PartState ps;
+    std::vector<void *> allocs;
 };
 
 namespace {
@@ -225,6 +234,11 @@ int launch_step(lbm_channel *h)
         mode = 2;
     else
         mode = (h->rho_off != 0.0) ? 1 : 0;
+    a.solid = nullptr;
+    if (h->part) {
+        mode |= 4;
+        a.solid = h->part->ps.solid;
+    }
     int rc = dispatch_err(PREC_CALL(h, step, a, mode, h->stream), "collision");
     if (rc)
         return rc;
@@ -236,22 +250,33 @@ int launch_step(lbm_channel *h)
     return LBM_OK;
 }
 
-int reduce_rho(lbm_channel *h, int src, double *sum)
+const int *solid_of(const lbm_channel *h) { return h->part ? h->part->ps.solid : nullptr; }
+
+// Sum of rho over fluid nodes (SRC 0: field array, SRC 1: macrovar of the
+// current lattice) and the number of fluid nodes.
+int reduce_rho(lbm_channel *h, int src, double *sum, double *count)
 {
     SumArgs a;
     a.g = h->g;
     a.L = h->lat[h->cur];
     a.rho = h->mac[0];
+    a.solid = solid_of(h);
     a.partials = h->partials;
     uint32_t grid = (uint32_t)std::min<uint64_t>(RED_BLOCKS, (h->g.N + 255) / 256);
     int rc = dispatch_err(PREC_CALL(h, rho_partials, a, src, grid, h->stream), "rho reduction");
     if (rc)
         return rc;
+    // partials: sums in [0, grid), counts in [grid, 2*grid)
     hipLaunchKernelGGL(k_sum_columns, dim3(1), dim3(64), 0, h->stream, h->partials, grid, 1u, h->red_out,
                        (double *)nullptr);
+    hipLaunchKernelGGL(k_sum_columns, dim3(1), dim3(64), 0, h->stream, h->partials + grid, grid, 1u, h->red_out + 1,
+                       (double *)nullptr);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(sum, h->red_out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    double out[2];
+    HIP_TRY(hipMemcpyAsync(out, h->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    *sum = out[0];
+    *count = out[1];
     return LBM_OK;
 }
 
@@ -270,29 +295,31 @@ int plane_sums(lbm_channel *h, bool acc)
     a.fy = h->force[1];
     a.fz = h->force[2];
     a.rho_off = h->rho_off;
-    a.partials = h->partials + RED_BLOCKS;
+    a.solid = solid_of(h);
+    a.partials = h->partials + 2 * RED_BLOCKS;
     const uint32_t rows = h->g.ly * h->g.lz;
     const uint32_t chunks = std::min<uint32_t>(PLANE_CHUNKS, rows);
     const int src = (h->ms == ARRAYS) ? 0 : 1;
     int rc = dispatch_err(PREC_CALL(h, plane_partials, a, src, chunks, h->stream), "plane sums");
     if (rc)
         return rc;
-    const uint32_t width = 11 * h->g.lx;
+    const uint32_t width = NPLANE * h->g.lx;
     hipLaunchKernelGGL(k_sum_columns, dim3((width + 255) / 256), dim3(256), 0, h->stream, a.partials, chunks, width,
-                       h->red_out + 1, acc ? h->stat_acc : (double *)nullptr);
+                       h->red_out + 2, acc ? h->stat_acc : (double *)nullptr);
     HIP_TRY(hipGetLastError());
     return LBM_OK;
 }
 
-// Columns of saveload.f90:1430-1492 from plane sums s[q*lx + i] over nf
-// samples per plane.
-void stat_columns(const lbm_channel *h, const double *s, double nf, double ustar, double ystar, double *out)
+// Columns of saveload.f90:1430-1492 from plane sums s[q*lx + i]; the
+// fluid-node count of plane i is s[11*lx + i] (per sample), nsamp the number
+// of accumulated samples.
+void stat_columns(const lbm_channel *h, const double *s, double nsamp, double ustar, double ystar, double *out)
 {
     const uint32_t lx = h->g.lx;
     const double us2 = ustar * ustar, us4 = us2 * us2;
-    const double volf = 1.0; // every node is fluid
     for (uint32_t i = 0; i < lx; i++) {
         const double *c = s + i;
+        const double nf = c[11 * lx] > 0.0 ? c[11 * lx] : 1.0;
         double vx = c[0 * lx] / nf / ustar, vy = c[1 * lx] / nf / ustar, vz = c[2 * lx] / nf / ustar;
         double sxz = c[3 * lx] / nf / us2 - vx * vz;
         double sxy = c[4 * lx] / nf / us2 - vx * vy;
@@ -302,6 +329,7 @@ void stat_columns(const lbm_channel *h, const double *s, double nf, double ustar
         double zz = c[8 * lx] / nf / us2 - vz * vz;
         double pr = c[9 * lx] / nf / us2;
         double p2 = c[10 * lx] / nf / us4 - pr * pr;
+        const double volf = c[11 * lx] / nsamp / ((double)h->g.ly * (double)h->g.lz);
         double yplus = ((double)(i + 1) - 0.5) / ystar;
         if (i + 1 > lx / 2)
             yplus = ((double)(lx - (i + 1)) + 0.5) / ystar;
@@ -404,13 +432,13 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         if (e != hipSuccess)
             return cleanup(fail(LBM_ERR_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e)));
     }
-    const size_t part = RED_BLOCKS + (size_t)PLANE_CHUNKS * 11 * c.lx;
+    const size_t part = 2 * RED_BLOCKS + (size_t)PLANE_CHUNKS * NPLANE * c.lx;
     if (hipMalloc((void **)&h->partials, part * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&h->red_out, (1 + 11 * (size_t)c.lx) * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&h->stat_acc, 11 * (size_t)c.lx * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&h->red_out, (2 + NPLANE * (size_t)c.lx) * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&h->stat_acc, NPLANE * (size_t)c.lx * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&h->errbits, sizeof(unsigned long long)) != hipSuccess)
         return cleanup(fail(LBM_ERR_NOMEM, "hipMalloc of reduction scratch failed"));
-    e = hipMemsetAsync(h->stat_acc, 0, 11 * (size_t)c.lx * sizeof(double), h->stream);
+    e = hipMemsetAsync(h->stat_acc, 0, NPLANE * (size_t)c.lx * sizeof(double), h->stream);
     if (e == hipSuccess)
         e = hipEventCreate(&h->ev[0]);
     if (e == hipSuccess)
@@ -446,6 +474,11 @@ int lbm_destroy(lbm_channel *h)
         (void)hipFree(h->errbits);
     if (h->stage)
         (void)hipFree(h->stage);
+    if (h->part) {
+        for (void *p : h->part->allocs)
+            (void)hipFree(p);
+        delete h->part;
+    }
     if (h->ev[0])
         (void)hipEventDestroy(h->ev[0]);
     if (h->ev[1])
@@ -643,24 +676,24 @@ int lbm_rhoupdat(lbm_channel *h, double *rhoerrmax)
 int lbm_avedensity(lbm_channel *h, double *rhomean)
 {
     CHECK_H(h);
-    double sum = 0.0, mean;
+    double sum = 0.0, cnt = 0.0, mean;
     int rc;
     if (h->ms == FROM_CUR && h->rho_off == 0.0) {
         // rho is virtual: reduce macrovar(F) directly and keep the correction
         // as an offset applied by the next collision
-        rc = reduce_rho(h, 1, &sum);
+        rc = reduce_rho(h, 1, &sum, &cnt);
         if (rc)
             return rc;
-        mean = sum / (double)h->g.N;
+        mean = cnt > 0.0 ? sum / cnt : 0.0;
         h->rho_off = mean;
     } else {
         rc = materialize(h);
         if (rc)
             return rc;
-        rc = reduce_rho(h, 0, &sum);
+        rc = reduce_rho(h, 0, &sum, &cnt);
         if (rc)
             return rc;
-        mean = sum / (double)h->g.N;
+        mean = cnt > 0.0 ? sum / cnt : 0.0;
         hipLaunchKernelGGL(k_sub_scalar, dim3((unsigned)((h->g.N + 255) / 256)), dim3(256), 0, h->stream, h->mac[0],
                            h->g.N, mean);
         HIP_TRY(hipGetLastError());
@@ -695,6 +728,7 @@ int lbm_mass_sum(lbm_channel *h, double *mass)
     a.g = h->g;
     a.L = h->lat[h->cur];
     a.rho = nullptr;
+    a.solid = solid_of(h);
     a.partials = h->partials;
     uint32_t grid = (uint32_t)std::min<uint64_t>(RED_BLOCKS, (h->g.N + 255) / 256);
     int rc = dispatch_err(PREC_CALL(h, mass_partials, a, grid, h->stream), "mass reduction");
@@ -716,10 +750,10 @@ int lbm_statistc2(lbm_channel *h, double ustar, double ystar, double *out)
     int rc = plane_sums(h, false);
     if (rc)
         return rc;
-    std::vector<double> s(11 * (size_t)h->g.lx);
-    HIP_TRY(hipMemcpyAsync(s.data(), h->red_out + 1, s.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    std::vector<double> s(NPLANE * (size_t)h->g.lx);
+    HIP_TRY(hipMemcpyAsync(s.data(), h->red_out + 2, s.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    stat_columns(h, s.data(), (double)h->g.ly * (double)h->g.lz, ustar, ystar, out);
+    stat_columns(h, s.data(), 1.0, ustar, ystar, out);
     return LBM_OK;
 }
 
@@ -742,17 +776,17 @@ int lbm_stats_read(lbm_channel *h, double ustar, double ystar, double *out, long
         *nsamples = h->stat_n;
     if (h->stat_n == 0)
         return fail(LBM_ERR_STATE, "lbm_stats_read: no samples accumulated");
-    std::vector<double> s(11 * (size_t)h->g.lx);
+    std::vector<double> s(NPLANE * (size_t)h->g.lx);
     HIP_TRY(hipMemcpyAsync(s.data(), h->stat_acc, s.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    stat_columns(h, s.data(), (double)h->g.ly * (double)h->g.lz * (double)h->stat_n, ustar, ystar, out);
+    stat_columns(h, s.data(), (double)h->stat_n, ustar, ystar, out);
     return LBM_OK;
 }
 
 int lbm_stats_reset(lbm_channel *h)
 {
     CHECK_H(h);
-    const uint64_t n = 11 * (uint64_t)h->g.lx;
+    const uint64_t n = NPLANE * (uint64_t)h->g.lx;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->stat_acc, n, 0.0);
     HIP_TRY(hipGetLastError());
     h->stat_n = 0;
@@ -811,8 +845,279 @@ int lbm_device_bytes(lbm_channel *h, size_t *bytes)
         b += 4 * (size_t)h->g.N * sizeof(double);
     if (h->stage)
         b += COPY_NODES * 19 * sizeof(double);
-    b += (RED_BLOCKS + (size_t)PLANE_CHUNKS * 11 * h->g.lx + 1 + 22 * (size_t)h->g.lx) * sizeof(double) + 8;
+    b += (2 * RED_BLOCKS + (size_t)PLANE_CHUNKS * NPLANE * h->g.lx + 2 + 2 * NPLANE * (size_t)h->g.lx) * sizeof(double) + 8;
     *bytes = b;
+    return LBM_OK;
+}
+
+int lbm_download_post(lbm_channel *h, double *post)
+{
+    CHECK_H(h);
+    if (!post)
+        return fail(LBM_ERR_ARG, "lbm_download_post: null output");
+    const uint64_t N = h->g.N;
+    if (h->prec == LBM_FP64) {
+        HIP_TRY(hipMemcpyAsync(post, h->lat[h->cur], N * 19 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return LBM_OK;
+    }
+    // FP32/FP16: widen on the host, population by population
+    std::vector<unsigned char> buf(N * h->elem);
+    for (int i = 0; i < 19; i++) {
+        HIP_TRY(hipMemcpyAsync(buf.data(), (const unsigned char *)h->lat[h->cur] + (size_t)i * N * h->elem,
+                               N * h->elem, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        double *o = post + (size_t)i * N;
+        if (h->prec == LBM_FP32) {
+            const float *v = (const float *)buf.data();
+            for (uint64_t n = 0; n < N; n++)
+                o[n] = v[n];
+        } else {
+            const _Float16 *v = (const _Float16 *)buf.data();
+            const double sc = i == 0 ? 0.25 : (i <= 6 ? 1.0 / 16.0 : 1.0 / 32.0);
+            for (uint64_t n = 0; n < N; n++)
+                o[n] = (double)(float)v[n] * sc;
+        }
+    }
+    return LBM_OK;
+}
+
+static int part_alloc(Part *pt, void **p, size_t bytes)
+{
+    if (hipMalloc(p, bytes) != hipSuccess)
+        return fail(LBM_ERR_NOMEM, "hipMalloc of particle state (%zu bytes) failed", bytes);
+    pt->allocs.push_back(*p);
+    return LBM_OK;
+}
+
+int lbm_particles_create(lbm_channel *h, const lbm_particle_config *pc, const double *pos, const double *vel,
+                         const double *omg)
+{
+    CHECK_H(h);
+    if (!pc || !pos)
+        return fail(LBM_ERR_ARG, "lbm_particles_create: null argument");
+    if (h->part)
+        return fail(LBM_ERR_STATE, "lbm_particles_create: particles already exist on this handle");
+    if (pc->npart < 1 || !(pc->rad > 0.0) || !(pc->rhopart > 0.0) || pc->reserved != 0)
+        return fail(LBM_ERR_ARG, "lbm_particles_create: need npart >= 1, rad > 0, rhopart > 0, reserved = 0");
+    const double lx = h->g.lx, ly = h->g.ly, lz = h->g.lz;
+    for (int p = 0; p < pc->npart; p++) {
+        const double *X = pos + 3 * p;
+        if (!(X[0] - pc->rad > 0.0 && X[0] + pc->rad < lx) || !(X[1] >= 0.0 && X[1] < ly) ||
+            !(X[2] >= 0.0 && X[2] < lz))
+            return fail(LBM_ERR_ARG,
+                        "lbm_particles_create: particle %d at (%g, %g, %g) must lie inside the channel "
+                        "(R < x < lx - R, 0 <= y < ly, 0 <= z < lz)",
+                        p, X[0], X[1], X[2]);
+    }
+    Part *pt = new Part();
+    PartGeom &pg = pt->pg;
+    PartState &ps = pt->ps;
+    memset(&pg, 0, sizeof(pg));
+    memset(&ps, 0, sizeof(ps));
+    const double pi = 3.14159265358979323846;
+    pg.g = h->g;
+    pg.np = pc->npart;
+    pg.R = pc->rad;
+    pg.R2 = pc->rad * pc->rad;
+    // cut links: about 5 per surface node; capacity with margin
+    const double area = 4.0 * pi * (pc->rad + 1.0) * (pc->rad + 1.0);
+    pg.maxlink = pc->maxlink > 0 ? pc->maxlink : (int)(12.0 * area) + 256;
+    pg.maxfill = (int)(4.0 * area) + 64;
+    pg.volume = 4.0 / 3.0 * pi * pc->rad * pc->rad * pc->rad;
+    pg.mass = pc->rhopart * pg.volume;
+    pg.inertia = 0.4 * pg.mass * pc->rad * pc->rad;
+    pg.mingap = pc->mingap;
+    pg.mingap_w = pc->mingap_w;
+    pg.stf = pc->stf;
+    pg.stf_w = pc->stf_w;
+    pg.pressure_force = pc->pressure_force;
+    pg.fx = h->force[0];
+    pg.fy = h->force[1];
+    pg.fz = h->force[2];
+    const size_t np = pc->npart, nl = np * pg.maxlink, nf = np * pg.maxfill;
+    int rc = LBM_OK;
+    auto A = [&](void **p, size_t bytes) {
+        if (rc == LBM_OK)
+            rc = part_alloc(pt, p, bytes);
+    };
+    A((void **)&ps.X, 6 * np * sizeof(double)); // X, then the previous positions
+    A((void **)&ps.V, 3 * np * sizeof(double));
+    A((void **)&ps.W, 3 * np * sizeof(double));
+    A((void **)&ps.TH, 3 * np * sizeof(double));
+    A((void **)&ps.Aprev, 3 * np * sizeof(double));
+    A((void **)&ps.ALprev, 3 * np * sizeof(double));
+    A((void **)&ps.FH, 3 * np * sizeof(double));
+    A((void **)&ps.TQ, 3 * np * sizeof(double));
+    A((void **)&ps.FC, 3 * np * sizeof(double));
+    A((void **)&ps.nsteps_done, sizeof(int));
+    A((void **)&ps.solid, h->g.N * sizeof(int));
+    A((void **)&ps.link_node, nl * sizeof(int));
+    A((void **)&ps.link_dir, nl);
+    A((void **)&ps.link_q, nl * sizeof(double));
+    A((void **)&ps.link_contrib, nl * 6 * sizeof(double));
+    A((void **)&ps.link_count, np * sizeof(int));
+    A((void **)&ps.fill_node, nf * sizeof(int));
+    A((void **)&ps.fill_count, np * sizeof(int));
+    A((void **)&ps.overflow, sizeof(int));
+    if (rc) {
+        for (void *p : pt->allocs)
+            (void)hipFree(p);
+        delete pt;
+        return rc;
+    }
+    std::vector<double> zero(3 * np, 0.0);
+    HIP_TRY(hipMemcpyAsync(ps.X, pos, 3 * np * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(ps.X + 3 * np, pos, 3 * np * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(ps.V, vel ? vel : zero.data(), 3 * np * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(ps.W, omg ? omg : zero.data(), 3 * np * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    for (double *d : {ps.TH, ps.Aprev, ps.ALprev, ps.FH, ps.TQ, ps.FC})
+        HIP_TRY(hipMemcpyAsync(d, zero.data(), 3 * np * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemsetAsync(ps.nsteps_done, 0, sizeof(int), h->stream));
+    HIP_TRY(hipMemsetAsync(ps.overflow, 0, sizeof(int), h->stream));
+    HIP_TRY(hipMemsetAsync(ps.solid, 0xff, h->g.N * sizeof(int), h->stream)); // -1: fluid
+    HIP_TRY(hipMemsetAsync(ps.fill_count, 0, np * sizeof(int), h->stream));
+    HIP_TRY(hipMemsetAsync(ps.link_count, 0, np * sizeof(int), h->stream));
+    h->part = pt;
+    rc = dispatch_err(part_classify(pg, ps, 1, h->stream), "beads_links");
+    if (rc)
+        return rc;
+    int ovf = 0;
+    HIP_TRY(hipMemcpyAsync(&ovf, ps.overflow, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (ovf)
+        return fail(LBM_ERR_ARG, "lbm_particles_create: cut-link list overflow (maxlink %d)", pg.maxlink);
+    return LBM_OK;
+}
+
+#define CHECK_P(h)                                                                       \
+    do {                                                                                 \
+        CHECK_H(h);                                                                      \
+        if (!(h)->part)                                                                  \
+            return fail(LBM_ERR_STATE, "%s: no particles on this handle", __func__);     \
+    } while (0)
+
+static int check_overflow(lbm_channel *h, const char *what)
+{
+    int ovf = 0;
+    HIP_TRY(hipMemcpyAsync(&ovf, h->part->ps.overflow, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (ovf)
+        return fail(LBM_ERR_STATE, "%s: a per-particle link or refill list overflowed (maxlink %d, maxfill %d)",
+                    what, h->part->pg.maxlink, h->part->pg.maxfill);
+    return LBM_OK;
+}
+
+int lbm_beads_links(lbm_channel *h)
+{
+    CHECK_P(h);
+    return dispatch_err(part_classify(h->part->pg, h->part->ps, 0, h->stream), "beads_links");
+}
+
+int lbm_beads_filling(lbm_channel *h)
+{
+    CHECK_P(h);
+    Part *pt = h->part;
+    hipError_t e = h->prec == LBM_FP64   ? PartLaunch<0>::refill(pt->pg, pt->ps, h->lat[h->cur], h->stream)
+                   : h->prec == LBM_FP32 ? PartLaunch<1>::refill(pt->pg, pt->ps, h->lat[h->cur], h->stream)
+                                         : PartLaunch<2>::refill(pt->pg, pt->ps, h->lat[h->cur], h->stream);
+    return dispatch_err(e, "beads_filling");
+}
+
+int lbm_beads_collision(lbm_channel *h)
+{
+    CHECK_P(h);
+    Part *pt = h->part;
+    hipError_t e = h->prec == LBM_FP64   ? PartLaunch<0>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream)
+                   : h->prec == LBM_FP32 ? PartLaunch<1>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream)
+                                         : PartLaunch<2>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream);
+    int rc = dispatch_err(e, "beads_collision");
+    if (rc)
+        return rc;
+    return dispatch_err(part_sum(pt->pg, pt->ps, h->stream), "beads_collision sum");
+}
+
+int lbm_beads_move(lbm_channel *h)
+{
+    CHECK_P(h);
+    Part *pt = h->part;
+    pt->pg.fx = h->force[0];
+    pt->pg.fy = h->force[1];
+    pt->pg.fz = h->force[2];
+    return dispatch_err(part_move(pt->pg, pt->ps, h->stream), "beads_move");
+}
+
+int lbm_run_particles(lbm_channel *h, int nsteps)
+{
+    CHECK_P(h);
+    if (nsteps < 0)
+        return fail(LBM_ERR_ARG, "lbm_run_particles: nsteps %d < 0", nsteps);
+    for (int s = 0; s < nsteps; s++) {
+        int rc = lbm_beads_collision(h);
+        if (!rc)
+            rc = launch_step(h);
+        if (rc)
+            return rc;
+        h->ms = FROM_CUR; // macrovar
+        h->rho_off = 0.0;
+        rc = lbm_beads_move(h);
+        if (!rc)
+            rc = lbm_beads_links(h);
+        if (!rc)
+            rc = lbm_beads_filling(h);
+        if (rc)
+            return rc;
+    }
+    return check_overflow(h, "lbm_run_particles");
+}
+
+int lbm_particles_get(lbm_channel *h, double *pos, double *vel, double *omg, double *force, double *torque,
+                      double *contact)
+{
+    CHECK_P(h);
+    const PartState &ps = h->part->ps;
+    const size_t b = 3 * (size_t)h->part->pg.np * sizeof(double);
+    const double *src[6] = {ps.X, ps.V, ps.W, ps.FH, ps.TQ, ps.FC};
+    double *dst[6] = {pos, vel, omg, force, torque, contact};
+    for (int k = 0; k < 6; k++)
+        if (dst[k])
+            HIP_TRY(hipMemcpyAsync(dst[k], src[k], b, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return check_overflow(h, "lbm_particles_get");
+}
+
+int lbm_particles_links(lbm_channel *h, int *maxlink, int *counts, int *node, int *dir, double *q)
+{
+    CHECK_P(h);
+    const PartGeom &pg = h->part->pg;
+    const PartState &ps = h->part->ps;
+    if (maxlink)
+        *maxlink = pg.maxlink;
+    const size_t nl = (size_t)pg.np * pg.maxlink;
+    if (counts)
+        HIP_TRY(hipMemcpyAsync(counts, ps.link_count, pg.np * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (node)
+        HIP_TRY(hipMemcpyAsync(node, ps.link_node, nl * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (q)
+        HIP_TRY(hipMemcpyAsync(q, ps.link_q, nl * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (dir) {
+        std::vector<signed char> d(nl);
+        HIP_TRY(hipMemcpyAsync(d.data(), ps.link_dir, nl, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (size_t k = 0; k < nl; k++)
+            dir[k] = d[k];
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return check_overflow(h, "lbm_particles_links");
+}
+
+int lbm_particles_solid(lbm_channel *h, int *solid)
+{
+    CHECK_P(h);
+    if (!solid)
+        return fail(LBM_ERR_ARG, "lbm_particles_solid: null output");
+    HIP_TRY(hipMemcpyAsync(solid, h->part->ps.solid, h->g.N * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return LBM_OK;
 }
 

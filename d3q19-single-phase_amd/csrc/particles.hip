@@ -1,0 +1,533 @@
+// Particle kernels (see particles.h for the methods and their sources).
+// Built with -ffp-contract=off: every FP64 operation rounds as written, so
+// the CPU oracle's identical expressions give identical bits.
+#include <utility>
+
+#include "particles.h"
+
+namespace d3q19 {
+
+namespace {
+
+constexpr int PBLOCK = 256;
+
+// Unwrapped bounding box of a sphere (node centres at i + 0.5), grown by
+// `pad` nodes; x is clamped to the channel, y/z are taken modulo the box
+// (a box at least as wide as the period covers it exactly once).
+struct Box {
+    int x0, x1, y0, y1, z0, z1;
+};
+
+__host__ __device__ inline int floor_i(double v) { return (int)floor(v); }
+
+__host__ __device__ inline Box sphere_box(const PartGeom &pg, const double *X, int pad)
+{
+    Box b;
+    b.x0 = floor_i(X[0] - 0.5 - pg.R) - pad;
+    b.x1 = floor_i(X[0] - 0.5 + pg.R) + 1 + pad;
+    b.y0 = floor_i(X[1] - 0.5 - pg.R) - pad;
+    b.y1 = floor_i(X[1] - 0.5 + pg.R) + 1 + pad;
+    b.z0 = floor_i(X[2] - 0.5 - pg.R) - pad;
+    b.z1 = floor_i(X[2] - 0.5 + pg.R) + 1 + pad;
+    if (b.x0 < 0)
+        b.x0 = 0;
+    if (b.x1 > (int)pg.g.lx - 1)
+        b.x1 = (int)pg.g.lx - 1;
+    if (b.y1 - b.y0 + 1 >= (int)pg.g.ly) {
+        b.y0 = 0;
+        b.y1 = (int)pg.g.ly - 1;
+    }
+    if (b.z1 - b.z0 + 1 >= (int)pg.g.lz) {
+        b.z0 = 0;
+        b.z1 = (int)pg.g.lz - 1;
+    }
+    return b;
+}
+
+__host__ __device__ inline int wrap(int i, int L)
+{
+    int r = i % L;
+    return r < 0 ? r + L : r;
+}
+
+// Minimum-image offset of a node coordinate from a centre along a periodic axis.
+__host__ __device__ inline double min_image(double d, double L)
+{
+    if (d > 0.5 * L)
+        d = d - L;
+    else if (d < -0.5 * L)
+        d = d + L;
+    return d;
+}
+
+__host__ __device__ inline void node_offset(const PartGeom &pg, int ix, int iy, int iz, const double *X, double &dx,
+                                            double &dy, double &dz)
+{
+    dx = ((double)ix + 0.5) - X[0];
+    dy = min_image(((double)iy + 0.5) - X[1], (double)pg.g.ly);
+    dz = min_image(((double)iz + 0.5) - X[2], (double)pg.g.lz);
+}
+
+__host__ __device__ inline bool inside(const PartGeom &pg, int ix, int iy, int iz, const double *X)
+{
+    double dx, dy, dz;
+    node_offset(pg, ix, iy, iz, X, dx, dy, dz);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    return d2 < pg.R2;
+}
+
+__device__ inline uint32_t node_of(const Geom &g, int ix, int iy, int iz)
+{
+    return (uint32_t)ix + g.lx * ((uint32_t)iy + g.ly * (uint32_t)iz);
+}
+
+// Block-wide exclusive scan of one int per thread (PBLOCK threads).
+__device__ inline int block_scan(int v, int &total)
+{
+    __shared__ int s[PBLOCK];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < PBLOCK; o <<= 1) {
+        int t = (threadIdx.x >= (unsigned)o) ? s[threadIdx.x - o] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    total = s[PBLOCK - 1];
+    const int ex = s[threadIdx.x] - v;
+    __syncthreads();
+    return ex;
+}
+
+// Phase 1: nodes of particle p's previous box still marked p but no longer
+// inside become -2-p and join p's refill list (canonical box order).
+__global__ void __launch_bounds__(PBLOCK) k_p_uncover(PartGeom pg, PartState ps, const double *Xold)
+{
+    const int p = blockIdx.x;
+    const Box b = sphere_box(pg, Xold + 3 * p, 0);
+    const int bx = b.x1 - b.x0 + 1, by = b.y1 - b.y0 + 1, bz = b.z1 - b.z0 + 1;
+    const int total = bx * by * bz;
+    const double *X = ps.X + 3 * p;
+    int base = 0;
+    for (int c0 = 0; c0 < total; c0 += PBLOCK) {
+        const int e = c0 + threadIdx.x;
+        int flag = 0;
+        uint32_t n = 0;
+        if (e < total) {
+            const int kx = e % bx, ky = (e / bx) % by, kz = e / (bx * by);
+            const int ix = b.x0 + kx, iy = wrap(b.y0 + ky, pg.g.ly), iz = wrap(b.z0 + kz, pg.g.lz);
+            n = node_of(pg.g, ix, iy, iz);
+            if (ps.solid[n] == p && !inside(pg, ix, iy, iz, X))
+                flag = 1;
+        }
+        int tot;
+        const int off = block_scan(flag, tot);
+        if (flag) {
+            ps.solid[n] = -2 - p;
+            if (base + off < pg.maxfill)
+                ps.fill_node[(size_t)p * pg.maxfill + base + off] = (int)n;
+            else
+                *ps.overflow = 1;
+        }
+        base += tot;
+    }
+    if (threadIdx.x == 0)
+        ps.fill_count[p] = base < pg.maxfill ? base : pg.maxfill;
+}
+
+// Phase 2: claim the nodes inside particle p (smallest id wins an overlap).
+__global__ void __launch_bounds__(PBLOCK) k_p_mark(PartGeom pg, PartState ps)
+{
+    const int p = blockIdx.x;
+    const double *X = ps.X + 3 * p;
+    const Box b = sphere_box(pg, X, 0);
+    const int bx = b.x1 - b.x0 + 1, by = b.y1 - b.y0 + 1, bz = b.z1 - b.z0 + 1;
+    const int total = bx * by * bz;
+    for (int e = threadIdx.x; e < total; e += PBLOCK) {
+        const int kx = e % bx, ky = (e / bx) % by, kz = e / (bx * by);
+        const int ix = b.x0 + kx, iy = wrap(b.y0 + ky, pg.g.ly), iz = wrap(b.z0 + kz, pg.g.lz);
+        if (inside(pg, ix, iy, iz, X)) {
+            // negative (fluid) values compare above every id as unsigned
+            atomicMin((unsigned int *)&ps.solid[node_of(pg.g, ix, iy, iz)], (unsigned int)p);
+        }
+    }
+}
+
+// Phase 3: cut links of particle p: fluid node x, direction i with x + c_i
+// inside p.  Canonical order: box node (z, y, x), then direction 1..18.
+__global__ void __launch_bounds__(PBLOCK) k_p_links(PartGeom pg, PartState ps)
+{
+    const int p = blockIdx.x;
+    const double *X = ps.X + 3 * p;
+    const Box b = sphere_box(pg, X, 1);
+    const int bx = b.x1 - b.x0 + 1, by = b.y1 - b.y0 + 1, bz = b.z1 - b.z0 + 1;
+    const int total = bx * by * bz;
+    int base = 0;
+    for (int c0 = 0; c0 < total; c0 += PBLOCK) {
+        const int e = c0 + threadIdx.x;
+        uint32_t mask = 0;
+        int ix = 0, iy = 0, iz = 0;
+        uint32_t n = 0;
+        if (e < total) {
+            const int kx = e % bx, ky = (e / bx) % by, kz = e / (bx * by);
+            ix = b.x0 + kx;
+            iy = wrap(b.y0 + ky, pg.g.ly);
+            iz = wrap(b.z0 + kz, pg.g.lz);
+            n = node_of(pg.g, ix, iy, iz);
+            if (ps.solid[n] < 0) {
+                for (int i = 1; i < 19; i++) {
+                    const int jx = ix + CX[i];
+                    if (jx < 0 || jx >= (int)pg.g.lx)
+                        continue;
+                    const int jy = wrap(iy + CY[i], pg.g.ly), jz = wrap(iz + CZ[i], pg.g.lz);
+                    if (ps.solid[node_of(pg.g, jx, jy, jz)] == p)
+                        mask |= 1u << i;
+                }
+            }
+        }
+        int tot;
+        const int off = block_scan(__builtin_popcount(mask), tot);
+        if (mask) {
+            double dx, dy, dz;
+            node_offset(pg, ix, iy, iz, X, dx, dy, dz);
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            const double cc = d2 - pg.R2;
+            int k = base + off;
+            for (int i = 1; i < 19; i++) {
+                if (!(mask & (1u << i)))
+                    continue;
+                const double cx = (double)CX[i], cy = (double)CY[i], cz = (double)CZ[i];
+                const double a = cx * cx + cy * cy + cz * cz;
+                const double bb = cx * dx + cy * dy + cz * dz;
+                const double disc = bb * bb - a * cc;
+                double q = (disc >= 0.0) ? (-bb - sqrt(disc)) / a : 0.5;
+                if (q < 0.0)
+                    q = 0.0;
+                if (q > 1.0)
+                    q = 1.0;
+                if (k < pg.maxlink) {
+                    const size_t s = (size_t)p * pg.maxlink + k;
+                    ps.link_node[s] = (int)n;
+                    ps.link_dir[s] = (signed char)i;
+                    ps.link_q[s] = q;
+                } else {
+                    *ps.overflow = 1;
+                }
+                k++;
+            }
+        }
+        base += tot;
+    }
+    if (threadIdx.x == 0)
+        ps.link_count[p] = base < pg.maxlink ? base : pg.maxlink;
+}
+
+// Storage access with a run-time direction (FP16 scale by direction class).
+template <int P>
+__device__ inline double ld_rt(const typename Prec<P>::S *L, uint64_t N, int i, uint32_t n)
+{
+    const typename Prec<P>::S v = L[(uint64_t)i * N + n];
+    if constexpr (P == 2)
+        return (double)((float)v * (i == 0 ? 0.25f : (i <= 6 ? 1.0f / 16.0f : 1.0f / 32.0f)));
+    else
+        return (double)v;
+}
+
+template <int P>
+__device__ inline void st_rt(typename Prec<P>::S *L, uint64_t N, int i, uint32_t n, double v)
+{
+    using S = typename Prec<P>::S;
+    if constexpr (P == 2)
+        L[(uint64_t)i * N + n] = (S)((float)v * (i == 0 ? 4.0f : (i <= 6 ? 16.0f : 32.0f)));
+    else
+        L[(uint64_t)i * N + n] = (S)v;
+}
+
+// Interpolated bounce-back on every cut link, written into the solid node's
+// slot that the fluid node pulls next step; per-link momentum exchange.
+template <int P>
+__global__ void __launch_bounds__(PBLOCK) k_p_ibb(PartGeom pg, PartState ps, void *Lv)
+{
+    using S = typename Prec<P>::S;
+    const int p = blockIdx.y;
+    const int k = blockIdx.x * PBLOCK + threadIdx.x;
+    if (k >= ps.link_count[p])
+        return;
+    S *L = (S *)Lv;
+    const uint64_t N = pg.g.N;
+    const size_t s = (size_t)p * pg.maxlink + k;
+    const uint32_t n = (uint32_t)ps.link_node[s];
+    const int i = ps.link_dir[s];
+    const int ib = OPP[i];
+    const double q = ps.link_q[s];
+    const uint32_t qn = fdiv(n, pg.g.dlx);
+    const int ix = (int)(n - qn * pg.g.lx);
+    const int iz = (int)fdiv(qn, pg.g.dly);
+    const int iy = (int)(qn - (uint32_t)iz * pg.g.ly);
+    const double cx = (double)CX[i], cy = (double)CY[i], cz = (double)CZ[i];
+    const double w = i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0;
+
+    const double *X = ps.X + 3 * p, *V = ps.V + 3 * p, *W = ps.W + 3 * p;
+    double dx, dy, dz;
+    node_offset(pg, ix, iy, iz, X, dx, dy, dz);
+    const double rx = dx + q * cx, ry = dy + q * cy, rz = dz + q * cz; // wall point - centre
+    const double uwx = V[0] + (W[1] * rz - W[2] * ry);
+    const double uwy = V[1] + (W[2] * rx - W[0] * rz);
+    const double uwz = V[2] + (W[0] * ry - W[1] * rx);
+    const double cu = cx * uwx + cy * uwy + cz * uwz;
+
+    const double fi = ld_rt<P>(L, N, i, n);
+    double g;
+    if (q < 0.5) {
+        // second fluid node behind x along -c_i
+        const int bx_ = ix - CX[i];
+        bool ok = (bx_ >= 0 && bx_ < (int)pg.g.lx);
+        uint32_t nb = 0;
+        if (ok) {
+            nb = node_of(pg.g, bx_, wrap(iy - CY[i], pg.g.ly), wrap(iz - CZ[i], pg.g.lz));
+            ok = ps.solid[nb] < 0;
+        }
+        if (ok) {
+            const double fb = ld_rt<P>(L, N, i, nb);
+            g = (2.0 * q) * fi + (1.0 - 2.0 * q) * fb - 6.0 * w * cu;
+        } else {
+            g = fi - 6.0 * w * cu;
+        }
+    } else {
+        const double fib = ld_rt<P>(L, N, ib, n);
+        g = (1.0 / (2.0 * q)) * fi + ((2.0 * q - 1.0) / (2.0 * q)) * fib - (3.0 / q) * w * cu;
+    }
+    // the solid node x + c_i: its slot ib is what x pulls for direction ib
+    const uint32_t ns = node_of(pg.g, ix + CX[i], wrap(iy + CY[i], pg.g.ly), wrap(iz + CZ[i], pg.g.lz));
+    st_rt<P>(L, N, ib, ns, g);
+
+    // momentum exchange with full populations f = shifted + w*rho0
+    const double ff = fi + w, gf = g + w;
+    const double px = ff * (cx - uwx) + gf * (cx + uwx);
+    const double py = ff * (cy - uwy) + gf * (cy + uwy);
+    const double pz = ff * (cz - uwz) + gf * (cz + uwz);
+    double *c = ps.link_contrib + s * 6;
+    c[0] = px;
+    c[1] = py;
+    c[2] = pz;
+    c[3] = ry * pz - rz * py;
+    c[4] = rz * px - rx * pz;
+    c[5] = rx * py - ry * px;
+}
+
+// Per-particle force and torque: links summed in canonical order.
+__global__ void k_p_sum(PartGeom pg, PartState ps)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pg.np)
+        return;
+    double f[6] = {0, 0, 0, 0, 0, 0};
+    const int nl = ps.link_count[p];
+    for (int k = 0; k < nl; k++) {
+        const double *c = ps.link_contrib + ((size_t)p * pg.maxlink + k) * 6;
+        for (int j = 0; j < 6; j++)
+            f[j] = f[j] + c[j];
+    }
+    for (int j = 0; j < 3; j++) {
+        ps.FH[3 * p + j] = f[j];
+        ps.TQ[3 * p + j] = f[3 + j];
+    }
+}
+
+// Contact forces and FP64 integration, one thread per particle.
+__global__ void k_p_move(PartGeom pg, PartState ps, double *Xold)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pg.np)
+        return;
+    const double Ly = (double)pg.g.ly, Lz = (double)pg.g.lz, Lx = (double)pg.g.lx;
+    const double *Xp = ps.X + 3 * p;
+    double fc[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < pg.np; q++) {
+        if (q == p)
+            continue;
+        const double *Xq = ps.X + 3 * q;
+        const double ddx = Xp[0] - Xq[0];
+        const double ddy = min_image(Xp[1] - Xq[1], Ly);
+        const double ddz = min_image(Xp[2] - Xq[2], Lz);
+        const double dist = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+        const double h = dist - 2.0 * pg.R;
+        if (h < pg.mingap && dist > 0.0) {
+            const double r = (pg.mingap - h) / pg.mingap;
+            const double mag = pg.stf * (r * r);
+            fc[0] = fc[0] + mag * (ddx / dist);
+            fc[1] = fc[1] + mag * (ddy / dist);
+            fc[2] = fc[2] + mag * (ddz / dist);
+        }
+    }
+    const double hlo = Xp[0] - pg.R;
+    if (hlo < pg.mingap_w) {
+        const double r = (pg.mingap_w - hlo) / pg.mingap_w;
+        fc[0] = fc[0] + pg.stf_w * (r * r);
+    }
+    const double hhi = Lx - Xp[0] - pg.R;
+    if (hhi < pg.mingap_w) {
+        const double r = (pg.mingap_w - hhi) / pg.mingap_w;
+        fc[0] = fc[0] - pg.stf_w * (r * r);
+    }
+    const double fb[3] = {pg.fx, pg.fy, pg.fz};
+    const bool first = (*ps.nsteps_done == 0);
+    double Xn[3], Vn[3], Wn[3];
+    for (int j = 0; j < 3; j++) {
+        ps.FC[3 * p + j] = fc[j];
+        double ftot = ps.FH[3 * p + j] + fc[j];
+        if (pg.pressure_force)
+            ftot = ftot + pg.volume * fb[j];
+        const double a = ftot / pg.mass;
+        const double al = ps.TQ[3 * p + j] / pg.inertia;
+        const double v0 = ps.V[3 * p + j], w0 = ps.W[3 * p + j];
+        Vn[j] = first ? v0 + a : v0 + (1.5 * a - 0.5 * ps.Aprev[3 * p + j]);
+        Wn[j] = first ? w0 + al : w0 + (1.5 * al - 0.5 * ps.ALprev[3 * p + j]);
+        Xn[j] = Xp[j] + 0.5 * (v0 + Vn[j]);
+        ps.TH[3 * p + j] = ps.TH[3 * p + j] + 0.5 * (w0 + Wn[j]);
+        ps.Aprev[3 * p + j] = a;
+        ps.ALprev[3 * p + j] = al;
+    }
+    if (Xn[1] >= Ly)
+        Xn[1] = Xn[1] - Ly;
+    else if (Xn[1] < 0.0)
+        Xn[1] = Xn[1] + Ly;
+    if (Xn[2] >= Lz)
+        Xn[2] = Xn[2] - Lz;
+    else if (Xn[2] < 0.0)
+        Xn[2] = Xn[2] + Lz;
+    // every thread has read all positions (contacts) before any is updated:
+    // positions go to Xold first and are swapped in by k_p_commit
+    for (int j = 0; j < 3; j++) {
+        Xold[3 * p + j] = Xn[j];
+        ps.V[3 * p + j] = Vn[j];
+        ps.W[3 * p + j] = Wn[j];
+    }
+}
+
+// Swap new positions in; the previous positions are kept in Xold.
+__global__ void k_p_commit(PartGeom pg, PartState ps, double *Xold)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pg.np)
+        return;
+    for (int j = 0; j < 3; j++) {
+        const double xn = Xold[3 * p + j];
+        Xold[3 * p + j] = ps.X[3 * p + j];
+        ps.X[3 * p + j] = xn;
+    }
+    if (p == 0)
+        *ps.nsteps_done = *ps.nsteps_done + 1;
+}
+
+// Refill: nodes uncovered by particle p get the equilibrium of the wall
+// velocity there and the mean density of their settled fluid neighbours.
+template <int P>
+__global__ void __launch_bounds__(64) k_p_refill(PartGeom pg, PartState ps, void *Lv)
+{
+    using S = typename Prec<P>::S;
+    const int p = blockIdx.y;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= ps.fill_count[p])
+        return;
+    S *L = (S *)Lv;
+    const uint64_t N = pg.g.N;
+    const uint32_t n = (uint32_t)ps.fill_node[(size_t)p * pg.maxfill + k];
+    if (ps.solid[n] != -2 - p)
+        return; // claimed by another particle since
+    const uint32_t qn = fdiv(n, pg.g.dlx);
+    const int ix = (int)(n - qn * pg.g.lx);
+    const int iz = (int)fdiv(qn, pg.g.dly);
+    const int iy = (int)(qn - (uint32_t)iz * pg.g.ly);
+    double rsum = 0.0;
+    int cnt = 0;
+    for (int j = 1; j < 19; j++) {
+        const int jx = ix + CX[j];
+        if (jx < 0 || jx >= (int)pg.g.lx)
+            continue;
+        const uint32_t m = node_of(pg.g, jx, wrap(iy + CY[j], pg.g.ly), wrap(iz + CZ[j], pg.g.lz));
+        if (ps.solid[m] != -1)
+            continue;
+        double r = ld_rt<P>(L, N, 0, m);
+        for (int i = 1; i < 19; i++)
+            r = r + ld_rt<P>(L, N, i, m);
+        rsum = rsum + r;
+        cnt++;
+    }
+    const double rho = cnt > 0 ? rsum / (double)cnt : 0.0;
+    const double *X = ps.X + 3 * p, *V = ps.V + 3 * p, *W = ps.W + 3 * p;
+    double dx, dy, dz;
+    node_offset(pg, ix, iy, iz, X, dx, dy, dz);
+    const double ux = V[0] + (W[1] * dz - W[2] * dy);
+    const double uy = V[1] + (W[2] * dx - W[0] * dz);
+    const double uz = V[2] + (W[0] * dy - W[1] * dx);
+    double usqr = ux * ux + uy * uy + uz * uz;
+    usqr = 1.5 * usqr;
+    st_rt<P>(L, N, 0, n, (1.0 / 3.0) * (rho - usqr));
+    for (int i = 1; i < 19; i++) {
+        const double G = (double)CX[i] * ux + (double)CY[i] * uy + (double)CZ[i] * uz;
+        const double w = i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0;
+        st_rt<P>(L, N, i, n, w * (rho + 3.0 * G + 4.5 * G * G - usqr));
+    }
+}
+
+// Uncovered nodes become ordinary fluid once every refill has read its
+// neighbours.
+__global__ void k_p_settle(PartGeom pg, PartState ps)
+{
+    const int p = blockIdx.y;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= ps.fill_count[p])
+        return;
+    const uint32_t n = (uint32_t)ps.fill_node[(size_t)p * pg.maxfill + k];
+    if (ps.solid[n] == -2 - p)
+        ps.solid[n] = -1;
+}
+
+} // namespace
+
+hipError_t part_classify(const PartGeom &pg, const PartState &ps, int first, hipStream_t s)
+{
+    // ps.Aprev is not touched; Xold lives after X in the state (see lbm_capi)
+    if (!first)
+        hipLaunchKernelGGL(k_p_uncover, dim3(pg.np), dim3(PBLOCK), 0, s, pg, ps, ps.X + 3 * (size_t)pg.np);
+    hipLaunchKernelGGL(k_p_mark, dim3(pg.np), dim3(PBLOCK), 0, s, pg, ps);
+    hipLaunchKernelGGL(k_p_links, dim3(pg.np), dim3(PBLOCK), 0, s, pg, ps);
+    return hipGetLastError();
+}
+
+hipError_t part_sum(const PartGeom &pg, const PartState &ps, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_p_sum, dim3((pg.np + 63) / 64), dim3(64), 0, s, pg, ps);
+    return hipGetLastError();
+}
+
+hipError_t part_move(const PartGeom &pg, const PartState &ps, hipStream_t s)
+{
+    double *Xold = ps.X + 3 * (size_t)pg.np;
+    hipLaunchKernelGGL(k_p_move, dim3((pg.np + 63) / 64), dim3(64), 0, s, pg, ps, Xold);
+    hipLaunchKernelGGL(k_p_commit, dim3((pg.np + 63) / 64), dim3(64), 0, s, pg, ps, Xold);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t PartLaunch<P>::ibb(const PartGeom &pg, const PartState &ps, void *L, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_p_ibb<P>), dim3((pg.maxlink + PBLOCK - 1) / PBLOCK, pg.np), dim3(PBLOCK), 0, s, pg, ps, L);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t PartLaunch<P>::refill(const PartGeom &pg, const PartState &ps, void *L, hipStream_t s)
+{
+    const dim3 grid((pg.maxfill + 63) / 64, pg.np);
+    hipLaunchKernelGGL((k_p_refill<P>), grid, dim3(64), 0, s, pg, ps, L);
+    hipLaunchKernelGGL(k_p_settle, grid, dim3(64), 0, s, pg, ps);
+    return hipGetLastError();
+}
+
+template struct PartLaunch<0>;
+template struct PartLaunch<1>;
+template struct PartLaunch<2>;
+
+} // namespace d3q19

@@ -46,6 +46,13 @@ class LbmConfig(ctypes.Structure):
             "ww0", "ww1", "ww2")]
 
 
+class ParticleConfig(ctypes.Structure):
+    """Mirror of struct lbm_particle_config in include/lbm.h."""
+    _fields_ = [("npart", ctypes.c_int), ("maxlink", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in ("rad", "rhopart", "mingap", "mingap_w", "stf", "stf_w")] + [
+        ("pressure_force", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
 # Every function the C ABI exports, with its argument types.
 _D = ctypes.c_void_p
 _H = ctypes.c_void_p
@@ -70,6 +77,16 @@ _SIGS = {
     "lbm_sync": [_H],
     "lbm_time_steps": [_H, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "lbm_device_bytes": [_H, ctypes.POINTER(ctypes.c_size_t)],
+    "lbm_download_post": [_H, _D],
+    "lbm_particles_create": [_H, _D, _D, _D, _D],
+    "lbm_beads_links": [_H],
+    "lbm_beads_filling": [_H],
+    "lbm_beads_collision": [_H],
+    "lbm_beads_move": [_H],
+    "lbm_run_particles": [_H, ctypes.c_int],
+    "lbm_particles_get": [_H, _D, _D, _D, _D, _D, _D],
+    "lbm_particles_links": [_H, ctypes.POINTER(ctypes.c_int), _D, _D, _D, _D],
+    "lbm_particles_solid": [_H, _D],
 }
 
 _lib = None
@@ -265,3 +282,61 @@ class Channel:
         v = ctypes.c_size_t()
         _call("lbm_device_bytes", self._h, ctypes.byref(v))
         return v.value
+
+    def download_post(self):
+        """Stored post-collision lattice, FP64, shape (19, N) (node = ix + lx*(iy + ly*iz))."""
+        out = np.empty((19, self.N))
+        _call("lbm_download_post", self._h, _ptr(out))
+        return out
+
+    # -- rigid particles (beads_* of Channel-Flow/var_inc.f90:166-168) ----------
+    def particles_create(self, pos, vel=None, omg=None, rad=5.0, rhopart=1.0, mingap=3.0, mingap_w=3.0,
+                         stf=0.025, stf_w=0.025, pressure_force=True, maxlink=0):
+        """Spheres at pos (npart, 3) in node coordinates (node ix at ix - 0.5, 1-based)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        pc = ParticleConfig()
+        pc.npart, pc.maxlink = pos.shape[0], int(maxlink)
+        pc.rad, pc.rhopart, pc.mingap, pc.mingap_w, pc.stf, pc.stf_w = rad, rhopart, mingap, mingap_w, stf, stf_w
+        pc.pressure_force, pc.reserved = int(bool(pressure_force)), 0
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        w = None if omg is None else np.ascontiguousarray(omg, dtype=np.float64).reshape(-1, 3)
+        _call("lbm_particles_create", self._h, ctypes.byref(pc), _ptr(pos), _ptr(v), _ptr(w))
+        self.npart = pos.shape[0]
+
+    def beads_links(self):
+        _call("lbm_beads_links", self._h)
+
+    def beads_filling(self):
+        _call("lbm_beads_filling", self._h)
+
+    def beads_collision(self):
+        _call("lbm_beads_collision", self._h)
+
+    def beads_move(self):
+        _call("lbm_beads_move", self._h)
+
+    def run_particles(self, nsteps):
+        _call("lbm_run_particles", self._h, int(nsteps))
+
+    def particles_get(self):
+        out = {k: np.empty((self.npart, 3)) for k in ("pos", "vel", "omg", "force", "torque", "contact")}
+        _call("lbm_particles_get", self._h, *[_ptr(out[k]) for k in ("pos", "vel", "omg", "force", "torque",
+                                                                   "contact")])
+        return out
+
+    def particles_links(self):
+        m = ctypes.c_int()
+        _call("lbm_particles_links", self._h, ctypes.byref(m), None, None, None, None)
+        cap = m.value
+        counts = np.empty(self.npart, dtype=np.int32)
+        node = np.empty(self.npart * cap, dtype=np.int32)
+        dirs = np.empty(self.npart * cap, dtype=np.int32)
+        q = np.empty(self.npart * cap)
+        _call("lbm_particles_links", self._h, ctypes.byref(m), counts.ctypes.data_as(ctypes.c_void_p),
+              node.ctypes.data_as(ctypes.c_void_p), dirs.ctypes.data_as(ctypes.c_void_p), _ptr(q))
+        return cap, counts, node, dirs, q
+
+    def particles_solid(self):
+        s = np.empty(self.N, dtype=np.int32)
+        _call("lbm_particles_solid", self._h, s.ctypes.data_as(ctypes.c_void_p))
+        return s

@@ -96,3 +96,38 @@ def loglaw_divfree(lx, ly, lz, ustar, ystar, seed=2, amp=0.10, nmodes=8):
     uz *= scale
     uy += prof[None, None, :]
     return ux, uy, uz
+
+
+def sphere_positions(lx, ly, lz, n, rad, gap=3.0, gap_w=3.0, seed=3, max_tries=2_000_000):
+    """n sphere centres in node coordinates (node i at i - 0.5, 1-based; walls at
+    x = 0 and x = lx), uniform by rejection sampling from stream `seed`:
+    surface gaps >= gap between spheres (minimum image in y, z) and >= gap_w to
+    the walls.  Raises if the spheres cannot be placed."""
+    pos = np.empty((n, 3))
+    xlo, xhi = rad + gap_w, lx - rad - gap_w
+    if xhi <= xlo:
+        raise ValueError("channel too narrow for the spheres")
+    dmin2 = (2.0 * rad + gap) ** 2
+    k = 0
+    tries = 0
+    batch = 4096
+    off = 0
+    while k < n:
+        r = uniform(seed, 3 * batch, offset=off)
+        off += 3 * batch
+        for j in range(batch):
+            tries += 1
+            if tries > max_tries:
+                raise ValueError(f"placed only {k} of {n} spheres")
+            c = np.array([xlo + (xhi - xlo) * r[3 * j], ly * r[3 * j + 1], lz * r[3 * j + 2]])
+            if k:
+                d = pos[:k] - c
+                d[:, 1] -= ly * np.round(d[:, 1] / ly)
+                d[:, 2] -= lz * np.round(d[:, 2] / lz)
+                if np.min(np.einsum("ij,ij->i", d, d)) < dmin2:
+                    continue
+            pos[k] = c
+            k += 1
+            if k == n:
+                break
+    return pos

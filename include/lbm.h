@@ -148,6 +148,73 @@ int lbm_time_steps(lbm_channel *h, int nsteps, double *avg_ms, double *total_ms)
 /* Device bytes held by the handle (lattices, fields, scratch). */
 int lbm_device_bytes(lbm_channel *h, size_t *bytes);
 
+/* The stored post-collision lattice as FP64 structure-of-arrays (19*N values,
+ * population-major, node index ix + lx*(iy + ly*iz)), for checkpoint-free
+ * inspection and tests.  Synchronises. */
+int lbm_download_post(lbm_channel *h, double *post);
+
+/* ---------------------------------------------------------------------------
+ * Rigid spherical particles.  The reference keeps only the data structures of
+ * its particle subsystem (Channel-Flow/var_inc.f90:30-38,120-150,
+ * para.f90:332-379,449-501) and the routine names beads_links,
+ * beads_collision, beads_filling, beads_move (var_inc.f90:166-168); the
+ * entry points below carry those names.  Positions use the reference's node
+ * coordinates: node ix (1-based) sits at x = ix - 0.5 (saveload.f90:1322),
+ * walls at x = 0 and x = lx, periodic y in [0, ly) and z in [0, lz).
+ * Particle state and all particle arithmetic are FP64.
+ */
+typedef struct lbm_particle_config {
+    int npart;             /* number of spheres */
+    int maxlink;           /* per-particle cut-link capacity (0 = from the radius) */
+    double rad;            /* radius (var_inc.f90:67 rad) */
+    double rhopart;        /* particle / fluid density ratio (var_inc.f90:65) */
+    double mingap;         /* particle-particle contact range (var_inc.f90:67) */
+    double mingap_w;       /* particle-wall contact range (var_inc.f90:67) */
+    double stf;            /* particle-particle contact stiffness 1/eps */
+    double stf_w;          /* particle-wall contact stiffness */
+    int pressure_force;    /* 1: each particle also feels volume x body force */
+    int reserved;          /* must be 0 */
+} lbm_particle_config;
+
+/* Creates the particle set (positions, velocities, angular velocities, each
+ * 3*npart, particle-major) and classifies the lattice (beads_links). */
+int lbm_particles_create(lbm_channel *h, const lbm_particle_config *pc, const double *pos, const double *vel,
+                         const double *omg);
+
+/* beads_links: marks nodes inside particles (bit-exact FP64 test
+ * |x - X|^2 < R^2), lists every cut link with its wall fraction q, and
+ * records the nodes the last move uncovered. */
+int lbm_beads_links(lbm_channel *h);
+
+/* beads_filling: gives nodes uncovered by the last move the equilibrium of
+ * the local wall velocity and the mean density of their fluid neighbours. */
+int lbm_beads_filling(lbm_channel *h);
+
+/* beads_collision: interpolated bounce-back on every cut link (written where
+ * the next collision_MRT pulls it) and the momentum-exchange force and torque
+ * on every particle (FP64, summed link by link in a fixed order). */
+int lbm_beads_collision(lbm_channel *h);
+
+/* beads_move: contact forces and FP64 integration of the rigid-body motion
+ * (Adams-Bashforth 2 velocities, trapezoidal positions). */
+int lbm_beads_move(lbm_channel *h);
+
+/* nsteps of: beads_collision, collision_MRT, macrovar, beads_move,
+ * beads_links, beads_filling. */
+int lbm_run_particles(lbm_channel *h, int nsteps);
+
+/* Particle state (each 3*npart, any pointer may be NULL).  force/torque are
+ * the last momentum-exchange sums; contact the last contact forces. */
+int lbm_particles_get(lbm_channel *h, double *pos, double *vel, double *omg, double *force, double *torque,
+                      double *contact);
+
+/* Cut links: counts[npart]; node/dir/q hold npart*maxlink entries
+ * (particle-major, counts[p] valid each).  *maxlink receives the capacity. */
+int lbm_particles_links(lbm_channel *h, int *maxlink, int *counts, int *node, int *dir, double *q);
+
+/* Node flags (N ints): -1 fluid, p >= 0 inside particle p. */
+int lbm_particles_solid(lbm_channel *h, int *solid);
+
 #ifdef __cplusplus
 }
 #endif

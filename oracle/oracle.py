@@ -187,3 +187,124 @@ class OracleChannel:
             else:
                 self.collision_MRT()
             self.macrovar()
+
+
+# --- rigid particles (oracle/particles_oracle.c) ------------------------------
+
+class PGeom(ctypes.Structure):
+    """Mirror of orc_pgeom (particles_oracle.h)."""
+    _fields_ = [(n, _INT) for n in ("lx", "ly", "lz", "np", "maxlink", "maxfill", "pressure_force",
+                                     "reserved")] + [
+        (n, _DBL) for n in ("R", "R2", "mass", "inertia", "volume", "mingap", "mingap_w", "stf", "stf_w",
+                            "fx", "fy", "fz")]
+
+
+class PState(ctypes.Structure):
+    """Mirror of orc_pstate (particles_oracle.h)."""
+    _fields_ = [(n, ctypes.c_void_p) for n in ("X", "Xold", "V", "W", "TH", "Aprev", "ALprev", "FH", "TQ", "FC",
+                                               "solid", "link_node", "link_dir", "link_q", "link_count",
+                                               "fill_node", "fill_count")] + [("nsteps_done", _INT)]
+
+
+def _plib():
+    L = lib()
+    if not hasattr(L, "_p_ready"):
+        G, S = ctypes.POINTER(PGeom), ctypes.POINTER(PState)
+        L.orc_p_classify.argtypes = [G, S, _INT]
+        L.orc_p_classify.restype = _INT
+        L.orc_p_ibb.argtypes = [G, S, ctypes.c_void_p]
+        L.orc_p_step.argtypes = [ctypes.POINTER(OrcParams), G, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 _DBL, _DBL, _DBL]
+        L.orc_p_move.argtypes = [G, S]
+        L.orc_p_refill.argtypes = [G, S, ctypes.c_void_p]
+        L._p_ready = True
+    return L
+
+
+def particle_geometry(lx, ly, lz, npart, rad, rhopart=1.0, mingap=3.0, mingap_w=3.0, stf=0.025, stf_w=0.025,
+                      pressure_force=1, force=(0.0, 0.0, 0.0), maxlink=0):
+    """Particle constants formed exactly as lbm_particles_create forms them."""
+    import math
+    g = PGeom()
+    g.lx, g.ly, g.lz, g.np = lx, ly, lz, npart
+    area = 4.0 * math.pi * (rad + 1.0) * (rad + 1.0)
+    g.maxlink = maxlink if maxlink > 0 else int(12.0 * area) + 256
+    g.maxfill = int(4.0 * area) + 64
+    g.pressure_force = pressure_force
+    g.R = rad
+    g.R2 = rad * rad
+    g.volume = 4.0 / 3.0 * math.pi * rad * rad * rad
+    g.mass = rhopart * g.volume
+    g.inertia = 0.4 * g.mass * rad * rad
+    g.mingap, g.mingap_w, g.stf, g.stf_w = mingap, mingap_w, stf, stf_w
+    g.fx, g.fy, g.fz = force
+    return g
+
+
+class OracleParticleChannel:
+    """Particle-laden channel in the pull form the device uses: two post-collision
+    lattices L[0], L[1] (19*N each, population-major), node flags, cut links."""
+
+    def __init__(self, params, geom, L0, pos, vel=None, omg=None, L1=None):
+        """L0 = None builds a classification-only instance (no lattices)."""
+        self.p, self.g = params, geom
+        self.N = geom.lx * geom.ly * geom.lz
+        np_ = geom.np
+        if L0 is None:
+            self.L = [np.zeros(1), np.zeros(1)]
+        else:
+            self.L = [np.ascontiguousarray(L0, dtype=np.float64).reshape(19 * self.N).copy(),
+                      np.zeros(19 * self.N) if L1 is None
+                      else np.ascontiguousarray(L1, dtype=np.float64).reshape(-1).copy()]
+        self.cur = 0
+        z3 = lambda: np.zeros(3 * np_)
+        self.X = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1).copy()
+        self.Xold = self.X.copy()
+        self.V = z3() if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1).copy()
+        self.W = z3() if omg is None else np.ascontiguousarray(omg, dtype=np.float64).reshape(-1).copy()
+        self.TH, self.Aprev, self.ALprev, self.FH, self.TQ, self.FC = (z3() for _ in range(6))
+        self.solid = np.full(self.N, -1, dtype=np.int32)
+        self.link_node = np.zeros(np_ * geom.maxlink, dtype=np.int32)
+        self.link_dir = np.zeros(np_ * geom.maxlink, dtype=np.int32)
+        self.link_q = np.zeros(np_ * geom.maxlink)
+        self.link_count = np.zeros(np_, dtype=np.int32)
+        self.fill_node = np.zeros(np_ * geom.maxfill, dtype=np.int32)
+        self.fill_count = np.zeros(np_, dtype=np.int32)
+        self.s = PState()
+        for name in ("X", "Xold", "V", "W", "TH", "Aprev", "ALprev", "FH", "TQ", "FC", "solid", "link_node",
+                     "link_dir", "link_q", "link_count", "fill_node", "fill_count"):
+            setattr(self.s, name, getattr(self, name).ctypes.data_as(ctypes.c_void_p))
+        self.s.nsteps_done = 0
+        self.force = (geom.fx, geom.fy, geom.fz)
+        self.beads_links(first=True)
+
+    def beads_links(self, first=False):
+        if _plib().orc_p_classify(ctypes.byref(self.g), ctypes.byref(self.s), 1 if first else 0):
+            raise RuntimeError("oracle: particle link or refill list overflow")
+
+    def beads_collision(self):
+        _plib().orc_p_ibb(ctypes.byref(self.g), ctypes.byref(self.s), _ptr(self.L[self.cur]))
+
+    def collision_MRT(self):
+        _plib().orc_p_step(ctypes.byref(self.p), ctypes.byref(self.g), self.solid.ctypes.data_as(ctypes.c_void_p),
+                           _ptr(self.L[self.cur]), _ptr(self.L[1 - self.cur]), *self.force)
+        self.cur = 1 - self.cur
+
+    def beads_move(self):
+        _plib().orc_p_move(ctypes.byref(self.g), ctypes.byref(self.s))
+
+    def beads_filling(self):
+        _plib().orc_p_refill(ctypes.byref(self.g), ctypes.byref(self.s), _ptr(self.L[self.cur]))
+
+    def step(self, n=1):
+        """n of: beads_collision, collision_MRT, beads_move, beads_links, beads_filling."""
+        for _ in range(n):
+            self.beads_collision()
+            self.collision_MRT()
+            self.beads_move()
+            self.beads_links()
+            self.beads_filling()
+
+    @property
+    def post(self):
+        return self.L[self.cur]

@@ -1,0 +1,402 @@
+/*
+ * CPU oracle for the rigid-particle path -- TEST INFRASTRUCTURE ONLY (used by
+ * tests/, as the checker).
+ *
+ * The reference has no particle source (only its data structures and routine
+ * names: Channel-Flow/var_inc.f90:30-38,120-150,166-168), so this file states
+ * the published methods directly, sequentially, in the order a single thread
+ * would apply them:
+ *   classification  |x - X|^2 < R^2 with minimum image in y, z; cut-link
+ *                   fraction q = smaller root of |x + t c - X|^2 = R^2
+ *   bounce-back     Bouzidi, Firdaouss & Lallemand (2001), linear, moving wall
+ *   force/torque    momentum exchange of Wen et al. (2014)
+ *   refill          equilibrium at the wall velocity, mean neighbour density
+ *   contact         Glowinski et al. (1999) repulsion
+ *   integration     Adams-Bashforth 2 velocities, trapezoidal positions
+ * The fluid step is the pull form of collision_MRT with nodes inside
+ * particles skipped.  State is the post-collision lattice P (19*N,
+ * population-major) in two buffers used alternately, as on the device.
+ *
+ * Pinned by tests/test_particles_oracle.py: brute-force classification over
+ * every node and direction, the q root against the quadratic, q = 1/2 giving
+ * halfway bounce-back, exact momentum balance between fluid and a fixed
+ * particle, Newton's third law for contacts, the closed-form AB2 trajectory
+ * of a particle under a constant force.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "lbm_oracle.h"
+#include "particles_oracle.h"
+
+#define NPOP 19
+
+static const int cix[NPOP] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+static const int ciy[NPOP] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+static const int ciz[NPOP] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+static const int ipopp[NPOP] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+
+static int wrapi(int i, int L)
+{
+    int r = i % L;
+    return r < 0 ? r + L : r;
+}
+
+static double min_image(double d, double L)
+{
+    if (d > 0.5 * L)
+        d = d - L;
+    else if (d < -0.5 * L)
+        d = d + L;
+    return d;
+}
+
+static size_t nodeof(const orc_pgeom *g, int ix, int iy, int iz)
+{
+    return (size_t)ix + (size_t)g->lx * ((size_t)iy + (size_t)g->ly * (size_t)iz);
+}
+
+static void offset(const orc_pgeom *g, int ix, int iy, int iz, const double *X, double *dx, double *dy, double *dz)
+{
+    *dx = ((double)ix + 0.5) - X[0];
+    *dy = min_image(((double)iy + 0.5) - X[1], (double)g->ly);
+    *dz = min_image(((double)iz + 0.5) - X[2], (double)g->lz);
+}
+
+static int inside(const orc_pgeom *g, int ix, int iy, int iz, const double *X)
+{
+    double dx, dy, dz;
+    offset(g, ix, iy, iz, X, &dx, &dy, &dz);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    return d2 < g->R2;
+}
+
+typedef struct {
+    int x0, x1, y0, y1, z0, z1;
+} box;
+
+static box sphere_box(const orc_pgeom *g, const double *X, int pad)
+{
+    box b;
+    b.x0 = (int)floor(X[0] - 0.5 - g->R) - pad;
+    b.x1 = (int)floor(X[0] - 0.5 + g->R) + 1 + pad;
+    b.y0 = (int)floor(X[1] - 0.5 - g->R) - pad;
+    b.y1 = (int)floor(X[1] - 0.5 + g->R) + 1 + pad;
+    b.z0 = (int)floor(X[2] - 0.5 - g->R) - pad;
+    b.z1 = (int)floor(X[2] - 0.5 + g->R) + 1 + pad;
+    if (b.x0 < 0)
+        b.x0 = 0;
+    if (b.x1 > g->lx - 1)
+        b.x1 = g->lx - 1;
+    if (b.y1 - b.y0 + 1 >= g->ly) {
+        b.y0 = 0;
+        b.y1 = g->ly - 1;
+    }
+    if (b.z1 - b.z0 + 1 >= g->lz) {
+        b.z0 = 0;
+        b.z1 = g->lz - 1;
+    }
+    return b;
+}
+
+int orc_p_classify(const orc_pgeom *g, orc_pstate *s, int first)
+{
+    int overflow = 0;
+    /* uncover: nodes of the previous box still marked p but no longer inside */
+    if (!first) {
+        for (int p = 0; p < g->np; p++) {
+            const double *Xo = s->Xold + 3 * p, *X = s->X + 3 * p;
+            box b = sphere_box(g, Xo, 0);
+            int cnt = 0;
+            for (int iz0 = b.z0; iz0 <= b.z1; iz0++)
+                for (int iy0 = b.y0; iy0 <= b.y1; iy0++)
+                    for (int ix = b.x0; ix <= b.x1; ix++) {
+                        int iy = wrapi(iy0, g->ly), iz = wrapi(iz0, g->lz);
+                        size_t n = nodeof(g, ix, iy, iz);
+                        if (s->solid[n] == p && !inside(g, ix, iy, iz, X)) {
+                            s->solid[n] = -2 - p;
+                            if (cnt < g->maxfill)
+                                s->fill_node[(size_t)p * g->maxfill + cnt] = (int)n;
+                            else
+                                overflow = 1;
+                            cnt++;
+                        }
+                    }
+            s->fill_count[p] = cnt < g->maxfill ? cnt : g->maxfill;
+        }
+    }
+    /* mark: smallest particle id wins an overlap */
+    for (int p = 0; p < g->np; p++) {
+        const double *X = s->X + 3 * p;
+        box b = sphere_box(g, X, 0);
+        for (int iz0 = b.z0; iz0 <= b.z1; iz0++)
+            for (int iy0 = b.y0; iy0 <= b.y1; iy0++)
+                for (int ix = b.x0; ix <= b.x1; ix++) {
+                    int iy = wrapi(iy0, g->ly), iz = wrapi(iz0, g->lz);
+                    if (inside(g, ix, iy, iz, X)) {
+                        size_t n = nodeof(g, ix, iy, iz);
+                        if (s->solid[n] < 0 || s->solid[n] > p)
+                            s->solid[n] = p;
+                    }
+                }
+    }
+    /* links in canonical order: box node (z, y, x), then direction */
+    for (int p = 0; p < g->np; p++) {
+        const double *X = s->X + 3 * p;
+        box b = sphere_box(g, X, 1);
+        int k = 0;
+        for (int iz0 = b.z0; iz0 <= b.z1; iz0++)
+            for (int iy0 = b.y0; iy0 <= b.y1; iy0++)
+                for (int ix = b.x0; ix <= b.x1; ix++) {
+                    int iy = wrapi(iy0, g->ly), iz = wrapi(iz0, g->lz);
+                    size_t n = nodeof(g, ix, iy, iz);
+                    if (s->solid[n] >= 0)
+                        continue;
+                    double dx, dy, dz;
+                    offset(g, ix, iy, iz, X, &dx, &dy, &dz);
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    const double cc = d2 - g->R2;
+                    for (int i = 1; i < NPOP; i++) {
+                        int jx = ix + cix[i];
+                        if (jx < 0 || jx >= g->lx)
+                            continue;
+                        size_t m = nodeof(g, jx, wrapi(iy + ciy[i], g->ly), wrapi(iz + ciz[i], g->lz));
+                        if (s->solid[m] != p)
+                            continue;
+                        const double cx = (double)cix[i], cy = (double)ciy[i], cz = (double)ciz[i];
+                        const double a = cx * cx + cy * cy + cz * cz;
+                        const double bb = cx * dx + cy * dy + cz * dz;
+                        const double disc = bb * bb - a * cc;
+                        double q = (disc >= 0.0) ? (-bb - sqrt(disc)) / a : 0.5;
+                        if (q < 0.0)
+                            q = 0.0;
+                        if (q > 1.0)
+                            q = 1.0;
+                        if (k < g->maxlink) {
+                            size_t sl = (size_t)p * g->maxlink + k;
+                            s->link_node[sl] = (int)n;
+                            s->link_dir[sl] = i;
+                            s->link_q[sl] = q;
+                        } else {
+                            overflow = 1;
+                        }
+                        k++;
+                    }
+                }
+        s->link_count[p] = k < g->maxlink ? k : g->maxlink;
+    }
+    return overflow;
+}
+
+void orc_p_ibb(const orc_pgeom *g, orc_pstate *s, double *L)
+{
+    const size_t N = (size_t)g->lx * g->ly * g->lz;
+    for (int p = 0; p < g->np; p++) {
+        const double *X = s->X + 3 * p, *V = s->V + 3 * p, *W = s->W + 3 * p;
+        double f[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < s->link_count[p]; k++) {
+            size_t sl = (size_t)p * g->maxlink + k;
+            size_t n = (size_t)s->link_node[sl];
+            int i = s->link_dir[sl], ib = ipopp[i];
+            double q = s->link_q[sl];
+            int ix = (int)(n % g->lx), iy = (int)((n / g->lx) % g->ly), iz = (int)(n / ((size_t)g->lx * g->ly));
+            const double cx = (double)cix[i], cy = (double)ciy[i], cz = (double)ciz[i];
+            const double w = i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0;
+            double dx, dy, dz;
+            offset(g, ix, iy, iz, X, &dx, &dy, &dz);
+            const double rx = dx + q * cx, ry = dy + q * cy, rz = dz + q * cz;
+            const double uwx = V[0] + (W[1] * rz - W[2] * ry);
+            const double uwy = V[1] + (W[2] * rx - W[0] * rz);
+            const double uwz = V[2] + (W[0] * ry - W[1] * rx);
+            const double cu = cx * uwx + cy * uwy + cz * uwz;
+            const double fi = L[(size_t)i * N + n];
+            double gval;
+            if (q < 0.5) {
+                int bx = ix - cix[i];
+                int ok = (bx >= 0 && bx < g->lx);
+                size_t nb = 0;
+                if (ok) {
+                    nb = nodeof(g, bx, wrapi(iy - ciy[i], g->ly), wrapi(iz - ciz[i], g->lz));
+                    ok = s->solid[nb] < 0;
+                }
+                if (ok) {
+                    const double fb = L[(size_t)i * N + nb];
+                    gval = (2.0 * q) * fi + (1.0 - 2.0 * q) * fb - 6.0 * w * cu;
+                } else {
+                    gval = fi - 6.0 * w * cu;
+                }
+            } else {
+                const double fib = L[(size_t)ib * N + n];
+                gval = (1.0 / (2.0 * q)) * fi + ((2.0 * q - 1.0) / (2.0 * q)) * fib - (3.0 / q) * w * cu;
+            }
+            size_t ns = nodeof(g, ix + cix[i], wrapi(iy + ciy[i], g->ly), wrapi(iz + ciz[i], g->lz));
+            L[(size_t)ib * N + ns] = gval;
+            const double ff = fi + w, gf = gval + w;
+            const double px = ff * (cx - uwx) + gf * (cx + uwx);
+            const double py = ff * (cy - uwy) + gf * (cy + uwy);
+            const double pz = ff * (cz - uwz) + gf * (cz + uwz);
+            const double c[6] = {px, py, pz, ry * pz - rz * py, rz * px - rx * pz, rx * py - ry * px};
+            for (int j = 0; j < 6; j++)
+                f[j] = f[j] + c[j];
+        }
+        for (int j = 0; j < 3; j++) {
+            s->FH[3 * p + j] = f[j];
+            s->TQ[3 * p + j] = f[3 + j];
+        }
+    }
+}
+
+void orc_p_step(const orc_params *pp, const orc_pgeom *g, const int *solid, const double *Lin, double *Lout,
+                double fx, double fy, double fz)
+{
+    const int lx = g->lx, ly = g->ly, lz = g->lz;
+    const size_t N = (size_t)lx * ly * lz;
+    for (int iz = 0; iz < lz; iz++)
+        for (int iy = 0; iy < ly; iy++)
+            for (int ix = 0; ix < lx; ix++) {
+                size_t n = nodeof(g, ix, iy, iz);
+                if (solid && solid[n] >= 0)
+                    continue;
+                double f[NPOP], post[NPOP];
+                for (int i = 0; i < NPOP; i++) {
+                    int sx = ix - cix[i];
+                    if (sx < 0 || sx >= lx)
+                        f[i] = Lin[(size_t)ipopp[i] * N + n];
+                    else
+                        f[i] = Lin[(size_t)i * N + nodeof(g, sx, wrapi(iy - ciy[i], ly), wrapi(iz - ciz[i], lz))];
+                }
+                /* macrovar, Channel-Flow/collision.f90:398-417 */
+                double sum1 = f[7] - f[10], sum2 = f[9] - f[8], sum3 = f[11] - f[14], sum4 = f[13] - f[12];
+                double sum5 = f[15] - f[18], sum6 = f[17] - f[16];
+                double ux9 = f[1] - f[2] + sum1 + sum2 + sum3 + sum4;
+                double uy9 = f[3] - f[4] + sum1 - sum2 + sum5 + sum6;
+                double uz9 = f[5] - f[6] + sum3 - sum4 + sum5 - sum6;
+                double rho9 = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] +
+                              f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+                ux9 = ux9 + fx / 2.;
+                uy9 = uy9 + fy / 2.;
+                uz9 = uz9 + fz / 2.;
+                orc_collide_node(pp, f, rho9, ux9, uy9, uz9, fx, fy, fz, post);
+                for (int i = 0; i < NPOP; i++)
+                    Lout[(size_t)i * N + n] = post[i];
+            }
+}
+
+void orc_p_move(const orc_pgeom *g, orc_pstate *s)
+{
+    const double Ly = (double)g->ly, Lz = (double)g->lz, Lx = (double)g->lx;
+    double *Xn = malloc(3 * (size_t)g->np * sizeof(double));
+    const int first = (s->nsteps_done == 0);
+    for (int p = 0; p < g->np; p++) {
+        const double *Xp = s->X + 3 * p;
+        double fc[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < g->np; q++) {
+            if (q == p)
+                continue;
+            const double *Xq = s->X + 3 * q;
+            const double ddx = Xp[0] - Xq[0];
+            const double ddy = min_image(Xp[1] - Xq[1], Ly);
+            const double ddz = min_image(Xp[2] - Xq[2], Lz);
+            const double dist = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            const double h = dist - 2.0 * g->R;
+            if (h < g->mingap && dist > 0.0) {
+                const double r = (g->mingap - h) / g->mingap;
+                const double mag = g->stf * (r * r);
+                fc[0] = fc[0] + mag * (ddx / dist);
+                fc[1] = fc[1] + mag * (ddy / dist);
+                fc[2] = fc[2] + mag * (ddz / dist);
+            }
+        }
+        const double hlo = Xp[0] - g->R;
+        if (hlo < g->mingap_w) {
+            const double r = (g->mingap_w - hlo) / g->mingap_w;
+            fc[0] = fc[0] + g->stf_w * (r * r);
+        }
+        const double hhi = Lx - Xp[0] - g->R;
+        if (hhi < g->mingap_w) {
+            const double r = (g->mingap_w - hhi) / g->mingap_w;
+            fc[0] = fc[0] - g->stf_w * (r * r);
+        }
+        const double fb[3] = {g->fx, g->fy, g->fz};
+        for (int j = 0; j < 3; j++) {
+            s->FC[3 * p + j] = fc[j];
+            double ftot = s->FH[3 * p + j] + fc[j];
+            if (g->pressure_force)
+                ftot = ftot + g->volume * fb[j];
+            const double a = ftot / g->mass;
+            const double al = s->TQ[3 * p + j] / g->inertia;
+            const double v0 = s->V[3 * p + j], w0 = s->W[3 * p + j];
+            const double vn = first ? v0 + a : v0 + (1.5 * a - 0.5 * s->Aprev[3 * p + j]);
+            const double wn = first ? w0 + al : w0 + (1.5 * al - 0.5 * s->ALprev[3 * p + j]);
+            Xn[3 * p + j] = Xp[j] + 0.5 * (v0 + vn);
+            s->TH[3 * p + j] = s->TH[3 * p + j] + 0.5 * (w0 + wn);
+            s->Aprev[3 * p + j] = a;
+            s->ALprev[3 * p + j] = al;
+            s->V[3 * p + j] = vn;
+            s->W[3 * p + j] = wn;
+        }
+        if (Xn[3 * p + 1] >= Ly)
+            Xn[3 * p + 1] = Xn[3 * p + 1] - Ly;
+        else if (Xn[3 * p + 1] < 0.0)
+            Xn[3 * p + 1] = Xn[3 * p + 1] + Ly;
+        if (Xn[3 * p + 2] >= Lz)
+            Xn[3 * p + 2] = Xn[3 * p + 2] - Lz;
+        else if (Xn[3 * p + 2] < 0.0)
+            Xn[3 * p + 2] = Xn[3 * p + 2] + Lz;
+    }
+    memcpy(s->Xold, s->X, 3 * (size_t)g->np * sizeof(double));
+    memcpy(s->X, Xn, 3 * (size_t)g->np * sizeof(double));
+    s->nsteps_done++;
+    free(Xn);
+}
+
+void orc_p_refill(const orc_pgeom *g, orc_pstate *s, double *L)
+{
+    const size_t N = (size_t)g->lx * g->ly * g->lz;
+    for (int p = 0; p < g->np; p++) {
+        const double *X = s->X + 3 * p, *V = s->V + 3 * p, *W = s->W + 3 * p;
+        for (int k = 0; k < s->fill_count[p]; k++) {
+            size_t n = (size_t)s->fill_node[(size_t)p * g->maxfill + k];
+            if (s->solid[n] != -2 - p)
+                continue;
+            int ix = (int)(n % g->lx), iy = (int)((n / g->lx) % g->ly), iz = (int)(n / ((size_t)g->lx * g->ly));
+            double rsum = 0.0;
+            int cnt = 0;
+            for (int j = 1; j < NPOP; j++) {
+                int jx = ix + cix[j];
+                if (jx < 0 || jx >= g->lx)
+                    continue;
+                size_t m = nodeof(g, jx, wrapi(iy + ciy[j], g->ly), wrapi(iz + ciz[j], g->lz));
+                if (s->solid[m] != -1)
+                    continue;
+                double r = L[m];
+                for (int i = 1; i < NPOP; i++)
+                    r = r + L[(size_t)i * N + m];
+                rsum = rsum + r;
+                cnt++;
+            }
+            const double rho = cnt > 0 ? rsum / (double)cnt : 0.0;
+            double dx, dy, dz;
+            offset(g, ix, iy, iz, X, &dx, &dy, &dz);
+            const double ux = V[0] + (W[1] * dz - W[2] * dy);
+            const double uy = V[1] + (W[2] * dx - W[0] * dz);
+            const double uz = V[2] + (W[0] * dy - W[1] * dx);
+            double usqr = ux * ux + uy * uy + uz * uz;
+            usqr = 1.5 * usqr;
+            L[n] = (1.0 / 3.0) * (rho - usqr);
+            for (int i = 1; i < NPOP; i++) {
+                const double G = (double)cix[i] * ux + (double)ciy[i] * uy + (double)ciz[i] * uz;
+                const double w = i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0;
+                L[(size_t)i * N + n] = w * (rho + 3.0 * G + 4.5 * G * G - usqr);
+            }
+        }
+    }
+    /* uncovered nodes settle after every refill has read its neighbours */
+    for (int p = 0; p < g->np; p++)
+        for (int k = 0; k < s->fill_count[p]; k++) {
+            size_t n = (size_t)s->fill_node[(size_t)p * g->maxfill + k];
+            if (s->solid[n] == -2 - p)
+                s->solid[n] = -1;
+        }
+}
