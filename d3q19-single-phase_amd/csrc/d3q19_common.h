@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace d3q19 {
 
 // Velocity set in d'Humieres et al. (2002) order, Channel-Flow/para.f90:201-206.
@@ -240,11 +242,20 @@ __device__ __forceinline__ void macro_sums(const T (&f9)[19], T &rho9, T &ux9, T
            f9[12] + f9[13] + f9[14] + f9[15] + f9[16] + f9[17] + f9[18];
 }
 
+// No closure: the shear rates stay s9 and s13 of para.f90:106-108.
+struct NoClosure {
+    template <typename T>
+    __device__ __forceinline__ void operator()(const T (&)[9], T &, T &) const {}
+};
+
 // One MRT collision with Guo forcing, collision.f90:60-220.  On entry f holds
 // the node's populations; on exit the stored post-collision values
 // f9(ip) + 0.5*Fbar(ip) (the values the swap loop writes, :230-262).
-template <typename T>
-__device__ __forceinline__ void mrt_collide(T (&f)[19], T rho9, T ux9, T uy9, T uz9, const Rates<T> &r)
+// A closure sees the non-equilibrium parts of the nine moments relaxed with
+// s1, s2, s9, s10, s13 and may replace this node's s9 and s13.
+template <typename T, typename Closure = NoClosure>
+__device__ __forceinline__ void mrt_collide(T (&f)[19], T rho9, T ux9, T uy9, T uz9, const Rates<T> &r,
+                                            const Closure &closure = Closure())
 {
     const T fx9 = r.fx, fy9 = r.fy, fz9 = r.fz;
     const T ux9s = ux9 * ux9; // :60-62
@@ -325,19 +336,26 @@ __device__ __forceinline__ void mrt_collide(T (&f)[19], T rho9, T ux9, T uy9, T 
     const T evlm14 = -f9[7] - f9[8] + f9[9] + f9[10] + f9[15] - f9[16] + f9[17] - f9[18];
     const T evlm15 = f9[11] + f9[12] - f9[13] - f9[14] - f9[15] - f9[16] + f9[17] + f9[18];
 
+    T s9 = r.s9, s13 = r.s13;
+    if constexpr (!std::is_same<Closure, NoClosure>::value) {
+        const T neq[9] = {evlm1 - eqm1, evlm2 - eqm2, evlm6 - eqm6, evlm7 - eqm7, evlm8 - eqm8,
+                          evlm9 - eqm9, evlm10 - eqm10, evlm11 - eqm11, evlm12 - eqm12};
+        closure(neq, s9, s13);
+    }
+
     // :140-154 relaxation (eqm13..15 = 0)
     const T eqmc1 = evlm1 - r.s1 * (evlm1 - eqm1);
     const T eqmc2 = evlm2 - r.s2 * (evlm2 - eqm2);
     const T eqmc3 = evlm3 - r.s4 * (evlm3 - eqm3);
     const T eqmc4 = evlm4 - r.s4 * (evlm4 - eqm4);
     const T eqmc5 = evlm5 - r.s4 * (evlm5 - eqm5);
-    const T eqmc6 = evlm6 - r.s9 * (evlm6 - eqm6);
+    const T eqmc6 = evlm6 - s9 * (evlm6 - eqm6);
     const T eqmc7 = evlm7 - r.s10 * (evlm7 - eqm7);
-    const T eqmc8 = evlm8 - r.s9 * (evlm8 - eqm8);
+    const T eqmc8 = evlm8 - s9 * (evlm8 - eqm8);
     const T eqmc9 = evlm9 - r.s10 * (evlm9 - eqm9);
-    const T eqmc10 = evlm10 - r.s13 * (evlm10 - eqm10);
-    const T eqmc11 = evlm11 - r.s13 * (evlm11 - eqm11);
-    const T eqmc12 = evlm12 - r.s13 * (evlm12 - eqm12);
+    const T eqmc10 = evlm10 - s13 * (evlm10 - eqm10);
+    const T eqmc11 = evlm11 - s13 * (evlm11 - eqm11);
+    const T eqmc12 = evlm12 - s13 * (evlm12 - eqm12);
     const T eqmc13 = evlm13 - r.s16 * (evlm13 - T(0.0));
     const T eqmc14 = evlm14 - r.s16 * (evlm14 - T(0.0));
     const T eqmc15 = evlm15 - r.s16 * (evlm15 - T(0.0));

@@ -24,6 +24,9 @@ struct StepArgs {
     const int *solid; // node flags (>= 0: inside a particle) for the particle modes
     double rho_off;
     RatesD r;
+    const float *mlp;  // learned closure parameters (mode bit 8), see mlp.h
+    float inv_ystar;   // u*/nu: y+ per lattice unit of wall distance
+    float yscale;      // y+ feature divisor
 };
 
 struct MacroArgs {

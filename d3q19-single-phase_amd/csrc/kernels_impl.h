@@ -18,6 +18,7 @@
 
 #include "d3q19_common.h"
 #include "kernels.h"
+#include "mlp.h"
 
 namespace d3q19 {
 
@@ -105,6 +106,8 @@ __device__ __forceinline__ uint32_t node_index(const Geom &g)
 // MODE + 4: nodes inside a particle are skipped (not collided, not stored);
 // fluid nodes pull the interpolated bounce-back values written into the
 // particle nodes' slots before the step.
+// MODE + 8: the learned closure (mlp.h) sets each node's shear rates
+// (FP32 compute only).
 template <int P, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
 {
@@ -153,7 +156,13 @@ __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
         if constexpr ((MODE & 3) == 1)
             rho9 = rho9 - (T)a.rho_off; // collision.f90:508
     }
-    mrt_collide(f, rho9, ux9, uy9, uz9, r);
+    if constexpr ((MODE & 8) != 0) {
+        const uint32_t ix = n - fdiv(n, a.g.dlx) * a.g.lx;
+        const float wd = fminf((float)ix + 0.5f, (float)a.g.lx - (float)ix - 0.5f);
+        mrt_collide(f, rho9, ux9, uy9, uz9, r, MlpClosure{a.mlp, wd * a.inv_ystar / a.yscale});
+    } else {
+        mrt_collide(f, rho9, ux9, uy9, uz9, r);
+    }
     store_all<P>(dst, a.g.N, n, f);
 }
 
@@ -441,6 +450,18 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
         break;
     case 6:
         hipLaunchKernelGGL((k_step<P, 6>), grid, blk, 0, s, a);
+        break;
+    case 8:
+        if constexpr (P == 0)
+            return hipErrorInvalidValue;
+        else
+            hipLaunchKernelGGL((k_step<P, 8>), grid, blk, 0, s, a);
+        break;
+    case 12:
+        if constexpr (P == 0)
+            return hipErrorInvalidValue;
+        else
+            hipLaunchKernelGGL((k_step<P, 12>), grid, blk, 0, s, a);
         break;
     default:
         return hipErrorInvalidValue;

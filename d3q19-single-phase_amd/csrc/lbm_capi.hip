@@ -25,6 +25,7 @@
 #include "kernels.h"
 #include "lbm.h"
 #include "particles.h"
+#include "mlp_train.h"
 
 using namespace d3q19;
 
@@ -78,6 +79,19 @@ struct lbm_channel {
     double *stage;      // COPY_NODES*19 doubles
     hipEvent_t ev[2];
     struct Part *part;  // rigid particles (nullptr without)
+    float *mlp;         // learned closure parameters on the device (nullptr: plain MRT)
+    float mlp_inv_ystar, mlp_yscale;
+};
+
+struct lbm_trainer {
+    int device;
+    hipStream_t stream;
+    long long capacity, count;
+    float *feat, *target;   // capacity x 10, capacity
+    float *w, *m, *v;       // LBM_MLP_TOTAL parameters, Adam moments
+    float *partial, *grad;  // per-wave partial gradients, summed gradient (+ loss)
+    uint32_t max_waves;
+    int step;
 };
 
 // Particle set on the device (see particles.h).
@@ -238,6 +252,14 @@ int launch_step(lbm_channel *h)
     if (h->part) {
         mode |= 4;
         a.solid = h->part->ps.solid;
+    }
+    a.mlp = h->mlp;
+    a.inv_ystar = h->mlp_inv_ystar;
+    a.yscale = h->mlp_yscale;
+    if (h->mlp) {
+        if (mode & 3) // the closure runs with fused fields only
+            return fail(LBM_ERR_STATE, "collision: the learned closure needs fields from macrovar");
+        mode |= 8;
     }
     int rc = dispatch_err(PREC_CALL(h, step, a, mode, h->stream), "collision");
     if (rc)
@@ -479,6 +501,8 @@ int lbm_destroy(lbm_channel *h)
             (void)hipFree(p);
         delete h->part;
     }
+    if (h->mlp)
+        (void)hipFree(h->mlp);
     if (h->ev[0])
         (void)hipEventDestroy(h->ev[0]);
     if (h->ev[1])
@@ -1118,6 +1142,273 @@ int lbm_particles_solid(lbm_channel *h, int *solid)
         return fail(LBM_ERR_ARG, "lbm_particles_solid: null output");
     HIP_TRY(hipMemcpyAsync(solid, h->part->ps.solid, h->g.N * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int lbm_closure_set(lbm_channel *h, const float *params, double ystar)
+{
+    CHECK_H(h);
+    if (!params || !(ystar > 0.0))
+        return fail(LBM_ERR_ARG, "lbm_closure_set: need parameters and ystar > 0");
+    if (h->prec == LBM_FP64)
+        return fail(LBM_ERR_ARG, "lbm_closure_set: the closure runs in FP32 arithmetic (FP32/FP16 storage)");
+    if (h->part)
+        return fail(LBM_ERR_STATE, "lbm_closure_set: not combined with particles");
+    if (!h->mlp)
+        HIP_TRY(hipMalloc((void **)&h->mlp, MLP_TOTAL * sizeof(float)));
+    HIP_TRY(hipMemcpyAsync(h->mlp, params, MLP_TOTAL * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->mlp_inv_ystar = (float)(1.0 / ystar);
+    h->mlp_yscale = params[MLP_SCALE + 9];
+    return LBM_OK;
+}
+
+int lbm_closure_clear(lbm_channel *h)
+{
+    CHECK_H(h);
+    if (h->mlp) {
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        (void)hipFree(h->mlp);
+        h->mlp = nullptr;
+    }
+    return LBM_OK;
+}
+
+#define CHECK_T(t)                                                   \
+    do {                                                             \
+        if (!(t))                                                    \
+            return fail(LBM_ERR_ARG, "%s: null trainer", __func__);  \
+        g_err[0] = 0;                                                \
+        HIP_TRY(hipSetDevice((t)->device));                          \
+    } while (0)
+
+int lbm_trainer_create(int device, long long capacity, const float *params, lbm_trainer **out)
+{
+    g_err[0] = 0;
+    if (!out || !params || capacity < 1 || capacity > (1ll << 31))
+        return fail(LBM_ERR_ARG, "lbm_trainer_create: need params, out and 1 <= capacity <= 2^31");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(device));
+    lbm_trainer *t = new lbm_trainer();
+    memset(t, 0, sizeof(*t));
+    t->device = device;
+    t->capacity = capacity;
+    t->max_waves = 1u << 16;
+    auto cleanup = [&](int code) {
+        lbm_trainer_destroy(t);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(LBM_ERR_HIP, "lbm_trainer_create: stream creation failed"));
+    if (hipMalloc((void **)&t->feat, (size_t)capacity * MLP_IN * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&t->target, (size_t)capacity * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&t->w, MLP_TOTAL * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&t->m, MLP_NPARAM * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&t->v, MLP_NPARAM * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&t->partial, (size_t)t->max_waves * (MLP_NPARAM + 1) * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&t->grad, (MLP_NPARAM + 1) * sizeof(float)) != hipSuccess)
+        return cleanup(fail(LBM_ERR_NOMEM, "lbm_trainer_create: device allocation failed"));
+    if (hipMemcpyAsync(t->w, params, MLP_TOTAL * sizeof(float), hipMemcpyHostToDevice, t->stream) != hipSuccess ||
+        hipMemsetAsync(t->m, 0, MLP_NPARAM * sizeof(float), t->stream) != hipSuccess ||
+        hipMemsetAsync(t->v, 0, MLP_NPARAM * sizeof(float), t->stream) != hipSuccess ||
+        hipStreamSynchronize(t->stream) != hipSuccess)
+        return cleanup(fail(LBM_ERR_HIP, "lbm_trainer_create: initialisation failed"));
+    *out = t;
+    return LBM_OK;
+}
+
+int lbm_trainer_destroy(lbm_trainer *t)
+{
+    if (!t)
+        return LBM_OK;
+    (void)hipSetDevice(t->device);
+    if (t->stream)
+        (void)hipStreamSynchronize(t->stream);
+    for (void *p : {(void *)t->feat, (void *)t->target, (void *)t->w, (void *)t->m, (void *)t->v, (void *)t->partial,
+                    (void *)t->grad})
+        if (p)
+            (void)hipFree(p);
+    if (t->stream)
+        (void)hipStreamDestroy(t->stream);
+    delete t;
+    return LBM_OK;
+}
+
+int lbm_trainer_add_filtered(lbm_trainer *t, lbm_channel *fine, double nu_coarse, double s9_coarse,
+                             double ystar_coarse, long long *added)
+{
+    CHECK_T(t);
+    if (!fine)
+        return fail(LBM_ERR_ARG, "lbm_trainer_add_filtered: null channel");
+    if (fine->cfg.device != t->device)
+        return fail(LBM_ERR_ARG, "lbm_trainer_add_filtered: channel and trainer on different devices");
+    if ((fine->g.lx | fine->g.ly | fine->g.lz) & 1)
+        return fail(LBM_ERR_ARG, "lbm_trainer_add_filtered: fine grid %u x %u x %u must be even", fine->g.lx,
+                    fine->g.ly, fine->g.lz);
+    if (fine->part)
+        return fail(LBM_ERR_ARG, "lbm_trainer_add_filtered: not for particle-laden channels");
+    const uint32_t lxc = fine->g.lx / 2, lyc = fine->g.ly / 2, lzc = fine->g.lz / 2;
+    const long long Nc = (long long)lxc * lyc * lzc;
+    if (t->count + Nc > t->capacity)
+        return fail(LBM_ERR_ARG, "lbm_trainer_add_filtered: %lld samples would exceed the capacity %lld",
+                    t->count + Nc, t->capacity);
+    HIP_TRY(hipStreamSynchronize(fine->stream));
+    float *fields = nullptr;
+    HIP_TRY(hipMalloc((void **)&fields, (size_t)Nc * 10 * sizeof(float)));
+    FilterArgs fa;
+    fa.gf = fine->g;
+    fa.L = fine->lat[fine->cur];
+    fa.fx = fine->force[0];
+    fa.fy = fine->force[1];
+    fa.fz = fine->force[2];
+    fa.lxc = lxc;
+    fa.lyc = lyc;
+    fa.lzc = lzc;
+    fa.out = fields;
+    hipError_t e = fine->prec == LBM_FP64   ? MlpLaunch<0>::filter(fa, t->stream)
+                   : fine->prec == LBM_FP32 ? MlpLaunch<1>::filter(fa, t->stream)
+                                            : MlpLaunch<2>::filter(fa, t->stream);
+    FeatArgs ft;
+    ft.lx = lxc;
+    ft.ly = lyc;
+    ft.lz = lzc;
+    ft.fields = fields;
+    ft.nu = (float)nu_coarse;
+    ft.s9 = (float)s9_coarse;
+    ft.inv_ystar = (float)(1.0 / ystar_coarse);
+    std::vector<float> hp(MLP_TOTAL);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hp.data(), t->w, MLP_TOTAL * sizeof(float), hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(t->stream);
+    ft.yscale = hp[MLP_SCALE + 9];
+    ft.scale = t->w + MLP_SCALE;
+    ft.feat = t->feat;
+    ft.target = t->target;
+    ft.offset = (uint64_t)t->count;
+    if (e == hipSuccess)
+        e = mlp_features(ft, t->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(t->stream);
+    (void)hipFree(fields);
+    if (e != hipSuccess)
+        return fail(LBM_ERR_HIP, "lbm_trainer_add_filtered: %s", hipGetErrorString(e));
+    t->count += Nc;
+    if (added)
+        *added = Nc;
+    return LBM_OK;
+}
+
+int lbm_trainer_add_samples(lbm_trainer *t, const float *features, const float *targets, long long n)
+{
+    CHECK_T(t);
+    if (!features || !targets || n < 0 || t->count + n > t->capacity)
+        return fail(LBM_ERR_ARG, "lbm_trainer_add_samples: bad arguments or capacity exceeded");
+    HIP_TRY(hipMemcpyAsync(t->feat + (size_t)t->count * MLP_IN, features, (size_t)n * MLP_IN * sizeof(float),
+                           hipMemcpyHostToDevice, t->stream));
+    HIP_TRY(hipMemcpyAsync(t->target + t->count, targets, (size_t)n * sizeof(float), hipMemcpyHostToDevice,
+                           t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    t->count += n;
+    return LBM_OK;
+}
+
+int lbm_trainer_samples(lbm_trainer *t, long long *n, float *features, float *targets)
+{
+    CHECK_T(t);
+    if (n)
+        *n = t->count;
+    if (features)
+        HIP_TRY(hipMemcpyAsync(features, t->feat, (size_t)t->count * MLP_IN * sizeof(float), hipMemcpyDeviceToHost,
+                               t->stream));
+    if (targets)
+        HIP_TRY(hipMemcpyAsync(targets, t->target, (size_t)t->count * sizeof(float), hipMemcpyDeviceToHost,
+                               t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    return LBM_OK;
+}
+
+static int trainer_range(lbm_trainer *t, long long first, long long count, const char *fn)
+{
+    if (first < 0 || count < 1 || first + count > t->count)
+        return fail(LBM_ERR_ARG, "%s: samples [%lld, %lld) outside [0, %lld)", fn, first, first + count, t->count);
+    if ((count + 63) / 64 > (long long)t->max_waves)
+        return fail(LBM_ERR_ARG, "%s: batch of %lld samples exceeds %u waves", fn, count, t->max_waves);
+    return LBM_OK;
+}
+
+int lbm_trainer_grad(lbm_trainer *t, long long first, long long count, float *grad, float *loss)
+{
+    CHECK_T(t);
+    int rc = trainer_range(t, first, count, "lbm_trainer_grad");
+    if (rc)
+        return rc;
+    HIP_TRY(mlp_grad(t->w, t->feat, t->target, (uint32_t)first, (uint32_t)count, t->partial, t->grad, t->w, t->m,
+                     t->v, 1, 0.0f, 0, t->stream));
+    std::vector<float> g(MLP_NPARAM + 1);
+    HIP_TRY(hipMemcpyAsync(g.data(), t->grad, g.size() * sizeof(float), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    if (grad)
+        memcpy(grad, g.data(), MLP_NPARAM * sizeof(float));
+    if (loss)
+        *loss = g[MLP_NPARAM];
+    return LBM_OK;
+}
+
+int lbm_trainer_train(lbm_trainer *t, long long first, long long count, int iterations, long long batch, float lr,
+                      float *loss)
+{
+    CHECK_T(t);
+    int rc = trainer_range(t, first, count, "lbm_trainer_train");
+    if (rc)
+        return rc;
+    if (iterations < 1 || batch < 1 || batch > count)
+        return fail(LBM_ERR_ARG, "lbm_trainer_train: need iterations >= 1 and 1 <= batch <= count");
+    rc = trainer_range(t, first, batch, "lbm_trainer_train");
+    if (rc)
+        return rc;
+    long long pos = 0;
+    for (int it = 0; it < iterations; it++) {
+        if (pos + batch > count)
+            pos = 0;
+        t->step++;
+        HIP_TRY(mlp_grad(t->w, t->feat, t->target, (uint32_t)(first + pos), (uint32_t)batch, t->partial, t->grad,
+                         t->w, t->m, t->v, t->step, lr, 1, t->stream));
+        pos += batch;
+    }
+    float g[MLP_NPARAM + 1];
+    HIP_TRY(hipMemcpyAsync(g, t->grad, sizeof(g), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    if (loss)
+        *loss = g[MLP_NPARAM];
+    return LBM_OK;
+}
+
+int lbm_trainer_predict(lbm_trainer *t, long long first, long long count, float *out)
+{
+    CHECK_T(t);
+    if (!out || first < 0 || count < 1 || first + count > t->count)
+        return fail(LBM_ERR_ARG, "lbm_trainer_predict: bad range or null output");
+    float *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, (size_t)count * sizeof(float)));
+    hipError_t e = mlp_predict(t->w, t->feat, (uint32_t)first, (uint32_t)count, d, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d, (size_t)count * sizeof(float), hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(t->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return fail(LBM_ERR_HIP, "lbm_trainer_predict: %s", hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int lbm_trainer_params(lbm_trainer *t, float *params)
+{
+    CHECK_T(t);
+    if (!params)
+        return fail(LBM_ERR_ARG, "lbm_trainer_params: null output");
+    HIP_TRY(hipMemcpyAsync(params, t->w, MLP_TOTAL * sizeof(float), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
     return LBM_OK;
 }
 

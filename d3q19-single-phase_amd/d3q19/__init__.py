@@ -87,7 +87,22 @@ _SIGS = {
     "lbm_particles_get": [_H, _D, _D, _D, _D, _D, _D],
     "lbm_particles_links": [_H, ctypes.POINTER(ctypes.c_int), _D, _D, _D, _D],
     "lbm_particles_solid": [_H, _D],
+    "lbm_closure_set": [_H, _D, ctypes.c_double],
+    "lbm_closure_clear": [_H],
+    "lbm_trainer_create": [ctypes.c_int, ctypes.c_longlong, _D, ctypes.POINTER(ctypes.c_void_p)],
+    "lbm_trainer_destroy": [_H],
+    "lbm_trainer_add_filtered": [_H, _H, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.POINTER(ctypes.c_longlong)],
+    "lbm_trainer_add_samples": [_H, _D, _D, ctypes.c_longlong],
+    "lbm_trainer_samples": [_H, ctypes.POINTER(ctypes.c_longlong), _D, _D],
+    "lbm_trainer_grad": [_H, ctypes.c_longlong, ctypes.c_longlong, _D, ctypes.POINTER(ctypes.c_float)],
+    "lbm_trainer_train": [_H, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int, ctypes.c_longlong, ctypes.c_float,
+                          ctypes.POINTER(ctypes.c_float)],
+    "lbm_trainer_predict": [_H, ctypes.c_longlong, ctypes.c_longlong, _D],
+    "lbm_trainer_params": [_H, _D],
 }
+
+MLP_NPARAM, MLP_TOTAL, MLP_IN, MLP_H = 1441, 1452, 10, 32
 
 _lib = None
 
@@ -127,6 +142,40 @@ def _ptr(a):
         return None
     assert a.dtype == np.float64 and a.flags.c_contiguous, "arrays must be C-contiguous float64"
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous, "arrays must be C-contiguous float32"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def closure_params(seed=0, scales=None, yscale=100.0, nu=0.0018, init="small"):
+    """Parameter vector of the learned closure (include/lbm.h: W1, b1, W2, b2, W3, b3,
+    9 input scales, y+ divisor, nu).  init="small": Xavier-uniform weights from
+    SplitMix64 stream `seed`, output bias -4 (nu_t ~ 0.018 nu at start)."""
+    from .workloads import uniform
+    p = np.zeros(MLP_TOTAL, dtype=np.float32)
+    r = uniform(1000 + seed, MLP_NPARAM)
+    o = 0
+
+    def take(n, fan_in, fan_out):
+        nonlocal o
+        lim = np.sqrt(6.0 / (fan_in + fan_out))
+        v = (2.0 * r[o:o + n] - 1.0) * lim
+        o += n
+        return v
+
+    if init == "small":
+        p[0:320] = take(320, 10, 32)
+        p[352:352 + 1024] = take(1024, 32, 32)
+        p[1408:1440] = take(32, 32, 1)
+        p[1440] = -4.0
+    p[1441:1450] = 1.0 if scales is None else np.asarray(scales, dtype=np.float32)
+    p[1450] = yscale
+    p[1451] = nu
+    return p
 
 
 class Channel:
@@ -336,7 +385,90 @@ class Channel:
               node.ctypes.data_as(ctypes.c_void_p), dirs.ctypes.data_as(ctypes.c_void_p), _ptr(q))
         return cap, counts, node, dirs, q
 
+    # -- learned closure ------------------------------------------------------
+    def closure_set(self, params, ystar=None):
+        """Evaluate the learned closure inside every collision (FP32/FP16 storage)."""
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.size == MLP_TOTAL
+        _call("lbm_closure_set", self._h, _fptr(params), float(self.p["ystar"] if ystar is None else ystar))
+
+    def closure_clear(self):
+        _call("lbm_closure_clear", self._h)
+
     def particles_solid(self):
         s = np.empty(self.N, dtype=np.int32)
         _call("lbm_particles_solid", self._h, s.ctypes.data_as(ctypes.c_void_p))
         return s
+
+
+class Trainer:
+    """Sample store and network of the learned closure on one device (include/lbm.h)."""
+
+    def __init__(self, capacity, params, device=0):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        h = ctypes.c_void_p()
+        _call("lbm_trainer_create", int(device), int(capacity), _fptr(params), ctypes.byref(h))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _call("lbm_trainer_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_filtered(self, fine, nu_coarse, s9_coarse, ystar_coarse):
+        n = ctypes.c_longlong()
+        _call("lbm_trainer_add_filtered", self._h, fine._h, float(nu_coarse), float(s9_coarse), float(ystar_coarse),
+              ctypes.byref(n))
+        return n.value
+
+    def add_samples(self, features, targets):
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        _call("lbm_trainer_add_samples", self._h, _fptr(f), _fptr(t), int(t.size))
+
+    def samples(self):
+        n = ctypes.c_longlong()
+        _call("lbm_trainer_samples", self._h, ctypes.byref(n), None, None)
+        f = np.empty((n.value, MLP_IN), dtype=np.float32)
+        t = np.empty(n.value, dtype=np.float32)
+        _call("lbm_trainer_samples", self._h, ctypes.byref(n), _fptr(f), _fptr(t))
+        return f, t
+
+    def count(self):
+        n = ctypes.c_longlong()
+        _call("lbm_trainer_samples", self._h, ctypes.byref(n), None, None)
+        return n.value
+
+    def grad(self, first, count):
+        g = np.empty(MLP_NPARAM, dtype=np.float32)
+        loss = ctypes.c_float()
+        _call("lbm_trainer_grad", self._h, int(first), int(count), _fptr(g), ctypes.byref(loss))
+        return loss.value, g
+
+    def train(self, first, count, iterations, batch, lr):
+        loss = ctypes.c_float()
+        _call("lbm_trainer_train", self._h, int(first), int(count), int(iterations), int(batch), float(lr),
+              ctypes.byref(loss))
+        return loss.value
+
+    def predict(self, first, count):
+        out = np.empty(count, dtype=np.float32)
+        _call("lbm_trainer_predict", self._h, int(first), int(count), _fptr(out))
+        return out
+
+    def params(self):
+        p = np.empty(MLP_TOTAL, dtype=np.float32)
+        _call("lbm_trainer_params", self._h, _fptr(p))
+        return p

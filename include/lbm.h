@@ -215,6 +215,47 @@ int lbm_particles_links(lbm_channel *h, int *maxlink, int *counts, int *node, in
 /* Node flags (N ints): -1 fluid, p >= 0 inside particle p. */
 int lbm_particles_solid(lbm_channel *h, int *solid);
 
+/* ---------------------------------------------------------------------------
+ * Learned relaxation-rate closure (no counterpart in the reference: the
+ * target of a 2x-coarsened channel that reproduces filtered fine statistics).
+ * A 10-32-32-1 tanh network maps the nine non-equilibrium moments the MRT
+ * relaxes with s1, s2, s9, s10, s13 (scaled) and y+ to an eddy viscosity
+ * nu_t = nu * softplus(o); the node's shear rates become
+ * s9 = s13 = 1 / (3 (nu + nu_t) + 0.5).  Parameters, FP32, LBM_MLP_TOTAL
+ * values: W1[32][10], b1[32], W2[32][32], b2[32], W3[32], b3 (the
+ * LBM_MLP_NPARAM trained values), then 9 input scales, the y+ divisor, nu.
+ */
+#define LBM_MLP_NPARAM 1441
+#define LBM_MLP_TOTAL 1452
+
+/* Evaluates the closure inside every following collision_MRT (FP32 and FP16
+ * storage only; FP64 stays the reference's plain MRT).  ystar = nu/u* of the
+ * run (para.f90:66). */
+int lbm_closure_set(lbm_channel *h, const float *params, double ystar);
+/* Back to plain MRT. */
+int lbm_closure_clear(lbm_channel *h);
+
+/* Trainer: a sample store and the network on one device.  Samples are
+ * (10 features, target nu_t/nu).  lbm_trainer_add_filtered box-filters the
+ * fine channel's current lattice 2x2x2 and appends one sample per coarse node
+ * (target: the eddy viscosity nu_t = -tau:S/(2 S:S) of the subgrid stress,
+ * features: the first-order non-equilibrium moments of the coarse MRT with
+ * shear rate s9_coarse, scaled by the parameters' input scales).  Gradients
+ * are of mean((softplus(o) - target)^2) over a sample range; training uses
+ * Adam (beta 0.9, 0.999) over consecutive batches. */
+typedef struct lbm_trainer lbm_trainer;
+int lbm_trainer_create(int device, long long capacity, const float *params, lbm_trainer **out);
+int lbm_trainer_destroy(lbm_trainer *t);
+int lbm_trainer_add_filtered(lbm_trainer *t, lbm_channel *fine, double nu_coarse, double s9_coarse,
+                             double ystar_coarse, long long *added);
+int lbm_trainer_add_samples(lbm_trainer *t, const float *features, const float *targets, long long n);
+int lbm_trainer_samples(lbm_trainer *t, long long *n, float *features, float *targets);
+int lbm_trainer_grad(lbm_trainer *t, long long first, long long count, float *grad, float *loss);
+int lbm_trainer_train(lbm_trainer *t, long long first, long long count, int iterations, long long batch, float lr,
+                      float *loss);
+int lbm_trainer_predict(lbm_trainer *t, long long first, long long count, float *out);
+int lbm_trainer_params(lbm_trainer *t, float *params);
+
 #ifdef __cplusplus
 }
 #endif
