@@ -186,12 +186,12 @@ struct Prec<2> {
 
 // Lattice velocity component times a value: c*a for c in {-1,0,1}.
 template <int C, typename T>
-__device__ __forceinline__ T cmul(T a) { return C > 0 ? a : -a; }
+__host__ __device__ __forceinline__ T cmul(T a) { return C > 0 ? a : -a; }
 
 // cix*a + ciy*b + ciz*c (collision.f90:73-74, 79-80) keeping only the
 // nonzero terms, summed in x, y, z order.
 template <int I, typename T>
-__device__ __forceinline__ T cdot(T a, T b, T c)
+__host__ __device__ __forceinline__ T cdot(T a, T b, T c)
 {
     constexpr int cx = CX[I], cy = CY[I], cz = CZ[I];
     if constexpr (cx != 0 && cy != 0)
@@ -212,7 +212,7 @@ __device__ __forceinline__ T cdot(T a, T b, T c)
 
 // Guo forcing term Fbar_i, collision.f90:70-82.
 template <int I, typename T>
-__device__ __forceinline__ T fbar(T ux9, T uy9, T uz9, T fx9, T fy9, T fz9, T G3)
+__host__ __device__ __forceinline__ T fbar(T ux9, T uy9, T uz9, T fx9, T fy9, T fz9, T G3)
 {
     if constexpr (I == 0) {
         return -G3;
@@ -227,7 +227,7 @@ __device__ __forceinline__ T fbar(T ux9, T uy9, T uz9, T fx9, T fy9, T fz9, T G3
 // macrovar, collision.f90:398-413: momentum sums and density in the
 // reference's order (before adding F/2).
 template <typename T>
-__device__ __forceinline__ void macro_sums(const T (&f9)[19], T &rho9, T &ux9, T &uy9, T &uz9)
+__host__ __device__ __forceinline__ void macro_sums(const T (&f9)[19], T &rho9, T &ux9, T &uy9, T &uz9)
 {
     T sum1 = f9[7] - f9[10];
     T sum2 = f9[9] - f9[8];
@@ -245,25 +245,13 @@ __device__ __forceinline__ void macro_sums(const T (&f9)[19], T &rho9, T &ux9, T
 // No closure: the shear rates stay s9 and s13 of para.f90:106-108.
 struct NoClosure {
     template <typename T>
-    __device__ __forceinline__ void operator()(const T (&)[9], T &, T &) const {}
+    __host__ __device__ __forceinline__ void operator()(const T (&)[9], T &, T &) const {}
 };
-
-// One MRT collision with Guo forcing, collision.f90:60-220.  On entry f holds
-// the node's populations; on exit the stored post-collision values
-// f9(ip) + 0.5*Fbar(ip) (the values the swap loop writes, :230-262).
-// A closure sees the non-equilibrium parts of the nine moments relaxed with
-// s1, s2, s9, s10, s13 and may replace this node's s9 and s13.
-template <typename T, typename Closure = NoClosure>
-__device__ __forceinline__ void mrt_collide(T (&f)[19], T rho9, T ux9, T uy9, T uz9, const Rates<T> &r,
-                                            const Closure &closure = Closure())
+// Guo forcing terms Fbar_0..18 of one node, collision.f90:60-82.
+template <typename T>
+__host__ __device__ __forceinline__ void guo_terms(T ux9, T uy9, T uz9, T fx9, T fy9, T fz9, T (&Fb)[19])
 {
-    const T fx9 = r.fx, fy9 = r.fy, fz9 = r.fz;
-    const T ux9s = ux9 * ux9; // :60-62
-    const T uy9s = uy9 * uy9;
-    const T uz9s = uz9 * uz9;
     const T G3 = ux9 * fx9 + uy9 * fy9 + uz9 * fz9; // :68
-
-    T Fb[19];
     Fb[0] = fbar<0>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
     Fb[1] = fbar<1>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
     Fb[2] = fbar<2>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
@@ -283,28 +271,14 @@ __device__ __forceinline__ void mrt_collide(T (&f)[19], T rho9, T ux9, T uy9, T 
     Fb[16] = fbar<16>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
     Fb[17] = fbar<17>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
     Fb[18] = fbar<18>(ux9, uy9, uz9, fx9, fy9, fz9, G3);
+}
 
-    T f9[19];
-#pragma unroll
-    for (int i = 0; i < 19; i++) // :84
-        f9[i] = f[i] + T(0.5) * Fb[i];
-
-    // :86-101 equilibrium moments (incompressible; no rho divisor)
-    const T t1 = ux9s + uy9s + uz9s;
-    const T eqm1 = -(T(11.0) * rho9) + T(19.0) * t1;
-    const T eqm2 = r.omegepsl * rho9 + r.omegepslj * t1;
-    constexpr T coef1 = T(-(2.0 / 3.0));
-    const T eqm3 = coef1 * ux9;
-    const T eqm4 = coef1 * uy9;
-    const T eqm5 = coef1 * uz9;
-    const T eqm6 = T(2.0) * ux9s - uy9s - uz9s;
-    const T eqm7 = r.omegxx * eqm6;
-    const T eqm8 = uy9s - uz9s;
-    const T eqm9 = r.omegxx * eqm8;
-    const T eqm10 = ux9 * uy9;
-    const T eqm11 = uy9 * uz9;
-    const T eqm12 = ux9 * uz9;
-
+// The 15 non-conserved moments evlm1..15 of f9 (collision.f90:103-138), in
+// d'Humieres et al. (2002) order: e, eps, qx, qy, qz, 3pxx, 3pixx, pww, piww,
+// pxy, pyz, pxz, mx, my, mz.
+template <typename T>
+__host__ __device__ __forceinline__ void moments(const T (&f9)[19], T (&m)[15])
+{
     // :103-119
     const T sum1 = f9[1] + f9[2] + f9[3] + f9[4] + f9[5] + f9[6];
     const T sum2 = f9[7] + f9[8] + f9[9] + f9[10] + f9[11] + f9[12] + f9[13] + f9[14] + f9[15] + f9[16] +
@@ -320,71 +294,78 @@ __device__ __forceinline__ void mrt_collide(T (&f)[19], T rho9, T ux9, T uy9, T 
     const T sum11 = f9[7] + f9[8] + f9[9] + f9[10] - f9[11] - f9[12] - f9[13] - f9[14];
 
     // :121-138 (coef2 = -11, coef3 = 8, coef4 = -4, coef5 = 2)
-    const T evlm1 = -(T(30.0) * f9[0]) + T(-11.0) * sum1 + T(8.0) * sum2;
-    const T evlm2 = T(12.0) * f9[0] + T(-4.0) * sum1 + sum2;
-    const T evlm3 = T(-4.0) * (f9[1] - f9[2]) + sum3;
-    const T evlm4 = T(-4.0) * (f9[3] - f9[4]) + sum4;
-    const T evlm5 = T(-4.0) * (f9[5] - f9[6]) + sum5;
-    const T evlm6 = T(2.0) * sum6 - sum7 + sum8 - T(2.0) * sum9;
-    const T evlm7 = T(-4.0) * sum6 + T(2.0) * sum7 + sum8 - T(2.0) * sum9;
-    const T evlm8 = sum10 + sum11;
-    const T evlm9 = -(T(2.0) * sum10) + sum11;
-    const T evlm10 = f9[7] - f9[8] - f9[9] + f9[10];
-    const T evlm11 = f9[15] - f9[16] - f9[17] + f9[18];
-    const T evlm12 = f9[11] - f9[12] - f9[13] + f9[14];
-    const T evlm13 = f9[7] - f9[8] + f9[9] - f9[10] - f9[11] + f9[12] - f9[13] + f9[14];
-    const T evlm14 = -f9[7] - f9[8] + f9[9] + f9[10] + f9[15] - f9[16] + f9[17] - f9[18];
-    const T evlm15 = f9[11] + f9[12] - f9[13] - f9[14] - f9[15] - f9[16] + f9[17] + f9[18];
+    m[0] = -(T(30.0) * f9[0]) + T(-11.0) * sum1 + T(8.0) * sum2;
+    m[1] = T(12.0) * f9[0] + T(-4.0) * sum1 + sum2;
+    m[2] = T(-4.0) * (f9[1] - f9[2]) + sum3;
+    m[3] = T(-4.0) * (f9[3] - f9[4]) + sum4;
+    m[4] = T(-4.0) * (f9[5] - f9[6]) + sum5;
+    m[5] = T(2.0) * sum6 - sum7 + sum8 - T(2.0) * sum9;
+    m[6] = T(-4.0) * sum6 + T(2.0) * sum7 + sum8 - T(2.0) * sum9;
+    m[7] = sum10 + sum11;
+    m[8] = -(T(2.0) * sum10) + sum11;
+    m[9] = f9[7] - f9[8] - f9[9] + f9[10];
+    m[10] = f9[15] - f9[16] - f9[17] + f9[18];
+    m[11] = f9[11] - f9[12] - f9[13] + f9[14];
+    m[12] = f9[7] - f9[8] + f9[9] - f9[10] - f9[11] + f9[12] - f9[13] + f9[14];
+    m[13] = -f9[7] - f9[8] + f9[9] + f9[10] + f9[15] - f9[16] + f9[17] - f9[18];
+    m[14] = f9[11] + f9[12] - f9[13] - f9[14] - f9[15] - f9[16] + f9[17] + f9[18];
+}
 
-    T s9 = r.s9, s13 = r.s13;
-    if constexpr (!std::is_same<Closure, NoClosure>::value) {
-        const T neq[9] = {evlm1 - eqm1, evlm2 - eqm2, evlm6 - eqm6, evlm7 - eqm7, evlm8 - eqm8,
-                          evlm9 - eqm9, evlm10 - eqm10, evlm11 - eqm11, evlm12 - eqm12};
-        closure(neq, s9, s13);
-    }
+// Equilibria eqm1..12 of the non-conserved moments, collision.f90:86-101
+// (incompressible: no rho divisor; eqm13..15 = 0).
+template <typename T>
+__host__ __device__ __forceinline__ void equilibria(T rho9, T ux9, T uy9, T uz9, const Rates<T> &r, T (&e)[12])
+{
+    const T ux9s = ux9 * ux9; // :60-62
+    const T uy9s = uy9 * uy9;
+    const T uz9s = uz9 * uz9;
+    const T t1 = ux9s + uy9s + uz9s;
+    constexpr T coef1 = T(-(2.0 / 3.0));
+    e[0] = -(T(11.0) * rho9) + T(19.0) * t1;
+    e[1] = r.omegepsl * rho9 + r.omegepslj * t1;
+    e[2] = coef1 * ux9;
+    e[3] = coef1 * uy9;
+    e[4] = coef1 * uz9;
+    e[5] = T(2.0) * ux9s - uy9s - uz9s;
+    e[6] = r.omegxx * e[5];
+    e[7] = uy9s - uz9s;
+    e[8] = r.omegxx * e[7];
+    e[9] = ux9 * uy9;
+    e[10] = uy9 * uz9;
+    e[11] = ux9 * uz9;
+}
 
-    // :140-154 relaxation (eqm13..15 = 0)
-    const T eqmc1 = evlm1 - r.s1 * (evlm1 - eqm1);
-    const T eqmc2 = evlm2 - r.s2 * (evlm2 - eqm2);
-    const T eqmc3 = evlm3 - r.s4 * (evlm3 - eqm3);
-    const T eqmc4 = evlm4 - r.s4 * (evlm4 - eqm4);
-    const T eqmc5 = evlm5 - r.s4 * (evlm5 - eqm5);
-    const T eqmc6 = evlm6 - s9 * (evlm6 - eqm6);
-    const T eqmc7 = evlm7 - r.s10 * (evlm7 - eqm7);
-    const T eqmc8 = evlm8 - s9 * (evlm8 - eqm8);
-    const T eqmc9 = evlm9 - r.s10 * (evlm9 - eqm9);
-    const T eqmc10 = evlm10 - s13 * (evlm10 - eqm10);
-    const T eqmc11 = evlm11 - s13 * (evlm11 - eqm11);
-    const T eqmc12 = evlm12 - s13 * (evlm12 - eqm12);
-    const T eqmc13 = evlm13 - r.s16 * (evlm13 - T(0.0));
-    const T eqmc14 = evlm14 - r.s16 * (evlm14 - T(0.0));
-    const T eqmc15 = evlm15 - r.s16 * (evlm15 - T(0.0));
-
-    // :157-177 inverse transform; val*i = 1/val*, coef3i = 1/8, coef4i = -1/4
-    const T tl1 = T(1.0 / 19.0) * rho9;
-    const T tl2 = T(-11.0 * (1.0 / 2394.0)) * eqmc1;
-    const T tl3 = T(8.0 * (1.0 / 2394.0)) * eqmc1;
-    const T tl4 = T(-4.0 * (1.0 / 252.0)) * eqmc2;
-    const T tl5 = T(1.0 / 252.0) * eqmc2;
-    const T tl6 = T(1.0 / 10.0) * ux9;
-    const T tl7 = T(1.0 / 40.0) * eqmc3;
-    const T tl8 = T(1.0 / 10.0) * uy9;
-    const T tl9 = T(1.0 / 40.0) * eqmc4;
-    const T tl10 = T(1.0 / 10.0) * uz9;
-    const T tl11 = T(1.0 / 40.0) * eqmc5;
-    const T tl12 = T(1.0 / 36.0) * eqmc6;
-    const T tl13 = T(1.0 / 72.0) * eqmc7;
-    const T tl14 = T(1.0 / 12.0) * eqmc8;
-    const T tl15 = T(1.0 / 24.0) * eqmc9;
-    const T tl16 = -(T(-0.25) * eqmc10);
-    const T tl17 = -(T(-0.25) * eqmc11);
-    const T tl18 = -(T(-0.25) * eqmc12);
-    const T tl19 = T(0.125) * eqmc13;
-    const T tl20 = T(0.125) * eqmc14;
-    const T tl21 = T(0.125) * eqmc15;
+// Inverse transform, collision.f90:157-220: populations from the conserved
+// moments c = (rho, jx, jy, jz) and the 15 others x (same order as
+// moments()), through the orthogonal basis (val*i = 1/val*, coef3i = 1/8,
+// coef4i = -1/4).
+template <typename T>
+__host__ __device__ __forceinline__ void inverse(T rho, T jx, T jy, T jz, const T (&x)[15], T (&out)[19])
+{
+    const T tl1 = T(1.0 / 19.0) * rho;
+    const T tl2 = T(-11.0 * (1.0 / 2394.0)) * x[0];
+    const T tl3 = T(8.0 * (1.0 / 2394.0)) * x[0];
+    const T tl4 = T(-4.0 * (1.0 / 252.0)) * x[1];
+    const T tl5 = T(1.0 / 252.0) * x[1];
+    const T tl6 = T(1.0 / 10.0) * jx;
+    const T tl7 = T(1.0 / 40.0) * x[2];
+    const T tl8 = T(1.0 / 10.0) * jy;
+    const T tl9 = T(1.0 / 40.0) * x[3];
+    const T tl10 = T(1.0 / 10.0) * jz;
+    const T tl11 = T(1.0 / 40.0) * x[4];
+    const T tl12 = T(1.0 / 36.0) * x[5];
+    const T tl13 = T(1.0 / 72.0) * x[6];
+    const T tl14 = T(1.0 / 12.0) * x[7];
+    const T tl15 = T(1.0 / 24.0) * x[8];
+    const T tl16 = -(T(-0.25) * x[9]);
+    const T tl17 = -(T(-0.25) * x[10]);
+    const T tl18 = -(T(-0.25) * x[11]);
+    const T tl19 = T(0.125) * x[12];
+    const T tl20 = T(0.125) * x[13];
+    const T tl21 = T(0.125) * x[14];
 
     // :180-198
-    f9[0] = tl1 - T(30.0 * (1.0 / 2394.0)) * eqmc1 + T(12.0 * (1.0 / 252.0)) * eqmc2;
+    out[0] = tl1 - T(30.0 * (1.0 / 2394.0)) * x[0] + T(12.0 * (1.0 / 252.0)) * x[1];
     const T suma = tl1 + tl2 + tl4;
     const T sumb = tl1 + tl3 + tl5;
     const T sumc = tl6 + T(-4.0) * tl7;
@@ -401,28 +382,157 @@ __device__ __forceinline__ void mrt_collide(T (&f)[19], T rho9, T ux9, T uy9, T 
     const T sum1011 = tl10 + tl11;
 
     // :200-220
-    f9[1] = suma + sumc + sumd;
-    f9[2] = suma - sumc + sumd;
-    f9[3] = suma + sume + sumf;
-    f9[4] = suma - sume + sumf;
-    f9[5] = suma + sumg + sumh;
-    f9[6] = suma - sumg + sumh;
-    f9[7] = sumb + sum67 + sum89 + sumi + tl16 + tl19 - tl20;
-    f9[8] = sumb - sum67 + sum89 + sumi - tl16 - tl19 - tl20;
-    f9[9] = sumb + sum67 - sum89 + sumi - tl16 + tl19 + tl20;
-    f9[10] = sumb - sum67 - sum89 + sumi + tl16 - tl19 + tl20;
-    f9[11] = sumb + sum67 + sum1011 + sumk + tl18 - tl19 + tl21;
-    f9[12] = sumb - sum67 + sum1011 + sumk - tl18 + tl19 + tl21;
-    f9[13] = sumb + sum67 - sum1011 + sumk - tl18 - tl19 - tl21;
-    f9[14] = sumb - sum67 - sum1011 + sumk + tl18 + tl19 - tl21;
-    f9[15] = sumb + sum89 + sum1011 + sump + tl17 + tl20 - tl21;
-    f9[16] = sumb - sum89 + sum1011 + sump - tl17 - tl20 - tl21;
-    f9[17] = sumb + sum89 - sum1011 + sump - tl17 + tl20 + tl21;
-    f9[18] = sumb - sum89 - sum1011 + sump + tl17 - tl20 + tl21;
+    out[1] = suma + sumc + sumd;
+    out[2] = suma - sumc + sumd;
+    out[3] = suma + sume + sumf;
+    out[4] = suma - sume + sumf;
+    out[5] = suma + sumg + sumh;
+    out[6] = suma - sumg + sumh;
+    out[7] = sumb + sum67 + sum89 + sumi + tl16 + tl19 - tl20;
+    out[8] = sumb - sum67 + sum89 + sumi - tl16 - tl19 - tl20;
+    out[9] = sumb + sum67 - sum89 + sumi - tl16 + tl19 + tl20;
+    out[10] = sumb - sum67 - sum89 + sumi + tl16 - tl19 + tl20;
+    out[11] = sumb + sum67 + sum1011 + sumk + tl18 - tl19 + tl21;
+    out[12] = sumb - sum67 + sum1011 + sumk - tl18 + tl19 + tl21;
+    out[13] = sumb + sum67 - sum1011 + sumk - tl18 - tl19 - tl21;
+    out[14] = sumb - sum67 - sum1011 + sumk + tl18 + tl19 - tl21;
+    out[15] = sumb + sum89 + sum1011 + sump + tl17 + tl20 - tl21;
+    out[16] = sumb - sum89 + sum1011 + sump - tl17 - tl20 - tl21;
+    out[17] = sumb + sum89 - sum1011 + sump - tl17 + tl20 + tl21;
+    out[18] = sumb - sum89 - sum1011 + sump + tl17 - tl20 + tl21;
+}
+
+// Relaxation rate of each of the 15 non-conserved moments (moments() order).
+template <typename T>
+__host__ __device__ __forceinline__ T rate_of(int k, const Rates<T> &r, T s9, T s13)
+{
+    switch (k) {
+    case 0: return r.s1;
+    case 1: return r.s2;
+    case 2: case 3: case 4: return r.s4;
+    case 5: case 7: return s9;
+    case 6: case 8: return r.s10;
+    case 9: case 10: case 11: return s13;
+    default: return r.s16;
+    }
+}
+
+// Non-equilibrium parts of the nine moments a closure sees (relaxed with s1,
+// s2, s9, s10, s13): e, eps, 3pxx, 3pixx, pww, piww, pxy, pyz, pxz.
+template <typename T>
+__host__ __device__ __forceinline__ void closure_inputs(const T (&m)[15], const T (&e)[12], T (&neq)[9])
+{
+    neq[0] = m[0] - e[0];
+    neq[1] = m[1] - e[1];
+    neq[2] = m[5] - e[5];
+    neq[3] = m[6] - e[6];
+    neq[4] = m[7] - e[7];
+    neq[5] = m[8] - e[8];
+    neq[6] = m[9] - e[9];
+    neq[7] = m[10] - e[10];
+    neq[8] = m[11] - e[11];
+}
+
+// One MRT collision with Guo forcing, collision.f90:60-220, in the
+// reference's form.  On entry f holds the node's populations; on exit the
+// stored post-collision values f9(ip) + 0.5*Fbar(ip) (the values the swap
+// loop writes, :230-262).  A closure sees the non-equilibrium parts of the
+// nine moments relaxed with s1, s2, s9, s10, s13 and may replace this node's
+// s9 and s13.
+template <typename T, typename Closure = NoClosure>
+__host__ __device__ __forceinline__ void mrt_collide(T (&f)[19], T rho9, T ux9, T uy9, T uz9, const Rates<T> &r,
+                                            const Closure &closure = Closure())
+{
+    T Fb[19];
+    guo_terms(ux9, uy9, uz9, r.fx, r.fy, r.fz, Fb);
+
+    T f9[19];
+#pragma unroll
+    for (int i = 0; i < 19; i++) // :84
+        f9[i] = f[i] + T(0.5) * Fb[i];
+
+    T eq[12];
+    equilibria(rho9, ux9, uy9, uz9, r, eq);
+    T m[15];
+    moments(f9, m);
+
+    T s9 = r.s9, s13 = r.s13;
+    if constexpr (!std::is_same<Closure, NoClosure>::value) {
+        T neq[9];
+        closure_inputs(m, eq, neq);
+        closure(neq, s9, s13);
+    }
+
+    // :140-154 relaxation (eqm13..15 = 0)
+    T post[15];
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+        post[k] = m[k] - rate_of(k, r, s9, s13) * (m[k] - eq[k]);
+#pragma unroll
+    for (int k = 12; k < 15; k++)
+        post[k] = m[k] - r.s16 * (m[k] - T(0.0));
+
+    // :157-166: the conserved moments come from the field values
+    inverse(rho9, ux9, uy9, uz9, post, f9);
 
 #pragma unroll
     for (int i = 0; i < 19; i++) // the stored value, :230-262
         f[i] = f9[i] + T(0.5) * Fb[i];
+}
+
+// The same collision written as an increment, for FP32 arithmetic:
+//
+//     P_i = f_i + Fbar_i + [M^-1 (m* - m)]_i
+//
+// with m the moments of f9 = f + Fbar/2, m* their relaxed values and the
+// conserved moments' changes (drho, djx, djy, djz) = field value - moment of
+// f9.  This equals the reference form f9 + M^-1 (m* - m) + Fbar/2 in exact
+// arithmetic.  In FP32 the reference form rebuilds every population from
+// moments of size |u| and adds F/2 ~ 1e-7 to sums of size 0.1, which
+// rounds a few per cent of the per-step momentum input away, the same way on
+// every node of a flow that is uniform along the walls: the mean profile
+// then drifts by ~1e-4 of U per thousand steps.  Here the small terms are
+// formed and summed first and f_i is touched by one addition, so momentum
+// enters exactly up to one rounding of each stored value.  When the fields
+// are macrovar of the same f, the conserved changes vanish exactly
+// (sum_i Fbar_i = 0, sum_i c_i Fbar_i = F) and the caller passes zeros.
+template <typename T, typename Closure = NoClosure>
+__host__ __device__ __forceinline__ void mrt_collide_delta(T (&f)[19], T rho9, T ux9, T uy9, T uz9, T drho, T djx, T djy,
+                                                  T djz, const Rates<T> &r, const Closure &closure = Closure())
+{
+    T Fb[19];
+    guo_terms(ux9, uy9, uz9, r.fx, r.fy, r.fz, Fb);
+
+    T f9[19];
+#pragma unroll
+    for (int i = 0; i < 19; i++)
+        f9[i] = f[i] + T(0.5) * Fb[i];
+
+    T eq[12];
+    equilibria(rho9, ux9, uy9, uz9, r, eq);
+    T m[15];
+    moments(f9, m);
+
+    T s9 = r.s9, s13 = r.s13;
+    if constexpr (!std::is_same<Closure, NoClosure>::value) {
+        T neq[9];
+        closure_inputs(m, eq, neq);
+        closure(neq, s9, s13);
+    }
+
+    T dm[15];
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+        dm[k] = -(rate_of(k, r, s9, s13) * (m[k] - eq[k]));
+#pragma unroll
+    for (int k = 12; k < 15; k++)
+        dm[k] = -(r.s16 * m[k]);
+
+    T d[19];
+    inverse(drho, djx, djy, djz, dm, d);
+#pragma unroll
+    for (int i = 0; i < 19; i++)
+        f[i] = f[i] + (Fb[i] + d[i]);
 }
 
 } // namespace d3q19

@@ -156,12 +156,31 @@ __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
         if constexpr ((MODE & 3) == 1)
             rho9 = rho9 - (T)a.rho_off; // collision.f90:508
     }
-    if constexpr ((MODE & 8) != 0) {
-        const uint32_t ix = n - fdiv(n, a.g.dlx) * a.g.lx;
-        const float wd = fminf((float)ix + 0.5f, (float)a.g.lx - (float)ix - 0.5f);
-        mrt_collide(f, rho9, ux9, uy9, uz9, r, MlpClosure{a.mlp, wd * a.inv_ystar / a.yscale});
-    } else {
+    if constexpr (P == 0) {
+        static_assert((MODE & 8) == 0, "the closure runs in FP32 arithmetic only");
         mrt_collide(f, rho9, ux9, uy9, uz9, r);
+    } else {
+        // increment form; conserved-moment changes are zero for fields fused
+        // from f, -rho_off after avedensity, field - moment for arrays
+        T drho = T(0), djx = T(0), djy = T(0), djz = T(0);
+        if constexpr ((MODE & 3) == 1) {
+            drho = -(T)a.rho_off;
+        } else if constexpr ((MODE & 3) == 2) {
+            T rf, jx, jy, jz;
+            macro_sums(f, rf, jx, jy, jz);
+            drho = rho9 - rf;
+            djx = ux9 - (jx + r.fx / T(2.));
+            djy = uy9 - (jy + r.fy / T(2.));
+            djz = uz9 - (jz + r.fz / T(2.));
+        }
+        if constexpr ((MODE & 8) != 0) {
+            const uint32_t ix = n - fdiv(n, a.g.dlx) * a.g.lx;
+            const float wd = fminf((float)ix + 0.5f, (float)a.g.lx - (float)ix - 0.5f);
+            mrt_collide_delta(f, rho9, ux9, uy9, uz9, drho, djx, djy, djz, r,
+                              MlpClosure{a.mlp, wd * a.inv_ystar / a.yscale});
+        } else {
+            mrt_collide_delta(f, rho9, ux9, uy9, uz9, drho, djx, djy, djz, r);
+        }
     }
     store_all<P>(dst, a.g.N, n, f);
 }

@@ -48,22 +48,28 @@ def laminar_parabola_noise(lx, ly, lz, seed=1, uc=0.05, amp=0.05):
     return ux, uy, uz
 
 
-def loglaw_divfree(lx, ly, lz, ustar, ystar, seed=2, amp=0.10, nmodes=8):
+def loglaw_divfree(lx, ly, lz, ustar, ystar, seed=2, amp=0.10, nmodes=8, kmax=(3, 3), near_wall=False):
     """Log-law mean profile (Channel-Flow/initial.f90:104-115) plus a
     divergence-free random-phase perturbation u' = curl(psi).
 
     psi_c(x, y, z) = sum_m a_{c,m} g(x) cos(ky_m y + kz_m z + phi_{c,m}) with the
-    wall envelope g(x) = sin^2(pi x / lx) (x = i - 0.5), integer wavenumbers
-    1..3 in y and z on the periodic box; derivatives are taken analytically, so
-    div u' = 0 exactly in the continuum.  u' is scaled so that max|u'| equals
-    amp times the centreline mean velocity.
+    wall envelope g(x) = sin^2(pi x / lx) (x = i - 0.5), or sin^2(2 pi x / lx)
+    with near_wall (peaks at a quarter of the width from each wall), integer
+    wavenumbers 1..kmax[0] in y and 1..kmax[1] in z on the periodic box;
+    derivatives are taken analytically, so div u' = 0 exactly in the
+    continuum.  u' is scaled so that max|u'| equals amp times the centreline
+    mean velocity.
     """
     prof = np.asarray(loglaw_profile(lx, ustar, ystar))
     x = np.arange(1, lx + 1, dtype=np.float64) - 0.5
     y = np.arange(1, ly + 1, dtype=np.float64) - 0.5
     z = np.arange(1, lz + 1, dtype=np.float64) - 0.5
-    g = np.sin(math.pi * x / lx) ** 2
-    dg = 2.0 * np.sin(math.pi * x / lx) * np.cos(math.pi * x / lx) * (math.pi / lx)
+    if near_wall:
+        g = np.sin(2.0 * math.pi * x / lx) ** 2
+        dg = 2.0 * np.sin(2.0 * math.pi * x / lx) * np.cos(2.0 * math.pi * x / lx) * (2.0 * math.pi / lx)
+    else:
+        g = np.sin(math.pi * x / lx) ** 2
+        dg = 2.0 * np.sin(math.pi * x / lx) * np.cos(math.pi * x / lx) * (math.pi / lx)
     r = uniform(seed, 9 * nmodes)
     # u' separates into wall envelopes times (z, y) plane fields:
     #   u_x = g A,  u_y = g B - g' C,  u_z = g' D - g E
@@ -74,8 +80,8 @@ def loglaw_divfree(lx, ly, lz, ustar, ystar, seed=2, amp=0.10, nmodes=8):
     E = np.zeros((lz, ly))
     for m in range(nmodes):
         a = 2.0 * r[9 * m: 9 * m + 3] - 1.0
-        ky = 2.0 * math.pi * (1 + int(r[9 * m + 3] * 3)) / ly
-        kz = 2.0 * math.pi * (1 + int(r[9 * m + 4] * 3)) / lz
+        ky = 2.0 * math.pi * (1 + int(r[9 * m + 3] * kmax[0])) / ly
+        kz = 2.0 * math.pi * (1 + int(r[9 * m + 4] * kmax[1])) / lz
         ph = 2.0 * math.pi * r[9 * m + 5: 9 * m + 8]
         arg = ky * y[None, :] + kz * z[:, None]
         c = [np.cos(arg + ph[k]) for k in range(3)]
