@@ -81,6 +81,7 @@ struct Launch {
     static hipError_t rho_partials(const SumArgs &a, int src, uint32_t grid, hipStream_t s);
     static hipError_t mass_partials(const SumArgs &a, uint32_t grid, hipStream_t s);
     static hipError_t plane_partials(const PlaneArgs &a, int src, uint32_t chunks, hipStream_t s);
+    static hipError_t filtered_plane_partials(const PlaneArgs &a, uint32_t chunks, hipStream_t s);
 };
 
 extern template struct Launch<0>;

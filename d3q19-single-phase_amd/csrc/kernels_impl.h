@@ -440,6 +440,61 @@ __global__ void __launch_bounds__(BLOCK) k_plane_partials(PlaneArgs a)
         out[(uint64_t)q * a.g.lx + ix] = s[q];
 }
 
+// The same plane sums for the 2x2x2 box-filtered fields on the grid of half
+// the size: coarse node (cx, cy, cz) carries the means of rho and u (macrovar
+// of F, with F/2 and rho_off) over fine nodes (2cx+{0,1}, 2cy+{0,1},
+// 2cz+{0,1}).  Partials are laid out for lx/2 planes.  Even lx, ly, lz.
+template <int P>
+__global__ void __launch_bounds__(BLOCK) k_filtered_plane_partials(PlaneArgs a)
+{
+    using S = typename Prec<P>::S;
+    using T = typename Prec<P>::T;
+    const uint32_t lxc = a.g.lx / 2, lyc = a.g.ly / 2;
+    const uint32_t ic = blockIdx.y * blockDim.x + threadIdx.x;
+    if (ic >= lxc)
+        return;
+    double s[NPLANE] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t rows = lyc * (a.g.lz / 2);
+    for (uint32_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const uint32_t cy = row % lyc, cz = row / lyc;
+        double rho = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
+        for (uint32_t dz = 0; dz < 2; dz++)
+            for (uint32_t dy = 0; dy < 2; dy++)
+                for (uint32_t dx = 0; dx < 2; dx++) {
+                    const uint32_t n = (2 * ic + dx) + a.g.lx * ((2 * cy + dy) + a.g.ly * (2 * cz + dz));
+                    const Nbr b = make_nbr(n, a.g);
+                    T f[19];
+                    pull_all<P>((const S *)a.L, a.g.N, b, f);
+                    T rho9, ux9, uy9, uz9;
+                    macro_sums(f, rho9, ux9, uy9, uz9);
+                    ux += (double)(ux9 + (T)a.fx / T(2.));
+                    uy += (double)(uy9 + (T)a.fy / T(2.));
+                    uz += (double)(uz9 + (T)a.fz / T(2.));
+                    rho += (double)(rho9 - (T)a.rho_off);
+                }
+        rho *= 0.125;
+        ux *= 0.125;
+        uy *= 0.125;
+        uz *= 0.125;
+        s[0] += ux;
+        s[1] += uy;
+        s[2] += uz;
+        s[3] += ux * uz;
+        s[4] += ux * uy;
+        s[5] += uy * uz;
+        s[6] += ux * ux;
+        s[7] += uy * uy;
+        s[8] += uz * uz;
+        s[9] += rho;
+        s[10] += rho * rho;
+        s[11] += 1.0;
+    }
+    double *out = a.partials + (uint64_t)blockIdx.x * NPLANE * lxc;
+#pragma unroll
+    for (int q = 0; q < NPLANE; q++)
+        out[(uint64_t)q * lxc + ic] = s[q];
+}
+
 // ---------------------------------------------------------------------------
 template <int P>
 static inline uint32_t nblocks(uint64_t n)
@@ -546,6 +601,16 @@ hipError_t Launch<P>::plane_partials(const PlaneArgs &a, int src, uint32_t chunk
         hipLaunchKernelGGL((k_plane_partials<P, 0>), grid, blk, 0, s, a);
     else
         hipLaunchKernelGGL((k_plane_partials<P, 1>), grid, blk, 0, s, a);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t Launch<P>::filtered_plane_partials(const PlaneArgs &a, uint32_t chunks, hipStream_t s)
+{
+    const uint32_t lxc = a.g.lx / 2;
+    const uint32_t bx = lxc >= BLOCK ? BLOCK : ((lxc + 63) / 64) * 64;
+    const dim3 grid(chunks, (lxc + bx - 1) / bx), blk(bx);
+    hipLaunchKernelGGL((k_filtered_plane_partials<P>), grid, blk, 0, s, a);
     return hipGetLastError();
 }
 

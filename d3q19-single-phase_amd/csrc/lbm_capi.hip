@@ -76,6 +76,8 @@ struct lbm_channel {
     unsigned long long *errbits;
     double *stat_acc;   // NPLANE*lx running sums
     long long stat_n;
+    double *fstat_acc;  // NPLANE*(lx/2) running sums of the box-filtered fields (allocated on first use)
+    long long fstat_n;
     double *stage;      // COPY_NODES*19 doubles
     hipEvent_t ev[2];
     struct Part *part;  // rigid particles (nullptr without)
@@ -332,12 +334,38 @@ int plane_sums(lbm_channel *h, bool acc)
     return LBM_OK;
 }
 
-// Columns of saveload.f90:1430-1492 from plane sums s[q*lx + i]; the
-// fluid-node count of plane i is s[11*lx + i] (per sample), nsamp the number
-// of accumulated samples.
-void stat_columns(const lbm_channel *h, const double *s, double nsamp, double ustar, double ystar, double *out)
+// Plane sums of the 2x2x2 box-filtered fields of the current populations
+// into red_out[2 .. 2 + 12*lx/2) and the filtered running accumulator.
+int filtered_plane_sums(lbm_channel *h)
 {
-    const uint32_t lx = h->g.lx;
+    PlaneArgs a;
+    a.g = h->g;
+    a.L = h->lat[h->ms == FROM_PREV ? 1 - h->cur : h->cur];
+    a.rho = a.ux = a.uy = a.uz = nullptr;
+    a.fx = h->force[0];
+    a.fy = h->force[1];
+    a.fz = h->force[2];
+    a.rho_off = h->rho_off;
+    a.solid = nullptr;
+    a.partials = h->partials + 2 * RED_BLOCKS;
+    const uint32_t rows = (h->g.ly / 2) * (h->g.lz / 2);
+    const uint32_t chunks = std::min<uint32_t>(PLANE_CHUNKS, rows);
+    int rc = dispatch_err(PREC_CALL(h, filtered_plane_partials, a, chunks, h->stream), "filtered plane sums");
+    if (rc)
+        return rc;
+    const uint32_t width = NPLANE * (h->g.lx / 2);
+    hipLaunchKernelGGL(k_sum_columns, dim3((width + 255) / 256), dim3(256), 0, h->stream, a.partials, chunks, width,
+                       h->red_out + 2, h->fstat_acc);
+    HIP_TRY(hipGetLastError());
+    return LBM_OK;
+}
+
+// Columns of saveload.f90:1430-1492 from plane sums s[q*lx + i] of a grid
+// with lx planes of plane_nodes nodes; the fluid-node count of plane i is
+// s[11*lx + i] (per sample), nsamp the number of accumulated samples.
+void stat_columns(uint32_t lx, double plane_nodes, const double *s, double nsamp, double ustar, double ystar,
+                  double *out)
+{
     const double us2 = ustar * ustar, us4 = us2 * us2;
     for (uint32_t i = 0; i < lx; i++) {
         const double *c = s + i;
@@ -351,7 +379,7 @@ void stat_columns(const lbm_channel *h, const double *s, double nsamp, double us
         double zz = c[8 * lx] / nf / us2 - vz * vz;
         double pr = c[9 * lx] / nf / us2;
         double p2 = c[10 * lx] / nf / us4 - pr * pr;
-        const double volf = c[11 * lx] / nsamp / ((double)h->g.ly * (double)h->g.lz);
+        const double volf = c[11 * lx] / nsamp / plane_nodes;
         double yplus = ((double)(i + 1) - 0.5) / ystar;
         if (i + 1 > lx / 2)
             yplus = ((double)(lx - (i + 1)) + 0.5) / ystar;
@@ -492,6 +520,8 @@ int lbm_destroy(lbm_channel *h)
         (void)hipFree(h->red_out);
     if (h->stat_acc)
         (void)hipFree(h->stat_acc);
+    if (h->fstat_acc)
+        (void)hipFree(h->fstat_acc);
     if (h->errbits)
         (void)hipFree(h->errbits);
     if (h->stage)
@@ -777,7 +807,7 @@ int lbm_statistc2(lbm_channel *h, double ustar, double ystar, double *out)
     std::vector<double> s(NPLANE * (size_t)h->g.lx);
     HIP_TRY(hipMemcpyAsync(s.data(), h->red_out + 2, s.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    stat_columns(h, s.data(), 1.0, ustar, ystar, out);
+    stat_columns(h->g.lx, (double)h->g.ly * (double)h->g.lz, s.data(), 1.0, ustar, ystar, out);
     return LBM_OK;
 }
 
@@ -803,7 +833,49 @@ int lbm_stats_read(lbm_channel *h, double ustar, double ystar, double *out, long
     std::vector<double> s(NPLANE * (size_t)h->g.lx);
     HIP_TRY(hipMemcpyAsync(s.data(), h->stat_acc, s.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    stat_columns(h, s.data(), (double)h->stat_n, ustar, ystar, out);
+    stat_columns(h->g.lx, (double)h->g.ly * (double)h->g.lz, s.data(), (double)h->stat_n, ustar, ystar, out);
+    return LBM_OK;
+}
+
+int lbm_stats_accumulate_filtered(lbm_channel *h)
+{
+    CHECK_H(h);
+    if (h->g.lx % 2 || h->g.ly % 2 || h->g.lz % 2)
+        return fail(LBM_ERR_ARG, "lbm_stats_accumulate_filtered: the 2x2x2 filter needs even lx, ly, lz (%u, %u, %u)",
+                    h->g.lx, h->g.ly, h->g.lz);
+    if (h->part)
+        return fail(LBM_ERR_STATE, "lbm_stats_accumulate_filtered: not defined with particles (filter cells would mix "
+                                   "fluid and solid nodes)");
+    if (!h->fstat_acc) {
+        const uint64_t n = NPLANE * (uint64_t)(h->g.lx / 2);
+        if (hipMalloc((void **)&h->fstat_acc, n * sizeof(double)) != hipSuccess)
+            return fail(LBM_ERR_NOMEM, "hipMalloc of the filtered statistics accumulator failed");
+        hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->fstat_acc, n, 0.0);
+        HIP_TRY(hipGetLastError());
+        h->fstat_n = 0;
+    }
+    int rc = filtered_plane_sums(h);
+    if (rc)
+        return rc;
+    h->fstat_n++;
+    return LBM_OK;
+}
+
+int lbm_stats_read_filtered(lbm_channel *h, double ustar, double ystar, double *out, long long *nsamples)
+{
+    CHECK_H(h);
+    if (!out)
+        return fail(LBM_ERR_ARG, "lbm_stats_read_filtered: null output");
+    if (nsamples)
+        *nsamples = h->fstat_n;
+    if (h->fstat_n == 0)
+        return fail(LBM_ERR_STATE, "lbm_stats_read_filtered: no samples accumulated");
+    const uint32_t lxc = h->g.lx / 2;
+    std::vector<double> s(NPLANE * (size_t)lxc);
+    HIP_TRY(hipMemcpyAsync(s.data(), h->fstat_acc, s.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    stat_columns(lxc, (double)(h->g.ly / 2) * (double)(h->g.lz / 2), s.data(), (double)h->fstat_n, ustar, ystar,
+                 out);
     return LBM_OK;
 }
 
@@ -814,6 +886,13 @@ int lbm_stats_reset(lbm_channel *h)
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->stat_acc, n, 0.0);
     HIP_TRY(hipGetLastError());
     h->stat_n = 0;
+    if (h->fstat_acc) {
+        const uint64_t nf = NPLANE * (uint64_t)(h->g.lx / 2);
+        hipLaunchKernelGGL(k_fill, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, h->stream, h->fstat_acc, nf,
+                           0.0);
+        HIP_TRY(hipGetLastError());
+    }
+    h->fstat_n = 0;
     return LBM_OK;
 }
 

@@ -74,6 +74,8 @@ _SIGS = {
     "lbm_stats_accumulate": [_H],
     "lbm_stats_read": [_H, ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_longlong)],
     "lbm_stats_reset": [_H],
+    "lbm_stats_accumulate_filtered": [_H],
+    "lbm_stats_read_filtered": [_H, ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_longlong)],
     "lbm_sync": [_H],
     "lbm_time_steps": [_H, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "lbm_device_bytes": [_H, ctypes.POINTER(ctypes.c_size_t)],
@@ -316,7 +318,23 @@ class Channel:
         return out, n.value
 
     def stats_reset(self):
+        """Clear the running (and the filtered running) statistics."""
         _call("lbm_stats_reset", self._h)
+
+    def stats_accumulate_filtered(self):
+        """Add the plane sums of the 2x2x2 box-filtered fields."""
+        _call("lbm_stats_accumulate_filtered", self._h)
+
+    def stats_read_filtered(self):
+        """statistc2 columns of the filtered running average on the grid of
+        half the size (ystar of that grid = ystar / 2; u* unchanged under the
+        acoustic scaling dx, dt -> 2 dx, 2 dt); array (lx/2, 15) and the
+        sample count."""
+        out = np.empty((self.lx // 2, 15))
+        n = ctypes.c_longlong()
+        _call("lbm_stats_read_filtered", self._h, self.p["ustar"], self.p["ystar"] / 2.0, _ptr(out),
+              ctypes.byref(n))
+        return out, n.value
 
     def sync(self):
         _call("lbm_sync", self._h)

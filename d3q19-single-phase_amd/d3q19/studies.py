@@ -128,9 +128,164 @@ def precision_study(lx=192, ly=768, lz=384, steps=20000, stats_from=10000, stats
     return report, cases
 
 
+# A start that becomes turbulent within ~10^4 steps on the 192 x 768 x 384
+# grid: near-wall divergence-free perturbation of 30 % of U_c with spanwise
+# wavelengths down to 48 nodes (~90 wall units).
+TURBULENT_START = dict(amp=0.3, nmodes=16, kmax=(4, 8), near_wall=True)
+
+
+def _start(ch, perturb, seed):
+    ux, uy, uz = workloads.loglaw_divfree(ch.lx, ch.ly, ch.lz, ch.p["ustar"], ch.p["ystar"], seed=seed, **perturb)
+    ch.set_macros(ux=ux, uy=uy, uz=uz)
+    del ux, uy, uz
+    ch.FORCING()
+    ch.initpop()
+    ch.macrovar()
+
+
+def _stat_errors(p, ref):
+    """Mean-profile and second-moment errors of a coarse run against the
+    filtered fine statistics, each as max over planes |v - v_ref| / max |v_ref|."""
+    return compare(p, ref)
+
+
+def _coarse_run(lx, ly, lz, nu_c, steps, stats_from, stats_every, perturb, seed, closure=None, log=None):
+    with Channel(lx, ly, lz, "fp32", mrttype=3, visc=nu_c) as ch:
+        _start(ch, perturb, seed)
+        if closure is not None:
+            ch.closure_set(closure, ystar=ch.p["ystar"])
+        ch.stats_reset()
+        ch.sync()
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            ch.run(stats_every)
+            done += stats_every
+            if done > stats_from:
+                ch.stats_accumulate()
+            if log and done % max(steps // 5, 1) == 0:
+                log(f"coarse{' + closure' if closure is not None else ''}: step {done}/{steps}")
+        ch.sync()
+        wall = time.perf_counter() - t0
+        stats, n = ch.stats_read()
+        return {"stats": stats, "samples": n, "wall_s": wall, "mlups": lx * ly * lz * steps / wall / 1e6}
+
+
+def closure_study(lx=192, ly=768, lz=384, spinup=40000, window=40000, stats_every=100, snapshots=8,
+                  iterations=4000, batch=65536, lr=2e-3, seed=2, perturb=TURBULENT_START, log=None):
+    """Learned closure against plain MRT (a posteriori).
+
+    1. Fine run, plain MRT at full resolution (FP32 storage, rate set 3), from
+       a start that becomes turbulent; statistics and their 2x2x2 box-filtered
+       counterparts are averaged over the window after the spin-up, and
+       `snapshots` filtered snapshots (from half the spin-up on) become
+       training samples.
+    2. The network is trained on all snapshots but the last by the device's
+       own kernels; the last one measures the a-priori gain over nu_t = 0.
+    3. Coarse runs on the grid of half the size (nu/2: dx and dt double) from
+       the same analytic start, plain and with the trained closure, over the
+       same physical time (half the steps); their statistics are compared
+       with the filtered fine ones, and their wall time with the fine run's.
+    """
+    from . import Trainer, closure_params
+    say = log or (lambda m: None)
+    total = spinup + window
+    snap_at = sorted({int(round(s / stats_every)) * stats_every
+                      for s in np.linspace(spinup // 2, total, snapshots)})
+    Nc = lx * ly * lz // 8
+    trainer = None
+    report = {"grid": [lx, ly, lz], "coarse_grid": [lx // 2, ly // 2, lz // 2], "spinup": spinup, "window": window,
+              "stats_every": stats_every, "perturbation": {k: list(v) if isinstance(v, tuple) else v
+                                                          for k, v in perturb.items()}}
+    try:
+        with Channel(lx, ly, lz, "fp32", mrttype=3) as fine:
+            _start(fine, perturb, seed)
+            nu_c = fine.p["visc"] / 2.0
+            s9_c = 1.0 / (3.0 * nu_c + 0.5)
+            ystar_c = fine.p["ystar"] / 2.0
+            fine.stats_reset()
+            fine.sync()
+            t0 = time.perf_counter()
+            t_snap = 0.0
+            done = 0
+            while done < total:
+                fine.run(stats_every)
+                done += stats_every
+                if done > spinup:
+                    fine.stats_accumulate()
+                    fine.stats_accumulate_filtered()
+                if done in snap_at:
+                    fine.sync()
+                    ts = time.perf_counter()
+                    if trainer is None:
+                        # input scales: spread of each feature over the first snapshot
+                        with Trainer(Nc, closure_params(nu=nu_c)) as probe:
+                            probe.add_filtered(fine, nu_c, s9_c, ystar_c)
+                            f, _ = probe.samples()
+                        sc = f[:, :9].astype(np.float64).std(axis=0)
+                        sc[~(sc > 1e-12)] = 1.0
+                        del f
+                        trainer = Trainer(len(snap_at) * Nc, closure_params(seed=1, scales=sc, nu=nu_c))
+                        report["input_scales"] = [float(v) for v in sc]
+                    trainer.add_filtered(fine, nu_c, s9_c, ystar_c)
+                    t_snap += time.perf_counter() - ts
+                if done % max(total // 10, 1) == 0:
+                    s = fine.statistc2()
+                    say(f"fine: step {done}/{total} U+c {s[lx // 2, 3]:.3f} "
+                        f"uu(y+~20) {s[int(20 * fine.p['ystar']), 6]:.3f}")
+            fine.sync()
+            wall_fine = time.perf_counter() - t0 - t_snap
+            fine_stats, n_fine = fine.stats_read()
+            fine_filt, _ = fine.stats_read_filtered()
+        report["fine"] = {"wall_s": round(wall_fine, 2), "samples": n_fine, "snapshot_s": round(t_snap, 2),
+                          "mlups": round(lx * ly * lz * total / wall_fine / 1e6, 1),
+                          # filtered against plane-pair means of the unfiltered statistics:
+                          # zero for U+ (the filter is linear), the subfilter share for the rest
+                          "filter_effect": _stat_errors(fine_filt, 0.5 * (fine_stats[0::2] + fine_stats[1::2]))}
+        # training on all snapshots but the last; the last one is held out
+        n_all = trainer.count()
+        n_train = n_all - Nc
+        _, t_all = trainer.samples()
+        held = t_all[n_train:].astype(np.float64)
+        del t_all
+        mse_zero = float(np.mean(held ** 2))
+        mse_init = float(np.mean((trainer.predict(n_train, Nc) - held) ** 2))
+        t1 = time.perf_counter()
+        loss = trainer.train(0, n_train, iterations=iterations, batch=min(batch, n_train), lr=lr)
+        t_train = time.perf_counter() - t1
+        mse_net = float(np.mean((trainer.predict(n_train, Nc) - held) ** 2))
+        trained = trainer.params()
+        report["training"] = {"samples": int(n_train), "held_out": int(Nc), "iterations": iterations,
+                              "batch": min(batch, n_train), "lr": lr, "last_batch_loss": float(loss),
+                              "wall_s": round(t_train, 2), "held_out_mse": {"zero": mse_zero, "initial": mse_init,
+                                                                            "trained": mse_net},
+                              "held_out_target_mean": float(held.mean())}
+        say(f"training: held-out mse zero {mse_zero:.4f} initial {mse_init:.4f} trained {mse_net:.4f}")
+    finally:
+        if trainer is not None:
+            trainer.close()
+    ref = fine_filt
+    for name, closure in (("coarse_plain", None), ("coarse_closure", trained)):
+        r = _coarse_run(lx // 2, ly // 2, lz // 2, nu_c, total // 2, spinup // 2, stats_every // 2, perturb, seed,
+                        closure=closure, log=say)
+        report[name] = {"wall_s": round(r["wall_s"], 2), "samples": r["samples"], "mlups": round(r["mlups"], 1),
+                        "errors_vs_filtered_fine": _stat_errors(r["stats"], ref),
+                        "time_to_stats_speedup": round(wall_fine / r["wall_s"], 2)}
+        report[name + "_profile"] = r["stats"]
+    report["fine_filtered_profile"] = ref
+    report["fine_profile"] = fine_stats
+    return report, trained
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("study", choices=["precision"])
+    ap.add_argument("study", choices=["precision", "closure"])
+    ap.add_argument("--spinup", type=int, default=40000, help="closure: fine steps before the statistics window")
+    ap.add_argument("--window", type=int, default=40000, help="closure: fine steps of the statistics window")
+    ap.add_argument("--snapshots", type=int, default=8, help="closure: filtered fine snapshots for training")
+    ap.add_argument("--iterations", type=int, default=4000, help="closure: Adam iterations")
+    ap.add_argument("--batch", type=int, default=65536, help="closure: samples per iteration")
+    ap.add_argument("--lr", type=float, default=2e-3, help="closure: Adam learning rate")
     ap.add_argument("--grid", default="192,768,384")
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--stats-from", type=int, default=10000)
@@ -142,6 +297,16 @@ def main(argv=None):
     args = ap.parse_args(argv)
     lx, ly, lz = (int(v) for v in args.grid.split(","))
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.study == "closure":
+        report, trained = closure_study(lx, ly, lz, args.spinup, args.window, args.stats_every, args.snapshots,
+                                        args.iterations, args.batch, args.lr, log=log)
+        arrays = {k: report.pop(k) for k in [k for k in report if k.endswith("_profile")]}
+        print(json.dumps(report, indent=1))
+        if args.out:
+            with open(args.out + ".json", "w") as fh:
+                json.dump(report, fh, indent=1)
+            np.savez(args.out + ".npz", params=trained, **arrays)
+        return
     report, res = precision_study(lx, ly, lz, args.steps, args.stats_from, args.stats_every,
                                   mrttype=args.mrttype, jitter=args.jitter,
                                   deterministic=not args.no_deterministic, log=log)

@@ -23,6 +23,15 @@
         real(c_double) :: ww0, ww1, ww2
       end type lbm_config
 
+      ! struct lbm_particle_config
+      type, bind(C) :: lbm_particle_config
+        integer(c_int) :: npart, maxlink
+        real(c_double) :: rad, rhopart, mingap, mingap_w, stf, stf_w
+        integer(c_int) :: pressure_force, reserved
+      end type lbm_particle_config
+
+      integer(c_int), parameter :: LBM_MLP_TOTAL = 1452
+
       interface
         function lbm_last_error() bind(C, name="lbm_last_error")
           import :: c_ptr
@@ -110,6 +119,81 @@
         end function
 
         integer(c_int) function lbm_sync(h) bind(C, name="lbm_sync")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+        end function
+
+        ! time-averaged statistics (running device sums)
+        integer(c_int) function lbm_stats_accumulate(h) bind(C, name="lbm_stats_accumulate")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+        end function
+
+        integer(c_int) function lbm_stats_read(h, ustar, ystar, out, nsamples) &
+            bind(C, name="lbm_stats_read")
+          import :: c_int, c_ptr, c_double, c_long_long
+          type(c_ptr), value :: h
+          real(c_double), value :: ustar, ystar
+          real(c_double), intent(out) :: out(*)
+          integer(c_long_long), intent(out) :: nsamples
+        end function
+
+        integer(c_int) function lbm_stats_reset(h) bind(C, name="lbm_stats_reset")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+        end function
+
+        ! rigid particles (beads_* of var_inc.f90:166-168)
+        integer(c_int) function lbm_particles_create(h, pc, pos, vel, omg) &
+            bind(C, name="lbm_particles_create")
+          import :: c_int, c_ptr, c_double, lbm_particle_config
+          type(c_ptr), value :: h
+          type(lbm_particle_config), intent(in) :: pc
+          real(c_double), intent(in) :: pos(*), vel(*), omg(*)
+        end function
+
+        integer(c_int) function lbm_beads_links(h) bind(C, name="lbm_beads_links")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+        end function
+
+        integer(c_int) function lbm_beads_filling(h) bind(C, name="lbm_beads_filling")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+        end function
+
+        integer(c_int) function lbm_beads_collision(h) bind(C, name="lbm_beads_collision")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+        end function
+
+        integer(c_int) function lbm_beads_move(h) bind(C, name="lbm_beads_move")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+        end function
+
+        integer(c_int) function lbm_run_particles(h, nsteps) bind(C, name="lbm_run_particles")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h
+          integer(c_int), value :: nsteps
+        end function
+
+        ! any of the six outputs may be c_null_ptr
+        integer(c_int) function lbm_particles_get(h, pos, vel, omg, force, torque, contact) &
+            bind(C, name="lbm_particles_get")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h, pos, vel, omg, force, torque, contact
+        end function
+
+        ! learned closure (FP32/FP16 storage only)
+        integer(c_int) function lbm_closure_set(h, params, ystar) bind(C, name="lbm_closure_set")
+          import :: c_int, c_ptr, c_float, c_double
+          type(c_ptr), value :: h
+          real(c_float), intent(in) :: params(*)
+          real(c_double), value :: ystar
+        end function
+
+        integer(c_int) function lbm_closure_clear(h) bind(C, name="lbm_closure_clear")
           import :: c_int, c_ptr
           type(c_ptr), value :: h
         end function

@@ -137,6 +137,16 @@ int lbm_stats_accumulate(lbm_channel *h);
 int lbm_stats_read(lbm_channel *h, double ustar, double ystar, double *out, long long *nsamples);
 int lbm_stats_reset(lbm_channel *h);
 
+/* The same running statistics for the 2x2x2 box-filtered fields (no
+ * reference counterpart: the filtered fine statistics a 2x-coarsened run is
+ * compared with).  Coarse node (i, j, k) carries the means of rho and u over
+ * fine nodes (2i..2i+1, 2j..2j+1, 2k..2k+1) of the current populations; out
+ * has lx/2 rows of 15 columns as statistc2 writes them for the coarse grid,
+ * with ustar, ystar in that grid's units (ystar of the fine grid / 2).  Even
+ * lx, ly, lz, no particles.  lbm_stats_reset clears these sums too. */
+int lbm_stats_accumulate_filtered(lbm_channel *h);
+int lbm_stats_read_filtered(lbm_channel *h, double ustar, double ystar, double *out, long long *nsamples);
+
 /* Blocks until all work on the handle's stream has finished. */
 int lbm_sync(lbm_channel *h);
 

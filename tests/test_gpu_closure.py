@@ -75,6 +75,42 @@ def test_filtered_samples_match_restatement():
     assert np.allclose(t[ok], t_ref[ok], rtol=2e-2, atol=2e-3)
 
 
+def _plane_columns(rho, ux, uy, uz, ustar):
+    """statistc2 columns 2..12 of one sample from (z, y, x) field arrays."""
+    m = lambda a: a.mean(axis=(0, 1))  # noqa: E731
+    vx, vy, vz = m(ux) / ustar, m(uy) / ustar, m(uz) / ustar
+    us2 = ustar * ustar
+    pr = m(rho) / us2
+    return np.stack([vx, vy, vz, m(ux * ux) / us2 - vx * vx, m(uy * uy) / us2 - vy * vy,
+                     m(uz * uz) / us2 - vz * vz, m(ux * uz) / us2 - vx * vz, m(ux * uy) / us2 - vx * vy,
+                     m(uy * uz) / us2 - vy * vz, pr, m(rho * rho) / us2 ** 2 - pr * pr], axis=1)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_filtered_statistics_match_box_filter_of_fields(prec):
+    lx, ly, lz = 24, 16, 12
+    with d3q19.Channel(lx, ly, lz, prec, laminar=True) as ch:
+        ux, uy, uz = _fine_flow(lx, ly, lz, 6)
+        ch.set_macros(ux=ux, uy=uy, uz=uz)
+        ch.FORCING()
+        ch.initpop()
+        ch.macrovar()
+        ch.run(7)
+        rho, ux, uy, uz = ch.macrovar(download=True)
+        ch.stats_reset()
+        ch.stats_accumulate_filtered()
+        ch.stats_accumulate_filtered()
+        out, n = ch.stats_read_filtered()
+        ustar = ch.p["ustar"]
+    assert n == 2 and out.shape == (lx // 2, 15)
+    box = lambda a: a.reshape(lz // 2, 2, ly // 2, 2, lx // 2, 2).mean(axis=(1, 3, 5))  # noqa: E731
+    want = _plane_columns(box(rho), box(ux), box(uy), box(uz), ustar)
+    # columns 2..12 of saveload.f90:1492; the fields are FP64 in both cases
+    assert np.allclose(out[:, 2:13], want, rtol=1e-9, atol=1e-9 * np.abs(want).max())
+    assert np.allclose(out[:, 13], 1.0) and np.allclose(out[:, 14], 0.0)
+    assert np.allclose(out[:, 0], np.arange(lx // 2) + 0.5)
+
+
 def _run_fp32(lx, ly, lz, vel, steps, visc=None, closure=None, ystar=None):
     with d3q19.Channel(lx, ly, lz, "fp32", visc=visc) as ch:
         ch.set_macros(ux=vel[0], uy=vel[1], uz=vel[2])
