@@ -9,7 +9,15 @@ Workload (default): turbulent channel Re_tau = 180, nu = 0.0036, MRTtype 1,
 lx=192, ly=768, lz=384; 56.6 M nodes), log-law mean profile plus a seeded
 divergence-free random-phase perturbation of 10% amplitude, populations
 resident in HBM before timing.  `value` is FP64 MLUPS; FP32 and FP16-storage
-variants of the same workload are reported under "variants".
+variants of the same workload are reported under "variants".  The timed
+steps stay far below the ~4k steps after which MRTtype 1's linear
+instability at this centreline speed shows (the rates do not change the
+kernel's cost).
+
+"closure_coarse": the 2x-coarse channel (384 x 192 x 96, nu = 0.0018) with
+the fused learned closure, FP32 storage; its ratio of wall time per unit of
+physical time against the fine grid (one coarse step = two fine steps, 1/8
+of the nodes) bounds the time-to-statistics speed-up of the closure run.
 
 Roofline: the step kernel is HBM-bound; algorithmic traffic is one read and
 one write of the 19 populations per node (304 B FP64, 152 B FP32, 76 B FP16).
@@ -95,6 +103,89 @@ def measure(prec, workload, steps, warmup):
     }
 
 
+def measure_closure(steps, warmup):
+    """The 2x-coarse channel (96 x 384 x 192, nu = 0.0018, rate set 3) with the
+    fused closure, FP32 storage.  The kernel cost does not depend on the
+    weights, so a seeded network stands in for a trained one."""
+    lx, ly, lz, _ = WORKLOADS["c5"]
+    n = lx * ly * lz
+    with d3q19.Channel(lx, ly, lz, "fp32", mrttype=3, visc=0.0018) as ch:
+        init_channel(ch, lx, ly, lz)
+        ch.closure_set(d3q19.closure_params(seed=1, nu=0.0018), ystar=ch.p["ystar"])
+        ch.run(warmup)
+        ch.sync()
+        t0 = time.perf_counter()
+        avg_ms, _ = ch.time_steps(steps)
+        ch.sync()
+        wall = time.perf_counter() - t0
+    return {"mlups": n * steps / wall / 1e6, "kernel_ms": avg_ms, "nodes": n}
+
+
+COPY_KERNEL_TAG = "MulFunctor"  # torch.mul(src, 2.0, out=dst): reads and writes 8 B per element
+
+
+def traffic_probe(prec, workload, steps):
+    """A short run for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: `steps` step
+    launches after two warm-up steps, then one elementwise kernel of known
+    traffic (read 2 GiB, write 2 GiB) that calibrates the counters."""
+    import torch
+    lx, ly, lz, _ = WORKLOADS[workload]
+    with d3q19.Channel(lx, ly, lz, prec) as ch:
+        init_channel(ch, lx, ly, lz)
+        ch.run(2)
+        ch.sync()
+        ch.run(steps)
+        ch.sync()
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.mul(src, 2.0, out=dst)
+    torch.cuda.synchronize()
+    info = {"prec": prec, "workload": workload, "step_launches": steps, "copy_read_bytes": n * 8,
+            "copy_write_bytes": n * 8, "copy_kernel_tag": COPY_KERNEL_TAG}
+    print(json.dumps(info), flush=True)
+
+
+def _counter_means(csv_path, counter):
+    """Mean value of `counter` per dispatch for the step kernel and the copy kernel."""
+    import csv
+    step, copy = [], []
+    with open(csv_path, newline="") as fh:
+        for row in csv.DictReader(fh):
+            if row.get("Counter_Name") != counter:
+                continue
+            name, v = row.get("Kernel_Name", ""), float(row["Counter_Value"])
+            if "k_step<" in name:
+                step.append(v)
+            elif COPY_KERNEL_TAG in name:
+                copy.append(v)
+    if not step or not copy:
+        raise SystemExit(f"{csv_path}: no {counter} rows for the step ({len(step)}) or copy ({len(copy)}) kernel")
+    return sum(step) / len(step), sum(copy) / len(copy), len(step)
+
+
+def traffic_summary(prec, workload, fetch_csv, write_csv):
+    """HBM bytes per step launch from the two counter runs, each counter
+    scaled by what it reads for the copy kernel of known traffic."""
+    lx, ly, lz, _ = WORKLOADS[workload]
+    algo = lx * ly * lz * BYTES_PER_NODE[prec]
+    f_step, f_copy, nf = _counter_means(fetch_csv, "FETCH_SIZE")
+    w_step, w_copy, nw = _counter_means(write_csv, "WRITE_SIZE")
+    copy_bytes = (1 << 28) * 8
+    f_scale = copy_bytes / (f_copy * 1024.0)
+    w_scale = copy_bytes / (w_copy * 1024.0)
+    fetch = f_step * 1024.0 * f_scale
+    write = w_step * 1024.0 * w_scale
+    return {"prec": prec, "workload": workload, "step_dispatches": [nf, nw],
+            "fetch_kib_per_launch": f_step, "write_kib_per_launch": w_step,
+            "copy_fetch_kib": f_copy, "copy_write_kib": w_copy,
+            "fetch_scale": f_scale, "write_scale": w_scale,
+            "fetch_bytes": fetch, "write_bytes": write,
+            "hbm_bytes_per_launch": round(fetch + write), "algorithmic_bytes": algo,
+            "traffic_over_algorithmic": (fetch + write) / algo}
+
+
 def cpu_baseline(sample_steps, threads):
     """Oracle out-of-place step (collision + macrovar) on the full default grid."""
     import oracle
@@ -124,8 +215,25 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--variants", default="fp32,fp16", help="comma list of extra storage precisions ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-closure", action="store_true", help="skip the coarse closure-run throughput")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--traffic-probe", metavar="PREC", choices=sorted(BYTES_PER_NODE),
+                    help="run the short program profiled by rocprofv3 --pmc (5 step launches + calibration copy)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="turn two rocprofv3 counter_collection.csv files (FETCH_SIZE, WRITE_SIZE runs of "
+                         "--traffic-probe) into profiles/step_<prec>_<workload>_traffic.json")
+    ap.add_argument("--prec", default="fp64", choices=sorted(BYTES_PER_NODE), help="precision for --traffic-from")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args.traffic_probe, args.workload, 5)
+        return
+    if args.traffic_from:
+        s = traffic_summary(args.prec, args.workload, *args.traffic_from)
+        path = os.path.join(ROOT, "profiles", f"step_{args.prec}_{args.workload}_traffic.json")
+        with open(path, "w") as fh:
+            json.dump(s, fh, indent=1)
+        print(json.dumps(s, indent=1))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU slab decomposition is a design, not part of this run", file=sys.stderr)
@@ -145,6 +253,22 @@ def main():
                           "speedup_vs_fp64": round(v["mlups"] / main_m["mlups"], 3),
                           "mass_drift_per_node": v["mass_drift_per_node"]}
         log(f"bench: {prec} {v['mlups']:.1f} MLUPS, {v['roofline']['achieved']:.0f} GB/s")
+
+    closure = None
+    if args.workload == "c2" and not args.no_closure:
+        log("bench: 2x-coarse channel with the fused closure")
+        c = measure_closure(args.steps, args.warmup)
+        # one coarse step covers two fine steps of physical time on 1/8 of the nodes
+        per_phys = lambda mlups, nodes, steps_per_unit: nodes * steps_per_unit / mlups  # noqa: E731
+        fine_n = WORKLOADS["c2"][0] * WORKLOADS["c2"][1] * WORKLOADS["c2"][2]
+        closure = {"workload": WORKLOADS["c5"][3], "value": round(c["mlups"], 1), "unit": "MLUPS",
+                   "kernel_ms": round(c["kernel_ms"], 4),
+                   "time_per_physical_time_vs_fp64_fine": round(
+                       per_phys(main_m["mlups"], fine_n, 2) / per_phys(c["mlups"], c["nodes"], 1), 2)}
+        if "fp32" in variants:
+            closure["time_per_physical_time_vs_fp32_fine"] = round(
+                per_phys(variants["fp32"]["value"], fine_n, 2) / per_phys(c["mlups"], c["nodes"], 1), 2)
+        log(f"bench: closure coarse {c['mlups']:.1f} MLUPS")
 
     cpu = None
     if not args.no_cpu and rank == 0:
@@ -172,6 +296,7 @@ def main():
         "roofline": main_m["roofline"],
         "cpu_baseline": cpu,
         "variants": variants,
+        "closure_coarse": closure,
     }
     if rank == 0:
         print(json.dumps(out), flush=True)

@@ -111,6 +111,23 @@ __host__ __device__ __forceinline__ uint32_t src_of(const Nbr &b)
     return s;
 }
 
+// Index of the node shifted by -c_I in y and z only (the start of the
+// aligned source run that the vector kernels load for direction I).
+template <int I>
+__host__ __device__ __forceinline__ uint32_t src_yz(const Nbr &b)
+{
+    uint32_t s = b.n;
+    if constexpr (CY[I] == 1)
+        s += b.ym;
+    else if constexpr (CY[I] == -1)
+        s += b.yp;
+    if constexpr (CZ[I] == 1)
+        s += b.zm;
+    else if constexpr (CZ[I] == -1)
+        s += b.zp;
+    return s;
+}
+
 // Is x - c_I beyond a wall?
 template <int I>
 __host__ __device__ __forceinline__ bool walled(const Nbr &b)

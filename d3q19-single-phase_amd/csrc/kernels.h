@@ -27,6 +27,7 @@ struct StepArgs {
     const float *mlp;  // learned closure parameters (mode bit 8), see mlp.h
     float inv_ystar;   // u*/nu: y+ per lattice unit of wall distance
     float yscale;      // y+ feature divisor
+    int runs;          // 1: modes 0 and 1 of FP64/FP32 take the vector-run kernel when lx allows
 };
 
 struct MacroArgs {

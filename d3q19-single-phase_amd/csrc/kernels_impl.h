@@ -185,6 +185,164 @@ __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
     store_all<P>(dst, a.g.N, n, f);
 }
 
+// ---------------------------------------------------------------------------
+// The same step for VW = 2 consecutive nodes of one x row per thread, so
+// every population moves as one aligned vector (16 B in FP64, 8 B in FP32).
+// A run of VW nodes never straddles a row (lx % VW == 0), so its y/z source
+// run is one aligned vector too.  The +-x shift of the ten populations with
+// c_x != 0 moves one value across the run boundary: it comes from the
+// neighbouring lane's vector (a cross-lane shuffle), or from memory for the
+// first/last lane of a wave; a run at a wall takes the halfway bounce-back
+// value from its own node's opposite slot instead.  The values and the
+// arithmetic per node are those of k_step: FP64 results are identical bit
+// for bit; in FP32, built with FMA contraction, the compiler may contract
+// the two kernels differently (round-off-level differences).
+// Modes 0 and 1 (fields fused from F), FP64 and FP32 storage.
+template <int P>
+struct Vec;
+template <>
+struct Vec<0> {
+    static constexpr int W = 2;
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Vec<1> {
+    // two nodes (8 B): four would need ~230 VGPRs and spill
+    static constexpr int W = 2;
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+
+// F_I for the run n0 .. n0+VW-1 (b = neighbours of n0).  Every lane of the
+// wave calls this (the shuffles need them all); inactive lanes load nothing.
+template <int P, int I>
+__device__ __forceinline__ void pull_run(const typename Prec<P>::S *__restrict__ L, uint64_t N, const Nbr &b,
+                                         bool whi_last, bool active, bool first_lane, bool last_lane,
+                                         typename Prec<P>::T (&F)[Vec<P>::W][19])
+{
+    using S = typename Prec<P>::S;
+    using VT = typename Vec<P>::type;
+    constexpr int VW = Vec<P>::W;
+    const uint64_t s0 = (uint64_t)I * N + src_yz<I>(b);
+    VT v = VT(S(0));
+    if (active)
+        v = *(const VT *)(L + s0);
+    if constexpr (CX[I] == 0) {
+#pragma unroll
+        for (int k = 0; k < VW; k++)
+            F[k][I] = Prec<P>::template to_T<I>(v[k]);
+    } else if constexpr (CX[I] == 1) {
+        // node k pulls x - 1: element k - 1 of the run, element VW-1 of the
+        // previous lane's run for k = 0
+        S prev = __shfl_up(v[VW - 1], 1);
+        if (active && first_lane && !b.wlo)
+            prev = L[s0 - 1];
+        if (active && b.wlo)
+            prev = L[(uint64_t)OPP[I] * N + b.n];
+        F[0][I] = Prec<P>::template to_T<I>(prev);
+#pragma unroll
+        for (int k = 1; k < VW; k++)
+            F[k][I] = Prec<P>::template to_T<I>(v[k - 1]);
+    } else {
+        // node k pulls x + 1: element k + 1, element 0 of the next lane's run
+        // for k = VW - 1
+        S next = __shfl_down(v[0], 1);
+        if (active && last_lane && !whi_last)
+            next = L[s0 + VW];
+        if (active && whi_last)
+            next = L[(uint64_t)OPP[I] * N + b.n + (VW - 1)];
+#pragma unroll
+        for (int k = 0; k < VW - 1; k++)
+            F[k][I] = Prec<P>::template to_T<I>(v[k + 1]);
+        F[VW - 1][I] = Prec<P>::template to_T<I>(next);
+    }
+}
+
+template <int P, int... Is>
+__device__ __forceinline__ void pull_run_all(const typename Prec<P>::S *__restrict__ L, uint64_t N, const Nbr &b,
+                                             bool whi_last, bool active, bool first_lane, bool last_lane,
+                                             typename Prec<P>::T (&F)[Vec<P>::W][19],
+                                             std::integer_sequence<int, Is...>)
+{
+    (pull_run<P, Is>(L, N, b, whi_last, active, first_lane, last_lane, F), ...);
+}
+
+template <int P, int... Is>
+__device__ __forceinline__ void store_run_all(typename Prec<P>::S *__restrict__ L, uint64_t N, uint32_t n0,
+                                              const typename Prec<P>::T (&F)[Vec<P>::W][19],
+                                              std::integer_sequence<int, Is...>)
+{
+    using VT = typename Vec<P>::type;
+    constexpr int VW = Vec<P>::W;
+    auto one = [&](auto ic) {
+        constexpr int I = decltype(ic)::value;
+        VT v;
+#pragma unroll
+        for (int k = 0; k < VW; k++)
+            v[k] = F[k][I];
+        __builtin_nontemporal_store(v, (VT *)(L + (uint64_t)I * N + n0));
+    };
+    (one(std::integral_constant<int, Is>{}), ...);
+}
+
+template <int P, int MODE>
+__global__ void __launch_bounds__(BLOCK) k_step_run(StepArgs a)
+{
+    static_assert(P == 0 || P == 1, "vector runs for FP64 and FP32 storage");
+    static_assert(MODE == 0 || MODE == 1, "fields fused from F only");
+    using S = typename Prec<P>::S;
+    using T = typename Prec<P>::T;
+    constexpr int VW = Vec<P>::W;
+    const uint32_t nruns = (uint32_t)(a.g.N / VW);
+    const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+    const bool active = t < nruns;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t n0 = active ? t * VW : 0u;
+    const Nbr b = make_nbr(n0, a.g);
+    const uint32_t ix0 = n0 - fdiv(n0, a.g.dlx) * a.g.lx;
+    const bool whi_last = (ix0 + VW == a.g.lx);
+    const bool first_lane = (lane == 0);
+    const bool last_lane = (lane == 63u) || (t + 1 >= nruns);
+
+    T F[VW][19];
+    pull_run_all<P>((const S *)a.src, a.g.N, b, whi_last, active, first_lane, last_lane, F,
+                    std::make_integer_sequence<int, 19>{});
+    if (!active)
+        return;
+
+    Rates<T> r;
+    r.s1 = (T)a.r.s1;
+    r.s2 = (T)a.r.s2;
+    r.s4 = (T)a.r.s4;
+    r.s9 = (T)a.r.s9;
+    r.s10 = (T)a.r.s10;
+    r.s13 = (T)a.r.s13;
+    r.s16 = (T)a.r.s16;
+    r.omegepsl = (T)a.r.omegepsl;
+    r.omegepslj = (T)a.r.omegepslj;
+    r.omegxx = (T)a.r.omegxx;
+    r.fx = (T)a.r.fx;
+    r.fy = (T)a.r.fy;
+    r.fz = (T)a.r.fz;
+
+#pragma unroll
+    for (int k = 0; k < VW; k++) {
+        T rho9, ux9, uy9, uz9;
+        macro_sums(F[k], rho9, ux9, uy9, uz9);
+        ux9 = ux9 + r.fx / T(2.); // collision.f90:415-417
+        uy9 = uy9 + r.fy / T(2.);
+        uz9 = uz9 + r.fz / T(2.);
+        if constexpr (MODE == 1)
+            rho9 = rho9 - (T)a.rho_off; // collision.f90:508
+        if constexpr (P == 0) {
+            mrt_collide(F[k], rho9, ux9, uy9, uz9, r);
+        } else {
+            const T drho = MODE == 1 ? -(T)a.rho_off : T(0);
+            mrt_collide_delta(F[k], rho9, ux9, uy9, uz9, drho, T(0), T(0), T(0), r);
+        }
+    }
+    store_run_all<P>((S *)a.dst, a.g.N, n0, F, std::make_integer_sequence<int, 19>{});
+}
+
 // macrovar / rhoupdat / materialisation of the fields of lattice L.
 // WHAT 0: rho, ux, uy, uz = macrovar(F) with rho - rho_off.
 // WHAT 1: rho = sum F (rhoupdat); max |rho_new - rho_old| into *errbits.
@@ -502,9 +660,30 @@ static inline uint32_t nblocks(uint64_t n)
     return (uint32_t)((n + BLOCK - 1) / BLOCK);
 }
 
+// Does the vector-run step kernel apply to this launch?
+template <int P>
+static inline bool use_runs(const StepArgs &a, int mode)
+{
+    if constexpr (P == 2) {
+        return false;
+    } else {
+        return a.runs && (mode == 0 || mode == 1) && a.g.lx % Vec<P>::W == 0 && a.g.N / Vec<P>::W < 0xffffffffull;
+    }
+}
+
 template <int P>
 hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
 {
+    if constexpr (P != 2) {
+        if (use_runs<P>(a, mode)) {
+            const dim3 grid(nblocks<P>(a.g.N / Vec<P>::W)), blk(BLOCK);
+            if (mode == 0)
+                hipLaunchKernelGGL((k_step_run<P, 0>), grid, blk, 0, s, a);
+            else
+                hipLaunchKernelGGL((k_step_run<P, 1>), grid, blk, 0, s, a);
+            return hipGetLastError();
+        }
+    }
     const dim3 grid(nblocks<P>(a.g.N)), blk(BLOCK);
     switch (mode) {
     case 0:

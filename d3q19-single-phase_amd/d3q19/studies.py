@@ -143,10 +143,35 @@ def _start(ch, perturb, seed):
     ch.macrovar()
 
 
+# statistc2 columns that change sign under the reflection x -> lx - x (u_x -> -u_x)
+_ANTISYMMETRIC = (2, 8, 9)
+
+
+def fold(p):
+    """Half-channel profile from both walls: plane i and plane lx-1-i averaged
+    (sign flipped for <u_x>, <u_x u_z>, <u_x u_y>)."""
+    h = p.shape[0] // 2
+    q = 0.5 * (p[:h] + p[::-1][:h])
+    for c in _ANTISYMMETRIC:
+        q[:, c] = 0.5 * (p[:h, c] - p[::-1][:h, c])
+    return q
+
+
 def _stat_errors(p, ref):
     """Mean-profile and second-moment errors of a coarse run against the
-    filtered fine statistics, each as max over planes |v - v_ref| / max |v_ref|."""
-    return compare(p, ref)
+    filtered fine statistics on the folded half channel, each as
+    max over planes |v - v_ref| / max |v_ref|."""
+    return compare(fold(p), fold(ref))
+
+
+def _asymmetry(p):
+    """Difference between the two walls' halves of one run (its own sampling
+    noise), in the units of _stat_errors."""
+    h = p.shape[0] // 2
+    a, b = p[:h].copy(), p[::-1][:h].copy()
+    for c in _ANTISYMMETRIC:
+        b[:, c] = -b[:, c]
+    return compare(a, b)
 
 
 def _coarse_run(lx, ly, lz, nu_c, steps, stats_from, stats_every, perturb, seed, closure=None, log=None):
@@ -171,7 +196,7 @@ def _coarse_run(lx, ly, lz, nu_c, steps, stats_from, stats_every, perturb, seed,
         return {"stats": stats, "samples": n, "wall_s": wall, "mlups": lx * ly * lz * steps / wall / 1e6}
 
 
-def closure_study(lx=192, ly=768, lz=384, spinup=40000, window=40000, stats_every=100, snapshots=8,
+def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_every=100, snapshots=8,
                   iterations=4000, batch=65536, lr=2e-3, seed=2, perturb=TURBULENT_START, log=None):
     """Learned closure against plain MRT (a posteriori).
 
@@ -239,6 +264,8 @@ def closure_study(lx=192, ly=768, lz=384, spinup=40000, window=40000, stats_ever
             fine_filt, _ = fine.stats_read_filtered()
         report["fine"] = {"wall_s": round(wall_fine, 2), "samples": n_fine, "snapshot_s": round(t_snap, 2),
                           "mlups": round(lx * ly * lz * total / wall_fine / 1e6, 1),
+                          # the two walls of the filtered fine statistics against each other
+                          "wall_to_wall_asymmetry": _asymmetry(fine_filt),
                           # filtered against plane-pair means of the unfiltered statistics:
                           # zero for U+ (the filter is linear), the subfilter share for the rest
                           "filter_effect": _stat_errors(fine_filt, 0.5 * (fine_stats[0::2] + fine_stats[1::2]))}
@@ -280,8 +307,8 @@ def closure_study(lx=192, ly=768, lz=384, spinup=40000, window=40000, stats_ever
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("study", choices=["precision", "closure"])
-    ap.add_argument("--spinup", type=int, default=40000, help="closure: fine steps before the statistics window")
-    ap.add_argument("--window", type=int, default=40000, help="closure: fine steps of the statistics window")
+    ap.add_argument("--spinup", type=int, default=100000, help="closure: fine steps before the statistics window")
+    ap.add_argument("--window", type=int, default=100000, help="closure: fine steps of the statistics window")
     ap.add_argument("--snapshots", type=int, default=8, help="closure: filtered fine snapshots for training")
     ap.add_argument("--iterations", type=int, default=4000, help="closure: Adam iterations")
     ap.add_argument("--batch", type=int, default=65536, help="closure: samples per iteration")

@@ -71,6 +71,83 @@ __global__ void __launch_bounds__(256) flat_copy_chunk(const dvec2* __restrict__
     for (int k = 0; k < 8; k++) { uint64_t i = b + k * 256 + threadIdx.x; if (i < n2) __builtin_nontemporal_store(v[k], &d[i]); }
 }
 
+// 19-stream SoA copy with two nodes per thread (16-byte aligned loads and
+// stores).  With SHIFT, the ten populations with c_x = +-1 are pulled from
+// x -+ 1 as the step kernel pulls them: the value that crosses the lane's
+// pair comes from the neighbouring lane (ds_bpermute), and the first / last
+// lane of a wave loads it from memory.  Every index is checked against N.
+template <int SHIFT>
+__global__ void __launch_bounds__(256) soa_pair(const double* __restrict__ s, double* __restrict__ d, uint64_t N) {
+    const uint64_t n2 = N / 2;
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool active = i < n2;
+    const uint32_t lane = threadIdx.x & 63;
+    dvec2 f[19];
+#pragma unroll
+    for (int p = 0; p < 19; p++) f[p] = active ? ((const dvec2*)(s + p * N))[i] : dvec2{0, 0};
+    if (SHIFT) {
+        const bool first = (lane == 0);
+        const bool last = (lane == 63) || (i + 1 >= n2);
+#pragma unroll
+        for (int p = 0; p < 19; p++) {
+            const int cx = (p == 1 || p == 7 || p == 9 || p == 11 || p == 13) ? 1
+                         : (p == 2 || p == 8 || p == 10 || p == 12 || p == 14) ? -1 : 0;
+            if (cx == 1) {
+                double prev = __shfl_up(f[p].y, 1);
+                if (first) prev = (active && i > 0) ? s[p * N + 2 * i - 1] : 0.0;
+                f[p] = dvec2{prev, f[p].x};
+            } else if (cx == -1) {
+                double next = __shfl_down(f[p].x, 1);
+                if (last) next = (active && 2 * i + 2 < N) ? s[p * N + 2 * i + 2] : 0.0;
+                f[p] = dvec2{f[p].y, next};
+            }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int p = 0; p < 19; p++) __builtin_nontemporal_store(f[p] * 1.0000001, ((dvec2*)(d + p * N)) + i);
+}
+
+// AoSoA with tiles of T nodes (T = lx: one wall-normal row per tile, its 19
+// populations contiguous) and two nodes per thread (16-byte accesses).  With
+// SHIFT the ten c_x != 0 populations take their +-1 neighbour across the pair
+// boundary from the adjacent lane (or from memory on the first/last lane of
+// a wave), as a row-tiled pull would.  N % T == 0 and T even.
+template <int T, int SHIFT>
+__global__ void __launch_bounds__(256) aosoa_pair(const double* __restrict__ s, double* __restrict__ d, uint64_t N) {
+    const uint64_t n2 = N / 2;
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool active = i < n2;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t n = active ? 2 * i : 0;
+    const uint64_t base = (n / T) * (19ull * T) + (n % T);
+    dvec2 f[19];
+#pragma unroll
+    for (int p = 0; p < 19; p++) f[p] = active ? *(const dvec2*)(s + base + (uint64_t)p * T) : dvec2{0, 0};
+    if (SHIFT) {
+        const bool first = (lane == 0) || (n % T == 0);
+        const bool last = (lane == 63) || (i + 1 >= n2) || (n % T == T - 2);
+#pragma unroll
+        for (int p = 0; p < 19; p++) {
+            const int cx = (p == 1 || p == 7 || p == 9 || p == 11 || p == 13) ? 1
+                         : (p == 2 || p == 8 || p == 10 || p == 12 || p == 14) ? -1 : 0;
+            if (cx == 1) {
+                double prev = __shfl_up(f[p].y, 1);
+                // first node of a row: the wall value (own opposite slot)
+                if (first) prev = active ? s[base + (uint64_t)(p + 1 == 19 ? 0 : p + 1) * T] : 0.0;
+                f[p] = dvec2{prev, f[p].x};
+            } else if (cx == -1) {
+                double next = __shfl_down(f[p].x, 1);
+                if (last) next = active ? s[base + (uint64_t)(p + 1 == 19 ? 0 : p + 1) * T + 1] : 0.0;
+                f[p] = dvec2{f[p].y, next};
+            }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int p = 0; p < 19; p++) __builtin_nontemporal_store(f[p] * 1.0000001, (dvec2*)(d + base + (uint64_t)p * T));
+}
+
 typedef void (*KF)(const double*, double*, uint64_t);
 
 int main() {
@@ -93,9 +170,18 @@ int main() {
         {"soa_read19", soa_read, 1.0 * bytes},
         {"soa_write19 nt", soa_write<1>, 1.0 * bytes},
         {"soa_write19 plain", soa_write<0>, 1.0 * bytes},
+        {"soa_pair19 16B nt", soa_pair<0>, 2.0 * bytes},
+        {"soa_pair19 16B x-shift nt", soa_pair<1>, 2.0 * bytes},
+        {"aosoa T=192 pair 16B nt", aosoa_pair<192, 0>, 2.0 * bytes},
+        {"aosoa T=192 pair 16B x-shift", aosoa_pair<192, 1>, 2.0 * bytes},
+        {"aosoa T=64 pair 16B nt", aosoa_pair<64, 0>, 2.0 * bytes},
     };
-    const uint32_t grid = (uint32_t)((N + 255) / 256);
     for (auto& v : vs) {
+        // the pair kernels cover two nodes per thread
+        const bool pair = v.k == soa_pair<0> || v.k == soa_pair<1> || v.k == aosoa_pair<192, 0> ||
+                          v.k == aosoa_pair<192, 1> || v.k == aosoa_pair<64, 0>;
+        const uint64_t threads = pair ? N / 2 : N;
+        const uint32_t grid = (uint32_t)((threads + 255) / 256);
         for (int w = 0; w < 2; w++) hipLaunchKernelGGL(v.k, dim3(grid), dim3(256), 0, 0, A, B, N);
         CK(hipDeviceSynchronize()); CK(hipEventRecord(e0));
         for (int r = 0; r < reps; r++) hipLaunchKernelGGL(v.k, dim3(grid), dim3(256), 0, 0, (r & 1) ? B : A, (r & 1) ? A : B, N);

@@ -68,7 +68,11 @@ const char *lbm_last_error(void);
 /* Allocates the device lattices for one rank's (lx,ly,lz) block -- replaces
  * Channel-Flow/para.f90:418-447 (allocarray) for f, rho, ux, uy, uz, force.
  * The populations start at zero; the macroscopic fields read as macrovar of
- * the lattice until a call sets them. */
+ * the lattice until a call sets them.  The step runs one node per thread;
+ * the environment variable LBM_STEP_KERNEL=runs, read here, makes FP64 and
+ * FP32 handles step two nodes per thread with 2-wide vector accesses when lx
+ * is even (FP64 results identical bit for bit, FP32 to round-off; measured
+ * slower on the 192x768x384 channel). */
 int lbm_create(const lbm_config *cfg, lbm_channel **out);
 int lbm_destroy(lbm_channel *h);
 

@@ -118,6 +118,113 @@ static bool check_sampled(uint32_t lx, uint32_t ly, uint32_t lz)
     return true;
 }
 
+// The vector-run pull of k_step_run (VW nodes per thread, 256-thread blocks,
+// 64-lane waves), emulated with slot indices as values: every load is in
+// bounds and aligned, every shuffle reads the lane the kernel reads, and each
+// node receives exactly the slot the one-node gather pulls.
+struct RunCtx {
+    Geom g;
+    uint64_t nruns;
+};
+
+template <int VW, int I>
+static int64_t run_start(const RunCtx &c, uint64_t t)
+{
+    if (t >= c.nruns)
+        return -1; // inactive lane: loads nothing
+    const Nbr b = make_nbr((uint32_t)(t * VW), c.g);
+    return (int64_t)((uint64_t)I * c.g.N + src_yz<I>(b));
+}
+
+template <int VW, int I>
+static bool check_run_dir(const RunCtx &c, uint64_t t)
+{
+    const uint64_t N = c.g.N;
+    const bool active = t < c.nruns;
+    const uint32_t lane = (uint32_t)(t & 63);
+    const uint32_t n0 = active ? (uint32_t)(t * VW) : 0u;
+    const Nbr b = make_nbr(n0, c.g);
+    const uint32_t ix0 = n0 - fdiv(n0, c.g.dlx) * c.g.lx;
+    const bool whi_last = (ix0 + VW == c.g.lx);
+    const bool first = (lane == 0), last = (lane == 63) || (t + 1 >= c.nruns);
+    const int64_t s0 = run_start<VW, I>(c, t);
+    if (active && (s0 < 0 || (uint64_t)s0 + VW > 19 * N || s0 % VW != 0)) {
+        fprintf(stderr, "run %llu dir %d: vector load at %lld out of range or unaligned\n", (unsigned long long)t, I,
+                (long long)s0);
+        return false;
+    }
+    int64_t F[VW];
+    if constexpr (CX[I] == 0) {
+        for (int k = 0; k < VW; k++)
+            F[k] = s0 + k;
+    } else if constexpr (CX[I] == 1) {
+        // __shfl_up: lane - 1 of the same wave, own value on lane 0
+        const int64_t src = lane > 0 ? run_start<VW, I>(c, t - 1) : s0;
+        int64_t prev = src < 0 ? -1 : src + VW - 1;
+        if (active && first && !b.wlo) {
+            if (s0 < 1) {
+                fprintf(stderr, "run %llu dir %d: edge load below 0\n", (unsigned long long)t, I);
+                return false;
+            }
+            prev = s0 - 1;
+        }
+        if (active && b.wlo)
+            prev = (int64_t)((uint64_t)OPP[I] * N + b.n);
+        F[0] = prev;
+        for (int k = 1; k < VW; k++)
+            F[k] = s0 + k - 1;
+    } else {
+        // __shfl_down: lane + 1 of the same wave, own value on lane 63
+        const int64_t src = lane < 63 ? run_start<VW, I>(c, t + 1) : s0;
+        int64_t next = src < 0 ? -1 : src;
+        if (active && last && !whi_last) {
+            if ((uint64_t)s0 + VW >= 19 * N) {
+                fprintf(stderr, "run %llu dir %d: edge load past the end\n", (unsigned long long)t, I);
+                return false;
+            }
+            next = s0 + VW;
+        }
+        if (active && whi_last)
+            next = (int64_t)((uint64_t)OPP[I] * N + b.n + (VW - 1));
+        for (int k = 0; k < VW - 1; k++)
+            F[k] = s0 + k + 1;
+        F[VW - 1] = next;
+    }
+    if (!active)
+        return true;
+    for (int k = 0; k < VW; k++) {
+        const Nbr bn = make_nbr(n0 + k, c.g);
+        const int64_t want = walled<I>(bn) ? (int64_t)((uint64_t)OPP[I] * N + bn.n)
+                                           : (int64_t)((uint64_t)I * N + src_of<I>(bn));
+        if (F[k] != want) {
+            fprintf(stderr, "run %llu node %u dir %d: pulled slot %lld, gather wants %lld\n", (unsigned long long)t,
+                    n0 + k, I, (long long)F[k], (long long)want);
+            return false;
+        }
+    }
+    return true;
+}
+
+template <int VW, int... Is>
+static bool check_run_all(const RunCtx &c, uint64_t t, std::integer_sequence<int, Is...>)
+{
+    return (check_run_dir<VW, Is>(c, t) && ...);
+}
+
+template <int VW>
+static bool check_runs(uint32_t lx, uint32_t ly, uint32_t lz)
+{
+    RunCtx c{geom(lx, ly, lz), 0};
+    if (lx % VW != 0)
+        return true; // the launcher keeps one node per thread for this shape
+    c.nruns = c.g.N / VW;
+    const uint64_t threads = (c.nruns + 255) / 256 * 256;
+    for (uint64_t t = 0; t < threads; t++)
+        if (!check_run_all<VW>(c, t, std::make_integer_sequence<int, 19>{}))
+            return false;
+    return true;
+}
+
 static bool check_remap(uint32_t nb)
 {
     std::vector<uint8_t> seen(nb, 0);
@@ -143,6 +250,11 @@ int main()
             return 1;
         }
         printf("ok full %u x %u x %u\n", s[0], s[1], s[2]);
+        if (!check_runs<2>(s[0], s[1], s[2]) || !check_runs<4>(s[0], s[1], s[2])) {
+            fprintf(stderr, "FAIL runs %u %u %u\n", s[0], s[1], s[2]);
+            return 1;
+        }
+        printf("ok runs %u x %u x %u\n", s[0], s[1], s[2]);
     }
     const uint32_t big[][3] = {{512, 2048, 1024}, {256, 1024, 512}, {1023, 1021, 1025}};
     for (auto &s : big) {

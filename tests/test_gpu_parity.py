@@ -248,6 +248,34 @@ def test_two_runs_are_bitwise_identical():
     assert_bitwise(outs[0], outs[1], "determinism")
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape", [(12, 10, 6), (48, 40, 24), (64, 3, 2)])
+def test_vector_run_kernel_equals_one_node_kernel(prec, shape, monkeypatch):
+    """The step kernel with two nodes per thread (even lx) against the
+    one-node kernel, including an avedensity (mode 1) step: bit for bit in
+    FP64 (no FMA contraction); in FP32, where the compiler may contract the
+    same expressions differently in the two kernels, to FP32 round-off."""
+    lx, ly, lz = shape
+    vel = workloads.laminar_parabola_noise(lx, ly, lz, seed=21)
+    outs = []
+    for kernel in ("node", "runs"):
+        monkeypatch.setenv("LBM_STEP_KERNEL", kernel)
+        with d3q19.Channel(lx, ly, lz, prec, laminar=True) as ch:
+            ch.set_macros(ux=vel[0], uy=vel[1], uz=vel[2])
+            ch.FORCING()
+            ch.initpop()
+            ch.macrovar()
+            ch.run(20)
+            ch.avedensity()
+            ch.run(20)
+            outs.append(ch.download_post())
+    if prec == "fp64":
+        assert_bitwise(outs[0], outs[1], f"{prec} runs vs node {shape}")
+    else:
+        scale = np.max(np.abs(outs[0]))
+        assert np.max(np.abs(outs[1] - outs[0])) < 1e-6 * scale
+
+
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3)])
 def test_reduced_precision_tracks_fp64(prec, tol):
     lx, ly, lz = 32, 64, 32
