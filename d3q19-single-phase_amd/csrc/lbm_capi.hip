@@ -95,6 +95,7 @@ struct lbm_trainer {
     float *partial, *grad;  // per-wave partial gradients, summed gradient (+ loss)
     uint32_t max_waves;
     int step;
+    int target_mode; // LBM_TARGET_LOCAL or LBM_TARGET_PLANE for lbm_trainer_add_filtered
 };
 
 // Particle set on the device (see particles.h).
@@ -1340,8 +1341,10 @@ int lbm_trainer_add_filtered(lbm_trainer *t, lbm_channel *fine, double nu_coarse
         return fail(LBM_ERR_ARG, "lbm_trainer_add_filtered: %lld samples would exceed the capacity %lld",
                     t->count + Nc, t->capacity);
     HIP_TRY(hipStreamSynchronize(fine->stream));
+    // 10 filtered fields, then (per-plane targets) tau:S and S:S per node
+    const bool plane = (t->target_mode == LBM_TARGET_PLANE);
     float *fields = nullptr;
-    HIP_TRY(hipMalloc((void **)&fields, (size_t)Nc * 10 * sizeof(float)));
+    HIP_TRY(hipMalloc((void **)&fields, (size_t)Nc * (plane ? 12 : 10) * sizeof(float)));
     FilterArgs fa;
     fa.gf = fine->g;
     fa.L = fine->lat[fine->cur];
@@ -1373,8 +1376,12 @@ int lbm_trainer_add_filtered(lbm_trainer *t, lbm_channel *fine, double nu_coarse
     ft.feat = t->feat;
     ft.target = t->target;
     ft.offset = (uint64_t)t->count;
+    ft.ts = plane ? fields + (size_t)Nc * 10 : nullptr;
+    ft.ss = plane ? fields + (size_t)Nc * 11 : nullptr;
     if (e == hipSuccess)
         e = mlp_features(ft, t->stream);
+    if (e == hipSuccess && plane)
+        e = mlp_plane_targets(lxc, lyc, lzc, ft.ts, ft.ss, ft.nu, t->target, ft.offset, t->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(t->stream);
     (void)hipFree(fields);
@@ -1383,6 +1390,16 @@ int lbm_trainer_add_filtered(lbm_trainer *t, lbm_channel *fine, double nu_coarse
     t->count += Nc;
     if (added)
         *added = Nc;
+    return LBM_OK;
+}
+
+int lbm_trainer_set_target(lbm_trainer *t, int mode)
+{
+    CHECK_T(t);
+    if (mode != LBM_TARGET_LOCAL && mode != LBM_TARGET_PLANE)
+        return fail(LBM_ERR_ARG, "lbm_trainer_set_target: mode %d is neither LBM_TARGET_LOCAL nor LBM_TARGET_PLANE",
+                    mode);
+    t->target_mode = mode;
     return LBM_OK;
 }
 

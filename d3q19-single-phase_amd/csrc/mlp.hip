@@ -6,7 +6,9 @@
 // S is taken by central differences of the filtered velocity (no-slip walls
 // half a cell beyond the first and last nodes), the subgrid stress is
 // tau_ab = <u_a u_b> - <u_a><u_b>, and the target is the eddy viscosity that
-// best carries it, nu_t = -tau:S / (2 S:S), as nu_t/nu clipped to [0, 10].
+// best carries it, nu_t = -tau:S / (2 S:S), as nu_t/nu clipped to [-10, 10]
+// (backscatter is kept: the network's non-negative output then fits the
+// conditional mean instead of the mean of the positive part).
 // The features are the non-equilibrium moments the coarse MRT would carry
 // for that strain to first order (Chapman-Enskog: m_neq = -(2/(3 s)) times
 // the strain combination of the moment; zero for e, eps, 3pi_xx, pi_ww), in
@@ -108,7 +110,11 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a)
             ss += S[c][d] * S[c][d];
         }
     float y = ss > 1e-30f ? -ts / (2.0f * ss) / a.nu : 0.0f;
-    y = fminf(fmaxf(y, 0.0f), 10.0f);
+    y = fminf(fmaxf(y, -10.0f), 10.0f);
+    if (a.ts) {
+        a.ts[n] = ts;
+        a.ss[n] = ss;
+    }
     // first-order non-equilibrium moments of the coarse MRT for strain S
     const float k = -2.0f / (3.0f * a.s9);
     const float neq[9] = {0.0f,
@@ -268,6 +274,45 @@ hipError_t mlp_features(const FeatArgs &a, hipStream_t s)
 {
     const uint32_t N = a.lx * a.ly * a.lz;
     hipLaunchKernelGGL(k_features, dim3((N + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// One block per coarse x plane: FP64 sums of tau:S and S:S over the plane in
+// a fixed order (strided per thread, then a fixed LDS tree), the plane's
+// least-squares eddy viscosity, written to every node of the plane.
+__global__ void __launch_bounds__(256) k_plane_targets(uint32_t lx, uint32_t ly, uint32_t lz,
+                                                       const float *__restrict__ ts, const float *__restrict__ ss,
+                                                       float nu, float *__restrict__ target, uint64_t offset)
+{
+    const uint32_t ix = blockIdx.x;
+    const uint32_t rows = ly * lz;
+    double a = 0.0, b = 0.0;
+    for (uint32_t r = threadIdx.x; r < rows; r += 256) {
+        const uint64_t n = ix + (uint64_t)lx * r;
+        a += (double)ts[n];
+        b += (double)ss[n];
+    }
+    __shared__ double sa[256], sb[256];
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            sa[threadIdx.x] += sa[threadIdx.x + h];
+            sb[threadIdx.x] += sb[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    float y = sb[0] > 1e-30 ? (float)(-sa[0] / (2.0 * sb[0]) / (double)nu) : 0.0f;
+    y = fminf(fmaxf(y, -10.0f), 10.0f);
+    for (uint32_t r = threadIdx.x; r < rows; r += 256)
+        target[offset + ix + (uint64_t)lx * r] = y;
+}
+
+hipError_t mlp_plane_targets(uint32_t lx, uint32_t ly, uint32_t lz, const float *ts, const float *ss, float nu,
+                             float *target, uint64_t offset, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_plane_targets, dim3(lx), dim3(256), 0, s, lx, ly, lz, ts, ss, nu, target, offset);
     return hipGetLastError();
 }
 

@@ -28,7 +28,13 @@ struct FeatArgs {
     float *feat;          // samples x 10
     float *target;        // samples
     uint64_t offset;      // first sample written
+    float *ts, *ss;       // nullable: per-node tau:S and S:S for the per-plane target
 };
+
+// Target of every node of a coarse x plane replaced by the plane's
+// least-squares eddy viscosity -sum(tau:S) / (2 sum(S:S)) / nu.
+hipError_t mlp_plane_targets(uint32_t lx, uint32_t ly, uint32_t lz, const float *ts, const float *ss, float nu,
+                             float *target, uint64_t offset, hipStream_t s);
 
 template <int P>
 struct MlpLaunch {

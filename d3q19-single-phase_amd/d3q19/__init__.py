@@ -93,6 +93,7 @@ _SIGS = {
     "lbm_closure_clear": [_H],
     "lbm_trainer_create": [ctypes.c_int, ctypes.c_longlong, _D, ctypes.POINTER(ctypes.c_void_p)],
     "lbm_trainer_destroy": [_H],
+    "lbm_trainer_set_target": [_H, ctypes.c_int],
     "lbm_trainer_add_filtered": [_H, _H, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_longlong)],
     "lbm_trainer_add_samples": [_H, _D, _D, ctypes.c_longlong],
@@ -444,6 +445,12 @@ class Trainer:
 
     def __exit__(self, *exc):
         self.close()
+
+    def set_target(self, mode):
+        """Target of later add_filtered calls: "local" (each node's own
+        -tau:S/(2 S:S)/nu) or "plane" (the least-squares value of its
+        wall-parallel plane)."""
+        _call("lbm_trainer_set_target", self._h, {"local": 0, "plane": 1}[mode])
 
     def add_filtered(self, fine, nu_coarse, s9_coarse, ystar_coarse):
         n = ctypes.c_longlong()

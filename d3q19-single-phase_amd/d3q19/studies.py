@@ -218,10 +218,12 @@ def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_ev
     snap_at = sorted({int(round(s / stats_every)) * stats_every
                       for s in np.linspace(spinup // 2, total, snapshots)})
     Nc = lx * ly * lz // 8
-    trainer = None
+    modes = ("local", "plane")
+    trainers = {}
     report = {"grid": [lx, ly, lz], "coarse_grid": [lx // 2, ly // 2, lz // 2], "spinup": spinup, "window": window,
               "stats_every": stats_every, "perturbation": {k: list(v) if isinstance(v, tuple) else v
                                                           for k, v in perturb.items()}}
+    trained = {}
     try:
         with Channel(lx, ly, lz, "fp32", mrttype=3) as fine:
             _start(fine, perturb, seed)
@@ -242,7 +244,7 @@ def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_ev
                 if done in snap_at:
                     fine.sync()
                     ts = time.perf_counter()
-                    if trainer is None:
+                    if not trainers:
                         # input scales: spread of each feature over the first snapshot
                         with Trainer(Nc, closure_params(nu=nu_c)) as probe:
                             probe.add_filtered(fine, nu_c, s9_c, ystar_c)
@@ -250,9 +252,12 @@ def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_ev
                         sc = f[:, :9].astype(np.float64).std(axis=0)
                         sc[~(sc > 1e-12)] = 1.0
                         del f
-                        trainer = Trainer(len(snap_at) * Nc, closure_params(seed=1, scales=sc, nu=nu_c))
+                        for mode in modes:
+                            trainers[mode] = Trainer(len(snap_at) * Nc, closure_params(seed=1, scales=sc, nu=nu_c))
+                            trainers[mode].set_target(mode)
                         report["input_scales"] = [float(v) for v in sc]
-                    trainer.add_filtered(fine, nu_c, s9_c, ystar_c)
+                    for tr in trainers.values():
+                        tr.add_filtered(fine, nu_c, s9_c, ystar_c)
                     t_snap += time.perf_counter() - ts
                 if done % max(total // 10, 1) == 0:
                     s = fine.statistc2()
@@ -269,34 +274,43 @@ def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_ev
                           # filtered against plane-pair means of the unfiltered statistics:
                           # zero for U+ (the filter is linear), the subfilter share for the rest
                           "filter_effect": _stat_errors(fine_filt, 0.5 * (fine_stats[0::2] + fine_stats[1::2]))}
-        # training on all snapshots but the last; the last one is held out
-        n_all = trainer.count()
-        n_train = n_all - Nc
-        _, t_all = trainer.samples()
-        held = t_all[n_train:].astype(np.float64)
-        del t_all
-        mse_zero = float(np.mean(held ** 2))
-        mse_init = float(np.mean((trainer.predict(n_train, Nc) - held) ** 2))
-        t1 = time.perf_counter()
-        loss = trainer.train(0, n_train, iterations=iterations, batch=min(batch, n_train), lr=lr)
-        t_train = time.perf_counter() - t1
-        mse_net = float(np.mean((trainer.predict(n_train, Nc) - held) ** 2))
-        trained = trainer.params()
-        report["training"] = {"samples": int(n_train), "held_out": int(Nc), "iterations": iterations,
-                              "batch": min(batch, n_train), "lr": lr, "last_batch_loss": float(loss),
-                              "wall_s": round(t_train, 2), "held_out_mse": {"zero": mse_zero, "initial": mse_init,
-                                                                            "trained": mse_net},
-                              "held_out_target_mean": float(held.mean())}
-        say(f"training: held-out mse zero {mse_zero:.4f} initial {mse_init:.4f} trained {mse_net:.4f}")
+        # mean nu_t/nu per coarse plane (samples are ordered x fastest), both walls folded
+        plane = lambda v: v.reshape(lz // 2, ly // 2, lx // 2).mean(axis=(0, 1))  # noqa: E731
+        half = lambda v: 0.5 * (v[:lx // 4] + v[::-1][:lx // 4])  # noqa: E731
+        for mode, tr in trainers.items():
+            # training on all snapshots but the last; the last one is held out
+            n_train = tr.count() - Nc
+            _, t_all = tr.samples()
+            held = t_all[n_train:].astype(np.float64)
+            del t_all
+            mse_zero = float(np.mean(held ** 2))
+            mse_init = float(np.mean((tr.predict(n_train, Nc) - held) ** 2))
+            t1 = time.perf_counter()
+            loss = tr.train(0, n_train, iterations=iterations, batch=min(batch, n_train), lr=lr)
+            t_train = time.perf_counter() - t1
+            pred = tr.predict(n_train, Nc).astype(np.float64)
+            mse_net = float(np.mean((pred - held) ** 2))
+            trained[mode] = tr.params()
+            report["training_" + mode] = {
+                "samples": int(n_train), "held_out": int(Nc), "iterations": iterations,
+                "batch": min(batch, n_train), "lr": lr, "last_batch_loss": float(loss),
+                "wall_s": round(t_train, 2),
+                "held_out_mse": {"zero": mse_zero, "initial": mse_init, "trained": mse_net},
+                "held_out_target_mean": float(held.mean()),
+                "nu_t_over_nu_by_plane": {"target": [round(float(v), 4) for v in half(plane(held))],
+                                          "predicted": [round(float(v), 4) for v in half(plane(pred))]}}
+            say(f"training ({mode}): held-out mse zero {mse_zero:.3e} initial {mse_init:.3e} trained {mse_net:.3e}")
     finally:
-        if trainer is not None:
-            trainer.close()
+        for tr in trainers.values():
+            tr.close()
     ref = fine_filt
-    for name, closure in (("coarse_plain", None), ("coarse_closure", trained)):
+    runs = [("coarse_plain", None)] + [("coarse_closure_" + m, trained[m]) for m in modes]
+    for name, closure in runs:
         r = _coarse_run(lx // 2, ly // 2, lz // 2, nu_c, total // 2, spinup // 2, stats_every // 2, perturb, seed,
                         closure=closure, log=say)
         report[name] = {"wall_s": round(r["wall_s"], 2), "samples": r["samples"], "mlups": round(r["mlups"], 1),
                         "errors_vs_filtered_fine": _stat_errors(r["stats"], ref),
+                        "wall_to_wall_asymmetry": _asymmetry(r["stats"]),
                         "time_to_stats_speedup": round(wall_fine / r["wall_s"], 2)}
         report[name + "_profile"] = r["stats"]
     report["fine_filtered_profile"] = ref
@@ -332,7 +346,7 @@ def main(argv=None):
         if args.out:
             with open(args.out + ".json", "w") as fh:
                 json.dump(report, fh, indent=1)
-            np.savez(args.out + ".npz", params=trained, **arrays)
+            np.savez(args.out + ".npz", **{"params_" + m: v for m, v in trained.items()}, **arrays)
         return
     report, res = precision_study(lx, ly, lz, args.steps, args.stats_from, args.stats_every,
                                   mrttype=args.mrttype, jitter=args.jitter,

@@ -256,10 +256,19 @@ int lbm_closure_clear(lbm_channel *h);
  * features: the first-order non-equilibrium moments of the coarse MRT with
  * shear rate s9_coarse, scaled by the parameters' input scales).  Gradients
  * are of mean((softplus(o) - target)^2) over a sample range; training uses
- * Adam (beta 0.9, 0.999) over consecutive batches. */
+ * Adam (beta 0.9, 0.999) over consecutive batches.
+ *
+ * lbm_trainer_set_target chooses the target of later add_filtered calls:
+ * LBM_TARGET_LOCAL (default) the node's own -tau:S/(2 S:S)/nu, clipped to
+ * [-10, 10]; LBM_TARGET_PLANE the least-squares eddy viscosity of the node's
+ * wall-parallel plane, -sum(tau:S)/(2 sum(S:S))/nu (the planar averaging of
+ * dynamic subgrid models), the same for every node of the plane. */
+#define LBM_TARGET_LOCAL 0
+#define LBM_TARGET_PLANE 1
 typedef struct lbm_trainer lbm_trainer;
 int lbm_trainer_create(int device, long long capacity, const float *params, lbm_trainer **out);
 int lbm_trainer_destroy(lbm_trainer *t);
+int lbm_trainer_set_target(lbm_trainer *t, int mode);
 int lbm_trainer_add_filtered(lbm_trainer *t, lbm_channel *fine, double nu_coarse, double s9_coarse,
                              double ystar_coarse, long long *added);
 int lbm_trainer_add_samples(lbm_trainer *t, const float *features, const float *targets, long long n);

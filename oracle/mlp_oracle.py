@@ -9,7 +9,7 @@ Float64 numpy restatements of:
     reference's macrovar, Channel-Flow/collision.f90:398-417);
   * the coarse features and targets: strain by central differences with
     no-slip walls half a cell outside, subgrid stress, eddy viscosity
-    nu_t = -tau:S/(2 S:S) as nu_t/nu clipped to [0, 10], first-order
+    nu_t = -tau:S/(2 S:S) as nu_t/nu clipped to [-10, 10], first-order
     non-equilibrium moments -(2/(3 s9)) x (strain combination).
 """
 import numpy as np
@@ -74,8 +74,10 @@ def box_filter(rho, ux, uy, uz):
             "uxuz": avg(ux * uz)}
 
 
-def features_targets(F, nu, s9, inv_ystar, scales, yscale):
-    """Features (N, 10) and targets (N,) of the coarse nodes, node order ix fastest."""
+def features_targets(F, nu, s9, inv_ystar, scales, yscale, plane=False):
+    """Features (N, 10) and targets (N,) of the coarse nodes, node order ix fastest.
+    plane=True: every node's target is its x plane's least-squares eddy
+    viscosity -sum(tau:S) / (2 sum(S:S)) / nu (clipped to [-10, 10])."""
     u = [F["ux"], F["uy"], F["uz"]]  # (lz, ly, lx)
     lz, ly, lx = u[0].shape
 
@@ -100,8 +102,13 @@ def features_targets(F, nu, s9, inv_ystar, scales, yscale):
     tau[0][2] = tau[2][0] = F["uxuz"] - u[0] * u[2]
     ts = sum(tau[c][d] * S[c][d] for c in range(3) for d in range(3))
     ss = sum(S[c][d] * S[c][d] for c in range(3) for d in range(3))
-    y = np.where(ss > 1e-30, -ts / (2.0 * np.where(ss > 1e-30, ss, 1.0)) / nu, 0.0)
-    y = np.clip(y, 0.0, 10.0)
+    if plane:
+        a, b = ts.sum(axis=(0, 1)), ss.sum(axis=(0, 1))
+        yp = np.where(b > 1e-30, -a / (2.0 * np.where(b > 1e-30, b, 1.0)) / nu, 0.0)
+        y = np.broadcast_to(np.clip(yp, -10.0, 10.0), ts.shape)
+    else:
+        y = np.where(ss > 1e-30, -ts / (2.0 * np.where(ss > 1e-30, ss, 1.0)) / nu, 0.0)
+        y = np.clip(y, -10.0, 10.0)
     k = -2.0 / (3.0 * s9)
     neq = [np.zeros_like(u[0]), np.zeros_like(u[0]), k * (2 * S[0][0] - S[1][1] - S[2][2]), np.zeros_like(u[0]),
            k * (S[1][1] - S[2][2]), np.zeros_like(u[0]), k * S[0][1], k * S[1][2], k * S[0][2]]
