@@ -14,6 +14,10 @@ steps stay far below the ~4k steps after which MRTtype 1's linear
 instability at this centreline speed shows (the rates do not change the
 kernel's cost).
 
+--workload c4: the large channel Re_tau = 395 on 2048 x 1024 x 512 nodes
+(1.07 G); `value` is FP32 MLUPS there (FP64 lattices would need 326 GB) and
+FP16 storage is the variant.
+
 "closure_coarse": the 2x-coarse channel (384 x 192 x 96, nu = 0.0018) with
 the fused learned closure, FP32 storage; its ratio of wall time per unit of
 physical time against the fine grid (one coarse step = two fine steps, 1/8
@@ -48,7 +52,11 @@ WORKLOADS = {
     "c2": (192, 768, 384, "turbulent channel Re_tau=180, 768x384x192 (streamwise x spanwise x wall-normal)"),
     "c1": (32, 64, 32, "channel 64x32x32 (streamwise x spanwise x wall-normal)"),
     "c5": (96, 384, 192, "coarse channel 384x192x96 (streamwise x spanwise x wall-normal)"),
+    # 1.07 G nodes: FP32 lattices 2 x 81.6 GB, FP16 2 x 40.8 GB; FP64 (2 x 163 GB) does not fit in 288 GB
+    "c4": (512, 2048, 1024, "large channel Re_tau=395, 2048x1024x512 (streamwise x spanwise x wall-normal)"),
 }
+RSTAR = {"c4": 395.0}  # Re_tau per workload (para.f90:62 otherwise: 180)
+MAIN_PREC = {"c4": "fp32"}  # precision of `value` (FP64 unless it does not fit)
 
 
 def log(msg):
@@ -76,7 +84,7 @@ def init_channel(ch, lx, ly, lz):
 def measure(prec, workload, steps, warmup):
     lx, ly, lz, _ = WORKLOADS[workload]
     n = lx * ly * lz
-    with d3q19.Channel(lx, ly, lz, prec) as ch:
+    with d3q19.Channel(lx, ly, lz, prec, rstar=RSTAR.get(workload)) as ch:
         init_channel(ch, lx, ly, lz)
         m0 = ch.mass()
         ch.run(warmup)
@@ -240,17 +248,18 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 
-    log(f"bench: workload {args.workload} fp64 steps={args.steps} warmup={args.warmup}")
-    main_m = measure("fp64", args.workload, args.steps, args.warmup)
-    log(f"bench: fp64 {main_m['mlups']:.1f} MLUPS, kernel {main_m['kernel_ms']:.3f} ms, "
+    main_prec = MAIN_PREC.get(args.workload, "fp64")
+    log(f"bench: workload {args.workload} {main_prec} steps={args.steps} warmup={args.warmup}")
+    main_m = measure(main_prec, args.workload, args.steps, args.warmup)
+    log(f"bench: {main_prec} {main_m['mlups']:.1f} MLUPS, kernel {main_m['kernel_ms']:.3f} ms, "
         f"{main_m['roofline']['achieved']:.0f} GB/s")
     variants = {}
-    for prec in [v for v in args.variants.split(",") if v]:
+    for prec in [v for v in args.variants.split(",") if v and v != main_prec]:
         log(f"bench: variant {prec}")
         v = measure(prec, args.workload, args.steps, args.warmup)
         variants[prec] = {"value": round(v["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(v["ms_per_step"], 4),
                           "kernel_ms": round(v["kernel_ms"], 4), "roofline": v["roofline"],
-                          "speedup_vs_fp64": round(v["mlups"] / main_m["mlups"], 3),
+                          f"speedup_vs_{main_prec}": round(v["mlups"] / main_m["mlups"], 3),
                           "mass_drift_per_node": v["mass_drift_per_node"]}
         log(f"bench: {prec} {v['mlups']:.1f} MLUPS, {v['roofline']['achieved']:.0f} GB/s")
 
@@ -278,7 +287,7 @@ def main():
 
     lx, ly, lz, desc = WORKLOADS[args.workload]
     out = {
-        "metric": "MLUPS (FP64 D3Q19 MRT channel step) + % of HBM peak",
+        "metric": f"MLUPS ({main_prec.upper()} D3Q19 MRT channel step) + % of HBM peak",
         "value": round(main_m["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -288,7 +297,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16 storage, f32 compute"}[main_prec],
         "data": "synthetic (seeded log-law + divergence-free perturbation)",
         "config": {"workload": desc, "lx": lx, "ly": ly, "lz": lz, "nodes": lx * ly * lz,
                    "kernel_ms": round(main_m["kernel_ms"], 4),

@@ -46,10 +46,10 @@ COLS = {"U+": 3, "uxux": 5, "uyuy": 6, "uzuz": 7, "uxuy": 9}
 
 
 def run_channel(prec, lx, ly, lz, steps, stats_from, stats_every, seed=2, mrttype=3, amp=0.10,
-                jitter=0.0, log=None):
+                jitter=0.0, rstar=None, visc=None, log=None):
     """One channel run; returns time-averaged profiles, instantaneous U+
     profiles at every tenth of the run, mass drift and timing."""
-    with Channel(lx, ly, lz, prec, mrttype=mrttype) as ch:
+    with Channel(lx, ly, lz, prec, mrttype=mrttype, rstar=rstar, visc=visc) as ch:
         ux, uy, uz = workloads.loglaw_divfree(lx, ly, lz, ch.p["ustar"], ch.p["ystar"], seed=seed, amp=amp)
         if jitter:
             uc = float(np.max(uy))
@@ -97,9 +97,10 @@ def _inst_errors(run, ref):
 
 
 def precision_study(lx=192, ly=768, lz=384, steps=20000, stats_from=10000, stats_every=100,
-                    precs=("fp64", "fp32", "fp16"), mrttype=3, jitter=1e-7, deterministic=True, log=None):
+                    precs=("fp64", "fp32", "fp16"), mrttype=3, jitter=1e-7, deterministic=True, rstar=None,
+                    visc=None, log=None):
     cases = {}
-    kw = dict(mrttype=mrttype, log=log)
+    kw = dict(mrttype=mrttype, rstar=rstar, visc=visc, log=log)
     for prec in precs:
         cases[prec] = run_channel(prec, lx, ly, lz, steps, stats_from, stats_every, **kw)
     if jitter:
@@ -109,6 +110,7 @@ def precision_study(lx=192, ly=768, lz=384, steps=20000, stats_from=10000, stats
             cases["deterministic_" + prec] = run_channel(prec, lx, ly, lz, steps, stats_from, stats_every,
                                                          amp=0.0, **kw)
     report = {"grid": [lx, ly, lz], "steps": steps, "stats_from": stats_from, "stats_every": stats_every,
+              "rstar": 180.0 if rstar is None else rstar, "visc": 0.0036 if visc is None else visc,
               "mrttype": mrttype, "jitter": jitter, "variants": {}}
     for name, r in cases.items():
         ref = cases["deterministic_fp64" if name.startswith("deterministic_") else "fp64"]
@@ -332,6 +334,8 @@ def main(argv=None):
     ap.add_argument("--stats-from", type=int, default=10000)
     ap.add_argument("--stats-every", type=int, default=100)
     ap.add_argument("--mrttype", type=int, default=3, choices=[1, 2, 3])
+    ap.add_argument("--rstar", type=float, default=None, help="precision: Re_tau (default 180, para.f90:62)")
+    ap.add_argument("--visc", type=float, default=None, help="precision: nu (default 0.0036, para.f90:61)")
     ap.add_argument("--jitter", type=float, default=1e-7, help="control run's disturbance / U_c (0: no control)")
     ap.add_argument("--no-deterministic", action="store_true", help="skip the unperturbed runs")
     ap.add_argument("--out", default=None, help="also write the report (JSON) and profiles (.npz) here")
@@ -349,7 +353,7 @@ def main(argv=None):
             np.savez(args.out + ".npz", **{"params_" + m: v for m, v in trained.items()}, **arrays)
         return
     report, res = precision_study(lx, ly, lz, args.steps, args.stats_from, args.stats_every,
-                                  mrttype=args.mrttype, jitter=args.jitter,
+                                  mrttype=args.mrttype, jitter=args.jitter, rstar=args.rstar, visc=args.visc,
                                   deterministic=not args.no_deterministic, log=log)
     print(json.dumps(report, indent=1))
     if args.out:
