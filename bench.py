@@ -221,6 +221,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--grid", default=None, metavar="LX,LY,LZ",
+                    help="measure this grid instead of the workload's (shape studies; Re_tau = 180)")
     ap.add_argument("--variants", default="fp32,fp16", help="comma list of extra storage precisions ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-closure", action="store_true", help="skip the coarse closure-run throughput")
@@ -232,6 +234,10 @@ def main():
                          "--traffic-probe) into profiles/step_<prec>_<workload>_traffic.json")
     ap.add_argument("--prec", default="fp64", choices=sorted(BYTES_PER_NODE), help="precision for --traffic-from")
     args = ap.parse_args()
+    if args.grid:
+        lx, ly, lz = (int(v) for v in args.grid.split(","))
+        WORKLOADS["grid"] = (lx, ly, lz, f"channel {ly}x{lz}x{lx} (streamwise x spanwise x wall-normal)")
+        args.workload = "grid"
     if args.traffic_probe:
         traffic_probe(args.traffic_probe, args.workload, 5)
         return

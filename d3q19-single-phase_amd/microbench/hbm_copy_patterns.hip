@@ -148,6 +148,19 @@ __global__ void __launch_bounds__(256) aosoa_pair(const double* __restrict__ s, 
     for (int p = 0; p < 19; p++) __builtin_nontemporal_store(f[p] * 1.0000001, (dvec2*)(d + base + (uint64_t)p * T));
 }
 
+// 19-stream SoA copy with population stride S >= N (padding between the
+// population arrays), to see whether the arrays' relative alignment matters.
+__global__ void __launch_bounds__(256) soa_copy_stride(const double* __restrict__ s, double* __restrict__ d,
+                                                       uint64_t N, uint64_t S) {
+    const uint64_t n = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (n >= N) return;
+    double f[19];
+#pragma unroll
+    for (int i = 0; i < 19; i++) f[i] = s[i * S + n];
+#pragma unroll
+    for (int i = 0; i < 19; i++) __builtin_nontemporal_store(f[i] * 1.0000001, d + i * S + n);
+}
+
 typedef void (*KF)(const double*, double*, uint64_t);
 
 int main() {
@@ -206,6 +219,25 @@ int main() {
         CK(hipEventRecord(e1)); CK(hipEventSynchronize(e1));
         float ms; CK(hipEventElapsedTime(&ms, e0, e1));
         printf("%-28s %8.3f ms %8.1f GB/s %5.1f%%\n", f.name, ms / reps, 2.0 * bytes / (ms / reps) / 1e6, 2.0 * bytes / (ms / reps) / 1e6 / 80.0);
+        fflush(stdout);
+    }
+    // population stride padding: strides from N up to the next multiple of 2^24
+    CK(hipFree(A)); CK(hipFree(B));
+    const uint64_t smax = (N + (1ull << 24) - 1) & ~((1ull << 24) - 1);
+    CK(hipMalloc(&A, smax * 19 * 8)); CK(hipMalloc(&B, smax * 19 * 8));
+    CK(hipMemset(A, 0, smax * 19 * 8)); CK(hipMemset(B, 0, smax * 19 * 8));
+    const uint64_t strides[] = {N, N + 32, N + 256, N + 2048, N + 16384, N + 131072,
+                                (N + (1ull << 20) - 1) & ~((1ull << 20) - 1), smax};
+    const uint32_t grid = (uint32_t)((N + 255) / 256);
+    for (uint64_t S : strides) {
+        for (int w = 0; w < 2; w++) hipLaunchKernelGGL(soa_copy_stride, dim3(grid), dim3(256), 0, 0, A, B, N, S);
+        CK(hipDeviceSynchronize()); CK(hipEventRecord(e0));
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(soa_copy_stride, dim3(grid), dim3(256), 0, 0, (r & 1) ? B : A, (r & 1) ? A : B, N, S);
+        CK(hipEventRecord(e1)); CK(hipEventSynchronize(e1));
+        float ms; CK(hipEventElapsedTime(&ms, e0, e1));
+        printf("soa_copy19 stride N+%-9llu %8.3f ms %8.1f GB/s %5.1f%%\n", (unsigned long long)(S - N), ms / reps,
+               2.0 * bytes / (ms / reps) / 1e6, 2.0 * bytes / (ms / reps) / 1e6 / 80.0);
         fflush(stdout);
     }
     return 0;
