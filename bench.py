@@ -129,6 +129,41 @@ def measure_closure(steps, warmup):
     return {"mlups": n * steps / wall / 1e6, "kernel_ms": avg_ms, "nodes": n}
 
 
+def measure_particles(prec, workload, npart, rad, steps, warmup):
+    """Coupled particle step (beads_collision, collision_MRT with solid nodes
+    skipped, macrovar, beads_move, beads_links, beads_filling) with npart
+    seeded spheres of radius rad on the workload's grid.
+
+    The flow uses the laminar constants of para.f90:74-88 (tau ~ 3.4) with a
+    parabolic profile (U_c = 0.05) and the spheres start at the local fluid
+    velocity: at the turbulent branch's tau = 0.5108 a coupled run diverges
+    within ~150 steps (both rate sets 1 and 3, CPU oracle), which a timing
+    run must not depend on.  The kernels' work is set by the geometry (links,
+    refills per step), not by the regime."""
+    lx, ly, lz, _ = WORKLOADS[workload]
+    n = lx * ly * lz
+    pos = workloads.sphere_positions(lx, ly, lz, npart, rad, seed=3)
+    vel = np.zeros_like(pos)
+    vel[:, 1] = 0.05 * (1.0 - ((pos[:, 0] - lx / 2) / (lx / 2)) ** 2)
+    with d3q19.Channel(lx, ly, lz, prec, laminar=True) as ch:
+        ux, uy, uz = workloads.laminar_parabola_noise(lx, ly, lz, seed=1, uc=0.05, amp=0.0)
+        ch.set_macros(ux=ux, uy=uy, uz=uz)
+        del ux, uy, uz
+        ch.FORCING()
+        ch.initpop()
+        ch.macrovar()
+        ch.particles_create(pos, vel=vel, rad=rad)
+        solid = int(np.count_nonzero(ch.particles_solid() >= 0))
+        ch.run_particles(warmup)
+        ch.sync()
+        t0 = time.perf_counter()
+        ch.run_particles(steps)
+        ch.sync()
+        wall = time.perf_counter() - t0
+    return {"npart": npart, "radius": rad, "solid_fraction": round(solid / n, 5), "value": round(n * steps / wall / 1e6, 1),
+            "unit": "MLUPS", "ms_per_step": round(wall / steps * 1e3, 4)}
+
+
 COPY_KERNEL_TAG = "MulFunctor"  # torch.mul(src, 2.0, out=dst): reads and writes 8 B per element
 
 
@@ -226,6 +261,8 @@ def main():
     ap.add_argument("--variants", default="fp32,fp16", help="comma list of extra storage precisions ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-closure", action="store_true", help="skip the coarse closure-run throughput")
+    ap.add_argument("--particles", type=int, default=0, metavar="NPART",
+                    help="also time the coupled particle step with NPART spheres of radius 5 (FP64 and FP32)")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--traffic-probe", metavar="PREC", choices=sorted(BYTES_PER_NODE),
                     help="run the short program profiled by rocprofv3 --pmc (5 step launches + calibration copy)")
@@ -285,6 +322,14 @@ def main():
                 per_phys(variants["fp32"]["value"], fine_n, 2) / per_phys(c["mlups"], c["nodes"], 1), 2)
         log(f"bench: closure coarse {c['mlups']:.1f} MLUPS")
 
+    particles = None
+    if args.particles:
+        particles = {}
+        for prec in ("fp64", "fp32"):
+            log(f"bench: {args.particles} particles, {prec}")
+            particles[prec] = measure_particles(prec, args.workload, args.particles, 5.0, args.steps, args.warmup)
+            log(f"bench: particles {prec} {particles[prec]['value']} MLUPS")
+
     cpu = None
     if not args.no_cpu and rank == 0:
         threads = min(16, os.cpu_count() or 1)
@@ -312,6 +357,7 @@ def main():
         "cpu_baseline": cpu,
         "variants": variants,
         "closure_coarse": closure,
+        "particles": particles,
     }
     if rank == 0:
         print(json.dumps(out), flush=True)

@@ -1114,6 +1114,9 @@ static int check_overflow(lbm_channel *h, const char *what)
     int ovf = 0;
     HIP_TRY(hipMemcpyAsync(&ovf, h->part->ps.overflow, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    if (ovf & 2)
+        return fail(LBM_ERR_STATE, "%s: a particle's position or velocity became non-finite (the run diverged)",
+                    what);
     if (ovf)
         return fail(LBM_ERR_STATE, "%s: a per-particle link or refill list overflowed (maxlink %d, maxfill %d)",
                     what, h->part->pg.maxlink, h->part->pg.maxfill);

@@ -54,8 +54,56 @@ struct PartState {
     int *link_count;               // np
     int *fill_node;                // np*maxfill
     int *fill_count;               // np
-    int *overflow;                 // 1 when a per-particle list overflowed
+    int *overflow;                 // bit 0: a per-particle list overflowed; bit 1: a state became non-finite
 };
+
+// Bounding box of a sphere's nodes (node centres at i + 0.5), grown by `pad`
+// nodes.  x is clamped to the channel.  y and z start at an index in [0, L]
+// and run on from there (the users wrap each index into [0, L)); a box at
+// least as wide as the period covers it exactly once, from 0.  Every bound is
+// formed in double and clamped before it becomes an int, so a centre that is
+// not finite or lies more than R + pad + 1 outside the channel in x gives the
+// empty box (x1 < x0, zero nodes) instead of indices out of range.
+struct Box {
+    int x0, x1, y0, y1, z0, z1;
+};
+
+__host__ __device__ inline void box_axis(double c, double R, int pad, double L, int &lo, int &hi)
+{
+    const double a = floor(c - 0.5 - R) - pad, e = floor(c - 0.5 + R) + 1 + pad;
+    if (e - a + 1.0 >= L) {
+        lo = 0;
+        hi = (int)L - 1;
+        return;
+    }
+    double s = fmod(a, L); // exact, in (-L, L)
+    if (s < 0.0)
+        s = s + L;
+    lo = (int)s;
+    hi = lo + (int)(e - a);
+}
+
+__host__ __device__ inline Box sphere_box(const PartGeom &pg, const double *X, int pad)
+{
+    Box b = {0, -1, 0, -1, 0, -1};
+    const double lx = (double)pg.g.lx;
+    const double reach = pg.R + pad + 1.0;
+    // comparisons are false for NaN; y and z need finite values for fmod
+    if (!(X[0] > -reach && X[0] < lx + reach) || !(X[1] - X[1] == 0.0) || !(X[2] - X[2] == 0.0))
+        return b;
+    double x0 = floor(X[0] - 0.5 - pg.R) - pad, x1 = floor(X[0] - 0.5 + pg.R) + 1 + pad;
+    if (x0 < 0.0)
+        x0 = 0.0;
+    if (x1 > lx - 1.0)
+        x1 = lx - 1.0;
+    if (x1 < x0)
+        return b;
+    b.x0 = (int)x0;
+    b.x1 = (int)x1;
+    box_axis(X[1], pg.R, pad, (double)pg.g.ly, b.y0, b.y1);
+    box_axis(X[2], pg.R, pad, (double)pg.g.lz, b.z0, b.z1);
+    return b;
+}
 
 template <int P>
 struct PartLaunch {

@@ -11,39 +11,6 @@ namespace {
 
 constexpr int PBLOCK = 256;
 
-// Unwrapped bounding box of a sphere (node centres at i + 0.5), grown by
-// `pad` nodes; x is clamped to the channel, y/z are taken modulo the box
-// (a box at least as wide as the period covers it exactly once).
-struct Box {
-    int x0, x1, y0, y1, z0, z1;
-};
-
-__host__ __device__ inline int floor_i(double v) { return (int)floor(v); }
-
-__host__ __device__ inline Box sphere_box(const PartGeom &pg, const double *X, int pad)
-{
-    Box b;
-    b.x0 = floor_i(X[0] - 0.5 - pg.R) - pad;
-    b.x1 = floor_i(X[0] - 0.5 + pg.R) + 1 + pad;
-    b.y0 = floor_i(X[1] - 0.5 - pg.R) - pad;
-    b.y1 = floor_i(X[1] - 0.5 + pg.R) + 1 + pad;
-    b.z0 = floor_i(X[2] - 0.5 - pg.R) - pad;
-    b.z1 = floor_i(X[2] - 0.5 + pg.R) + 1 + pad;
-    if (b.x0 < 0)
-        b.x0 = 0;
-    if (b.x1 > (int)pg.g.lx - 1)
-        b.x1 = (int)pg.g.lx - 1;
-    if (b.y1 - b.y0 + 1 >= (int)pg.g.ly) {
-        b.y0 = 0;
-        b.y1 = (int)pg.g.ly - 1;
-    }
-    if (b.z1 - b.z0 + 1 >= (int)pg.g.lz) {
-        b.z0 = 0;
-        b.z1 = (int)pg.g.lz - 1;
-    }
-    return b;
-}
-
 __host__ __device__ inline int wrap(int i, int L)
 {
     int r = i % L;
@@ -127,7 +94,7 @@ __global__ void __launch_bounds__(PBLOCK) k_p_uncover(PartGeom pg, PartState ps,
             if (base + off < pg.maxfill)
                 ps.fill_node[(size_t)p * pg.maxfill + base + off] = (int)n;
             else
-                *ps.overflow = 1;
+                atomicOr(ps.overflow, 1);
         }
         base += tot;
     }
@@ -211,7 +178,7 @@ __global__ void __launch_bounds__(PBLOCK) k_p_links(PartGeom pg, PartState ps)
                     ps.link_dir[s] = (signed char)i;
                     ps.link_q[s] = q;
                 } else {
-                    *ps.overflow = 1;
+                    atomicOr(ps.overflow, 1);
                 }
                 k++;
             }
@@ -396,6 +363,13 @@ __global__ void k_p_move(PartGeom pg, PartState ps, double *Xold)
         Xn[2] = Xn[2] - Lz;
     else if (Xn[2] < 0.0)
         Xn[2] = Xn[2] + Lz;
+    // a diverged run: flag it (sphere_box then gives empty boxes for such a
+    // particle, so no kernel indexes from it)
+    bool finite = true;
+    for (int j = 0; j < 3; j++)
+        finite = finite && (Xn[j] - Xn[j] == 0.0) && (Vn[j] - Vn[j] == 0.0) && (Wn[j] - Wn[j] == 0.0);
+    if (!finite)
+        atomicOr(ps.overflow, 2);
     // every thread has read all positions (contacts) before any is updated:
     // positions go to Xold first and are swapped in by k_p_commit
     for (int j = 0; j < 3; j++) {

@@ -99,3 +99,29 @@ def test_particle_run_fp32_is_close_to_fp64():
     for key in ("pos", "vel"):
         assert np.allclose(out["fp32"][key], out["fp64"][key], rtol=1e-4, atol=1e-5), key
     assert np.allclose(out["fp32"]["force"], out["fp64"]["force"], rtol=1e-2, atol=1e-5)
+
+
+def test_diverged_particle_is_reported_not_indexed():
+    """A particle whose state becomes non-finite (here: an infinite initial
+    velocity, so its centre is infinite after the first move) yields empty
+    node boxes and an error from run_particles; the device stays usable."""
+    lx, ly, lz, rad = 32, 64, 32, 5.0
+    pos = workloads.sphere_positions(lx, ly, lz, 3, rad, seed=3)
+    vel = np.zeros((3, 3))
+    vel[1, 0] = np.inf
+    with d3q19.Channel(lx, ly, lz) as ch:
+        ch.initvel()
+        ch.FORCING()
+        ch.initpop()
+        ch.macrovar()
+        ch.particles_create(pos, vel=vel, rad=rad)
+        with pytest.raises(d3q19.LbmError, match="non-finite"):
+            ch.run_particles(3)
+    # the card still runs an ordinary channel
+    with d3q19.Channel(12, 10, 6) as ch:
+        ch.initvel()
+        ch.FORCING()
+        ch.initpop()
+        ch.macrovar()
+        ch.run(5)
+        assert np.isfinite(ch.download_f()).all()

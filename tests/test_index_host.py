@@ -23,3 +23,20 @@ def test_index_arithmetic_on_host():
         r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stderr + r.stdout
         assert "ok remap" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
+                    reason="hipcc not available")
+def test_particle_boxes_stay_inside_the_lattice():
+    """tests/host/box_check.hip: sphere_box for centres inside, outside, huge,
+    infinite and NaN enumerates only lattice nodes and covers every node
+    within R of a centre inside the channel."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "box_check")
+        subprocess.run([hipcc, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "d3q19-single-phase_amd", "csrc"),
+                        "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "host", "box_check.hip"),
+                        "-o", exe], check=True, capture_output=True)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr + r.stdout
+        assert "ok boxes" in r.stdout
