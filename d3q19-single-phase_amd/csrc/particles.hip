@@ -10,6 +10,7 @@ namespace d3q19 {
 namespace {
 
 constexpr int PBLOCK = 256;
+constexpr int IBB_BLOCKS = 4; // blocks per particle for the cut links (~1000-2000 each at R = 5)
 
 __host__ __device__ inline int wrap(int i, int L)
 {
@@ -141,7 +142,12 @@ __global__ void __launch_bounds__(PBLOCK) k_p_links(PartGeom pg, PartState ps)
             iy = wrap(b.y0 + ky, pg.g.ly);
             iz = wrap(b.z0 + kz, pg.g.lz);
             n = node_of(pg.g, ix, iy, iz);
-            if (ps.solid[n] < 0) {
+            // a link is at most sqrt(2) long: nodes farther than R + 1.5 from
+            // the centre cannot reach the sphere
+            double ox, oy, oz;
+            node_offset(pg, ix, iy, iz, X, ox, oy, oz);
+            const double reach = pg.R + 1.5;
+            if (ox * ox + oy * oy + oz * oz < reach * reach && ps.solid[n] < 0) {
                 for (int i = 1; i < 19; i++) {
                     const int jx = ix + CX[i];
                     if (jx < 0 || jx >= (int)pg.g.lx)
@@ -210,18 +216,13 @@ __device__ inline void st_rt(typename Prec<P>::S *L, uint64_t N, int i, uint32_t
         L[(uint64_t)i * N + n] = (S)v;
 }
 
-// Interpolated bounce-back on every cut link, written into the solid node's
-// slot that the fluid node pulls next step; per-link momentum exchange.
+// Interpolated bounce-back on one cut link of particle p (k < link_count[p]),
+// written into the solid node's slot that the fluid node pulls next step;
+// the link's momentum-exchange contribution.
 template <int P>
-__global__ void __launch_bounds__(PBLOCK) k_p_ibb(PartGeom pg, PartState ps, void *Lv)
+__device__ inline void ibb_link(const PartGeom &pg, const PartState &ps, typename Prec<P>::S *L, uint64_t N,
+                                     int p, int k)
 {
-    using S = typename Prec<P>::S;
-    const int p = blockIdx.y;
-    const int k = blockIdx.x * PBLOCK + threadIdx.x;
-    if (k >= ps.link_count[p])
-        return;
-    S *L = (S *)Lv;
-    const uint64_t N = pg.g.N;
     const size_t s = (size_t)p * pg.maxlink + k;
     const uint32_t n = (uint32_t)ps.link_node[s];
     const int i = ps.link_dir[s];
@@ -282,35 +283,78 @@ __global__ void __launch_bounds__(PBLOCK) k_p_ibb(PartGeom pg, PartState ps, voi
     c[5] = rx * py - ry * px;
 }
 
-// Per-particle force and torque: links summed in canonical order.
-__global__ void k_p_sum(PartGeom pg, PartState ps)
+// Every cut link: IBB_BLOCKS blocks per particle stride over its links.
+template <int P>
+__global__ void __launch_bounds__(PBLOCK) k_p_ibb(PartGeom pg, PartState ps, void *Lv)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pg.np)
-        return;
-    double f[6] = {0, 0, 0, 0, 0, 0};
+    using S = typename Prec<P>::S;
+    const int p = blockIdx.y;
     const int nl = ps.link_count[p];
-    for (int k = 0; k < nl; k++) {
-        const double *c = ps.link_contrib + ((size_t)p * pg.maxlink + k) * 6;
-        for (int j = 0; j < 6; j++)
-            f[j] = f[j] + c[j];
-    }
-    for (int j = 0; j < 3; j++) {
-        ps.FH[3 * p + j] = f[j];
-        ps.TQ[3 * p + j] = f[3 + j];
+    S *L = (S *)Lv;
+    const uint64_t N = pg.g.N;
+    for (int k = blockIdx.x * PBLOCK + threadIdx.x; k < nl; k += gridDim.x * PBLOCK)
+        ibb_link<P>(pg, ps, L, N, p, k);
+}
+
+// Sums in a fixed order over one 64-lane block: lane l adds terms l, l+64,
+// l+128, ... in increasing order, then the 64 partials are added pairwise
+// (l += l + 32, l += l + 16, ..., l += l + 1).  The CPU oracle sums in the
+// same order, so results agree bit for bit.
+constexpr int SUMW = 64;
+
+template <int C>
+__device__ inline void lane_tree(double (&v)[C], double (*s)[SUMW])
+{
+    const int l = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < C; j++)
+        s[j][l] = v[j];
+    __syncthreads();
+    for (int off = SUMW / 2; off > 0; off >>= 1) {
+        if (l < off) {
+#pragma unroll
+            for (int j = 0; j < C; j++)
+                s[j][l] = s[j][l] + s[j][l + off];
+        }
+        __syncthreads();
     }
 }
 
-// Contact forces and FP64 integration, one thread per particle.
-__global__ void k_p_move(PartGeom pg, PartState ps, double *Xold)
+// Per-particle force and torque from the per-link contributions, one block
+// of SUMW lanes per particle.
+__global__ void __launch_bounds__(SUMW) k_p_sum(PartGeom pg, PartState ps)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pg.np)
-        return;
+    const int p = blockIdx.x;
+    const int l = threadIdx.x;
+    double f[6] = {0, 0, 0, 0, 0, 0};
+    const int nl = ps.link_count[p];
+    for (int k = l; k < nl; k += SUMW) {
+        const double *c = ps.link_contrib + ((size_t)p * pg.maxlink + k) * 6;
+#pragma unroll
+        for (int j = 0; j < 6; j++)
+            f[j] = f[j] + c[j];
+    }
+    __shared__ double s[6][SUMW];
+    lane_tree<6>(f, s);
+    if (l == 0) {
+        for (int j = 0; j < 3; j++) {
+            ps.FH[3 * p + j] = s[j][0];
+            ps.TQ[3 * p + j] = s[3 + j][0];
+        }
+    }
+}
+
+// Contact forces and FP64 integration, one block of SUMW lanes per particle:
+// the lanes split the other particles (q = l, l+64, ...), their contact sums
+// are added by lane_tree, lane 0 adds the walls and integrates.
+__global__ void __launch_bounds__(SUMW) k_p_move(PartGeom pg, PartState ps, double *Xold)
+{
+    const int p = blockIdx.x;
+    const int l = threadIdx.x;
     const double Ly = (double)pg.g.ly, Lz = (double)pg.g.lz, Lx = (double)pg.g.lx;
     const double *Xp = ps.X + 3 * p;
     double fc[3] = {0.0, 0.0, 0.0};
-    for (int q = 0; q < pg.np; q++) {
+    for (int q = l; q < pg.np; q += SUMW) {
         if (q == p)
             continue;
         const double *Xq = ps.X + 3 * q;
@@ -327,6 +371,13 @@ __global__ void k_p_move(PartGeom pg, PartState ps, double *Xold)
             fc[2] = fc[2] + mag * (ddz / dist);
         }
     }
+    __shared__ double s[3][SUMW];
+    lane_tree<3>(fc, s);
+    if (l != 0)
+        return;
+    fc[0] = s[0][0];
+    fc[1] = s[1][0];
+    fc[2] = s[2][0];
     const double hlo = Xp[0] - pg.R;
     if (hlo < pg.mingap_w) {
         const double r = (pg.mingap_w - hlo) / pg.mingap_w;
@@ -397,15 +448,9 @@ __global__ void k_p_commit(PartGeom pg, PartState ps, double *Xold)
 // Refill: nodes uncovered by particle p get the equilibrium of the wall
 // velocity there and the mean density of their settled fluid neighbours.
 template <int P>
-__global__ void __launch_bounds__(64) k_p_refill(PartGeom pg, PartState ps, void *Lv)
+__device__ inline void refill_node(const PartGeom &pg, const PartState &ps, typename Prec<P>::S *L, uint64_t N,
+                                   int p, int k)
 {
-    using S = typename Prec<P>::S;
-    const int p = blockIdx.y;
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    if (k >= ps.fill_count[p])
-        return;
-    S *L = (S *)Lv;
-    const uint64_t N = pg.g.N;
     const uint32_t n = (uint32_t)ps.fill_node[(size_t)p * pg.maxfill + k];
     if (ps.solid[n] != -2 - p)
         return; // claimed by another particle since
@@ -445,17 +490,26 @@ __global__ void __launch_bounds__(64) k_p_refill(PartGeom pg, PartState ps, void
     }
 }
 
+template <int P>
+__global__ void __launch_bounds__(64) k_p_refill(PartGeom pg, PartState ps, void *Lv)
+{
+    const int p = blockIdx.y;
+    const int nf = ps.fill_count[p];
+    for (int k = blockIdx.x * 64 + threadIdx.x; k < nf; k += gridDim.x * 64)
+        refill_node<P>(pg, ps, (typename Prec<P>::S *)Lv, pg.g.N, p, k);
+}
+
 // Uncovered nodes become ordinary fluid once every refill has read its
 // neighbours.
 __global__ void k_p_settle(PartGeom pg, PartState ps)
 {
     const int p = blockIdx.y;
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    if (k >= ps.fill_count[p])
-        return;
-    const uint32_t n = (uint32_t)ps.fill_node[(size_t)p * pg.maxfill + k];
-    if (ps.solid[n] == -2 - p)
-        ps.solid[n] = -1;
+    const int nf = ps.fill_count[p];
+    for (int k = blockIdx.x * 64 + threadIdx.x; k < nf; k += gridDim.x * 64) {
+        const uint32_t n = (uint32_t)ps.fill_node[(size_t)p * pg.maxfill + k];
+        if (ps.solid[n] == -2 - p)
+            ps.solid[n] = -1;
+    }
 }
 
 } // namespace
@@ -472,14 +526,14 @@ hipError_t part_classify(const PartGeom &pg, const PartState &ps, int first, hip
 
 hipError_t part_sum(const PartGeom &pg, const PartState &ps, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_p_sum, dim3((pg.np + 63) / 64), dim3(64), 0, s, pg, ps);
+    hipLaunchKernelGGL(k_p_sum, dim3(pg.np), dim3(SUMW), 0, s, pg, ps);
     return hipGetLastError();
 }
 
 hipError_t part_move(const PartGeom &pg, const PartState &ps, hipStream_t s)
 {
     double *Xold = ps.X + 3 * (size_t)pg.np;
-    hipLaunchKernelGGL(k_p_move, dim3((pg.np + 63) / 64), dim3(64), 0, s, pg, ps, Xold);
+    hipLaunchKernelGGL(k_p_move, dim3(pg.np), dim3(SUMW), 0, s, pg, ps, Xold);
     hipLaunchKernelGGL(k_p_commit, dim3((pg.np + 63) / 64), dim3(64), 0, s, pg, ps, Xold);
     return hipGetLastError();
 }
@@ -487,14 +541,14 @@ hipError_t part_move(const PartGeom &pg, const PartState &ps, hipStream_t s)
 template <int P>
 hipError_t PartLaunch<P>::ibb(const PartGeom &pg, const PartState &ps, void *L, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_p_ibb<P>), dim3((pg.maxlink + PBLOCK - 1) / PBLOCK, pg.np), dim3(PBLOCK), 0, s, pg, ps, L);
+    hipLaunchKernelGGL((k_p_ibb<P>), dim3(IBB_BLOCKS, pg.np), dim3(PBLOCK), 0, s, pg, ps, L);
     return hipGetLastError();
 }
 
 template <int P>
 hipError_t PartLaunch<P>::refill(const PartGeom &pg, const PartState &ps, void *L, hipStream_t s)
 {
-    const dim3 grid((pg.maxfill + 63) / 64, pg.np);
+    const dim3 grid(1, pg.np); // one 64-lane block per particle strides over its refill list
     hipLaunchKernelGGL((k_p_refill<P>), grid, dim3(64), 0, s, pg, ps, L);
     hipLaunchKernelGGL(k_p_settle, grid, dim3(64), 0, s, pg, ps);
     return hipGetLastError();

@@ -52,6 +52,20 @@ static double min_image(double d, double L)
     return d;
 }
 
+/* Fixed summation order of the per-particle sums (force/torque over links,
+ * contact force over the other particles): term k goes to partial k % 64,
+ * each partial adds its terms in increasing k, then the 64 partials are
+ * added pairwise (l += l + 32, l += l + 16, ..., l += l + 1). */
+#define SUMW 64
+
+static void lane_tree(double (*part)[6], int ncomp)
+{
+    for (int off = SUMW / 2; off > 0; off >>= 1)
+        for (int l = 0; l < off; l++)
+            for (int j = 0; j < ncomp; j++)
+                part[l][j] = part[l][j] + part[l + off][j];
+}
+
 static size_t nodeof(const orc_pgeom *g, int ix, int iy, int iz)
 {
     return (size_t)ix + (size_t)g->lx * ((size_t)iy + (size_t)g->ly * (size_t)iz);
@@ -194,7 +208,8 @@ void orc_p_ibb(const orc_pgeom *g, orc_pstate *s, double *L)
     const size_t N = (size_t)g->lx * g->ly * g->lz;
     for (int p = 0; p < g->np; p++) {
         const double *X = s->X + 3 * p, *V = s->V + 3 * p, *W = s->W + 3 * p;
-        double f[6] = {0, 0, 0, 0, 0, 0};
+        double part[SUMW][6];
+        memset(part, 0, sizeof(part));
         for (int k = 0; k < s->link_count[p]; k++) {
             size_t sl = (size_t)p * g->maxlink + k;
             size_t n = (size_t)s->link_node[sl];
@@ -238,11 +253,12 @@ void orc_p_ibb(const orc_pgeom *g, orc_pstate *s, double *L)
             const double pz = ff * (cz - uwz) + gf * (cz + uwz);
             const double c[6] = {px, py, pz, ry * pz - rz * py, rz * px - rx * pz, rx * py - ry * px};
             for (int j = 0; j < 6; j++)
-                f[j] = f[j] + c[j];
+                part[k % SUMW][j] = part[k % SUMW][j] + c[j];
         }
+        lane_tree(part, 6);
         for (int j = 0; j < 3; j++) {
-            s->FH[3 * p + j] = f[j];
-            s->TQ[3 * p + j] = f[3 + j];
+            s->FH[3 * p + j] = part[0][j];
+            s->TQ[3 * p + j] = part[0][3 + j];
         }
     }
 }
@@ -290,7 +306,8 @@ void orc_p_move(const orc_pgeom *g, orc_pstate *s)
     const int first = (s->nsteps_done == 0);
     for (int p = 0; p < g->np; p++) {
         const double *Xp = s->X + 3 * p;
-        double fc[3] = {0.0, 0.0, 0.0};
+        double part[SUMW][6];
+        memset(part, 0, sizeof(part));
         for (int q = 0; q < g->np; q++) {
             if (q == p)
                 continue;
@@ -303,11 +320,14 @@ void orc_p_move(const orc_pgeom *g, orc_pstate *s)
             if (h < g->mingap && dist > 0.0) {
                 const double r = (g->mingap - h) / g->mingap;
                 const double mag = g->stf * (r * r);
-                fc[0] = fc[0] + mag * (ddx / dist);
-                fc[1] = fc[1] + mag * (ddy / dist);
-                fc[2] = fc[2] + mag * (ddz / dist);
+                double *pc = part[q % SUMW];
+                pc[0] = pc[0] + mag * (ddx / dist);
+                pc[1] = pc[1] + mag * (ddy / dist);
+                pc[2] = pc[2] + mag * (ddz / dist);
             }
         }
+        lane_tree(part, 3);
+        double fc[3] = {part[0][0], part[0][1], part[0][2]};
         const double hlo = Xp[0] - g->R;
         if (hlo < g->mingap_w) {
             const double r = (g->mingap_w - hlo) / g->mingap_w;
