@@ -18,6 +18,10 @@ kernel's cost).
 (1.07 G); `value` is FP32 MLUPS there (FP64 lattices would need 326 GB) and
 FP16 storage is the variant.
 
+"forcingp": the same step under the perturbation forcing FORCINGP
+(Channel-Flow/collision.f90:529-602), its force evaluated per node inside
+the step, timed by wall clock around the host loop.
+
 "closure_coarse": the 2x-coarse channel (384 x 192 x 96, nu = 0.0018) with
 the fused learned closure, FP32 storage; its ratio of wall time per unit of
 physical time against the fine grid (one coarse step = two fine steps, 1/8
@@ -109,6 +113,25 @@ def measure(prec, workload, steps, warmup):
         # change of the mean density fluctuation (sum of shifted populations / N)
         "mass_drift_per_node": (m1 - m0) / n,
     }
+
+
+def measure_forcingp(prec, workload, steps, warmup):
+    """The same step under the perturbation forcing FORCINGP (collision.f90:
+    529-602), the force evaluated per node inside the step: one FORCINGP,
+    collision_MRT and macrovar per step from the host, wall-clock timed."""
+    lx, ly, lz, _ = WORKLOADS[workload]
+    n = lx * ly * lz
+    with d3q19.Channel(lx, ly, lz, prec, rstar=RSTAR.get(workload)) as ch:
+        init_channel(ch, lx, ly, lz)
+        ch.run_forcingp(0, warmup, 10**9)
+        ch.sync()
+        t0 = time.perf_counter()
+        ch.run_forcingp(warmup, steps, 10**9)
+        ch.sync()
+        wall = time.perf_counter() - t0
+    achieved = n * BYTES_PER_NODE[prec] / (wall / steps) / 1e9
+    return {"mlups": n * steps / wall / 1e6, "ms_per_step": wall / steps * 1e3,
+            "wall_frac": achieved / HBM_PEAK_GBS}
 
 
 def measure_closure(steps, warmup):
@@ -261,6 +284,7 @@ def main():
     ap.add_argument("--variants", default="fp32,fp16", help="comma list of extra storage precisions ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-closure", action="store_true", help="skip the coarse closure-run throughput")
+    ap.add_argument("--no-forcingp", action="store_true", help="skip the perturbation-forcing step throughput")
     ap.add_argument("--particles", type=int, default=0, metavar="NPART",
                     help="also time the coupled particle step with NPART spheres of radius 5 (FP64 and FP32)")
     ap.add_argument("--cpu-steps", type=int, default=20)
@@ -305,6 +329,15 @@ def main():
                           f"speedup_vs_{main_prec}": round(v["mlups"] / main_m["mlups"], 3),
                           "mass_drift_per_node": v["mass_drift_per_node"]}
         log(f"bench: {prec} {v['mlups']:.1f} MLUPS, {v['roofline']['achieved']:.0f} GB/s")
+
+    forcingp = None
+    if not args.no_forcingp:
+        log(f"bench: perturbation forcing (FORCINGP), {main_prec}")
+        fp = measure_forcingp(main_prec, args.workload, args.steps, args.warmup)
+        forcingp = {"value": round(fp["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(fp["ms_per_step"], 4),
+                    "hbm_frac_wall": round(fp["wall_frac"], 4),
+                    "vs_uniform_force": round(fp["mlups"] / main_m["mlups"], 3)}
+        log(f"bench: forcingp {fp['mlups']:.1f} MLUPS")
 
     closure = None
     if args.workload == "c2" and not args.no_closure:
@@ -357,6 +390,7 @@ def main():
         "cpu_baseline": cpu,
         "variants": variants,
         "closure_coarse": closure,
+        "forcingp": forcingp,
         "particles": particles,
     }
     if rank == 0:

@@ -16,6 +16,23 @@ struct RatesD {
     double fx, fy, fz;
 };
 
+// Body force at the nodes: the force_realx/y/z arrays of
+// Channel-Flow/var_inc.f90 as FORCING (collision.f90:515-527) and FORCINGP
+// (collision.f90:529-602) set them.  tab == nullptr: (fx, fy, fz) at every
+// node.  Otherwise the FORCINGP field: (fx, fy, fz) = (0, force_in_y, 0)
+// outside its two bands, and tab holds the factors that depend on one
+// coordinate only, computed once per grid:
+//   tab[3*ix + 0..2]             band (0 none, 1, 2), 1 - cos(x9), sin(x9)
+//   tab[3*lx + 4*iy + 0..3]      cos(3 y9), sin(3 y9) of band 1, then of band 2
+//   tab[3*lx + 4*ly + 2*iz + 0..1] cos(2 z9), sin(2 z9)
+// a, b, c are band 1's products of the amplitude with the constants in front
+// of those factors (band 2 has the opposite signs).
+struct ForceSpec {
+    double fx, fy, fz;
+    const double *tab;
+    double a, b, c;
+};
+
 struct StepArgs {
     Geom g;
     const void *src; // P_k
@@ -28,13 +45,16 @@ struct StepArgs {
     float inv_ystar;   // u*/nu: y+ per lattice unit of wall distance
     float yscale;      // y+ feature divisor
     int runs;          // 1: modes 0 and 1 of FP64/FP32 take the vector-run kernel when lx allows
+    ForceSpec fc;      // mode bit 16: collision force (r.fx, r.fy, r.fz are then unused)
+    ForceSpec fm;      // mode bit 16: force that the fused fields u = j + F/2 are formed with
 };
 
 struct MacroArgs {
     Geom g;
     const void *L;
     double *rho, *ux, *uy, *uz;
-    double fx, fy, fz, rho_off;
+    ForceSpec f;
+    double rho_off;
     unsigned long long *errbits;
 };
 
@@ -65,7 +85,8 @@ struct PlaneArgs {
     const void *L;
     const double *rho, *ux, *uy, *uz;
     const int *solid; // nullable: when set, only fluid nodes count
-    double fx, fy, fz, rho_off;
+    ForceSpec f;
+    double rho_off;
     double *partials;
 };
 

@@ -91,6 +91,40 @@ __device__ __forceinline__ void scatter_all(typename Prec<P>::S *__restrict__ L,
     scatter_all_impl<P>(L, N, b, f, std::make_integer_sequence<int, 19>{});
 }
 
+__device__ __forceinline__ void node_coords(uint32_t n, const Geom &g, uint32_t &ix, uint32_t &iy, uint32_t &iz)
+{
+    const uint32_t q = fdiv(n, g.dlx);
+    ix = n - q * g.lx;
+    iz = fdiv(q, g.dly);
+    iy = q - iz * g.ly;
+}
+
+// Body force of s at node (ix, iy, iz), the value the reference's force
+// arrays hold there.  The FORCINGP products keep the operand order of
+// Channel-Flow/collision.f90:571-576 and 591-596; band 2's leading minus is
+// carried by the first factor, which changes no bit of the product.
+__device__ __forceinline__ void node_force(const ForceSpec &s, const Geom &g, uint32_t ix, uint32_t iy, uint32_t iz,
+                                           double &fx, double &fy, double &fz)
+{
+#pragma clang fp contract(off)
+    fx = s.fx;
+    fy = s.fy;
+    fz = s.fz;
+    if (s.tab == nullptr)
+        return;
+    const double *xt = s.tab + 3 * ix;
+    const double band = xt[0];
+    if (band == 0.0)
+        return;
+    const double sg = band == 1.0 ? 1.0 : -1.0;
+    const double *yt = s.tab + 3 * g.lx + 4 * iy + (band == 1.0 ? 0 : 2);
+    const double *zt = s.tab + 3 * g.lx + 4 * g.ly + 2 * iz;
+    const double cy = yt[0], sy = yt[1], cz = zt[0], sz = zt[1];
+    fx = (((sg * s.a) * xt[1]) * cy) * cz;
+    fy = s.fy * (1.0 + (((-sg * s.b) * xt[2]) * sy) * cz);
+    fz = (((sg * s.c) * xt[2]) * cy) * sz;
+}
+
 __device__ __forceinline__ uint32_t node_index(const Geom &g)
 {
     uint32_t blk = blockIdx.x;
@@ -108,6 +142,10 @@ __device__ __forceinline__ uint32_t node_index(const Geom &g)
 // particle nodes' slots before the step.
 // MODE + 8: the learned closure (mlp.h) sets each node's shear rates
 // (FP32 compute only).
+// MODE + 16: the body force varies over the nodes (FORCINGP), or the fused
+// fields were formed with another force than the collision's (the force was
+// changed after the last macrovar): a.fc is the collision's force, a.fm the
+// fields'.
 template <int P, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
 {
@@ -141,8 +179,25 @@ __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
     r.fx = (T)a.r.fx;
     r.fy = (T)a.r.fy;
     r.fz = (T)a.r.fz;
+    T fmx = r.fx, fmy = r.fy, fmz = r.fz; // force in u = j + F/2 of the fused fields
+    if constexpr ((MODE & 16) != 0) {
+        uint32_t ix, iy, iz;
+        node_coords(n, a.g, ix, iy, iz);
+        double x, y, z;
+        node_force(a.fc, a.g, ix, iy, iz, x, y, z);
+        r.fx = (T)x;
+        r.fy = (T)y;
+        r.fz = (T)z;
+        if constexpr ((MODE & 3) != 2) {
+            node_force(a.fm, a.g, ix, iy, iz, x, y, z);
+            fmx = (T)x;
+            fmy = (T)y;
+            fmz = (T)z;
+        }
+    }
 
     T rho9, ux9, uy9, uz9;
+    T jx = T(0), jy = T(0), jz = T(0);
     if constexpr ((MODE & 3) == 2) {
         rho9 = (T)a.rho[n];
         ux9 = (T)a.ux[n];
@@ -150,9 +205,12 @@ __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
         uz9 = (T)a.uz[n];
     } else {
         macro_sums(f, rho9, ux9, uy9, uz9);
-        ux9 = ux9 + r.fx / T(2.); // collision.f90:415-417
-        uy9 = uy9 + r.fy / T(2.);
-        uz9 = uz9 + r.fz / T(2.);
+        jx = ux9;
+        jy = uy9;
+        jz = uz9;
+        ux9 = ux9 + fmx / T(2.); // collision.f90:415-417
+        uy9 = uy9 + fmy / T(2.);
+        uz9 = uz9 + fmz / T(2.);
         if constexpr ((MODE & 3) == 1)
             rho9 = rho9 - (T)a.rho_off; // collision.f90:508
     }
@@ -163,9 +221,15 @@ __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
         // increment form; conserved-moment changes are zero for fields fused
         // from f, -rho_off after avedensity, field - moment for arrays
         T drho = T(0), djx = T(0), djy = T(0), djz = T(0);
-        if constexpr ((MODE & 3) == 1) {
+        if constexpr ((MODE & 3) == 1)
             drho = -(T)a.rho_off;
-        } else if constexpr ((MODE & 3) == 2) {
+        if constexpr ((MODE & 16) != 0 && (MODE & 3) != 2) {
+            // fields formed with fm, collision with fc
+            djx = ux9 - (jx + r.fx / T(2.));
+            djy = uy9 - (jy + r.fy / T(2.));
+            djz = uz9 - (jz + r.fz / T(2.));
+        }
+        if constexpr ((MODE & 3) == 2) {
             T rf, jx, jy, jz;
             macro_sums(f, rf, jx, jy, jz);
             drho = rho9 - rf;
@@ -360,9 +424,13 @@ __global__ void __launch_bounds__(BLOCK) k_macro(MacroArgs a)
         T rho9, ux9, uy9, uz9;
         macro_sums(f, rho9, ux9, uy9, uz9);
         if constexpr (WHAT == 0) {
-            a.ux[n] = (double)(ux9 + (T)a.fx / T(2.));
-            a.uy[n] = (double)(uy9 + (T)a.fy / T(2.));
-            a.uz[n] = (double)(uz9 + (T)a.fz / T(2.));
+            uint32_t ix, iy, iz;
+            node_coords(n, a.g, ix, iy, iz);
+            double fx, fy, fz;
+            node_force(a.f, a.g, ix, iy, iz, fx, fy, fz);
+            a.ux[n] = (double)(ux9 + (T)fx / T(2.));
+            a.uy[n] = (double)(uy9 + (T)fy / T(2.));
+            a.uz[n] = (double)(uz9 + (T)fz / T(2.));
             a.rho[n] = (double)(rho9 - (T)a.rho_off);
         } else {
             const double rnew = (double)rho9;
@@ -574,9 +642,13 @@ __global__ void __launch_bounds__(BLOCK) k_plane_partials(PlaneArgs a)
             pull_all<P>((const S *)a.L, a.g.N, b, f);
             T rho9, ux9, uy9, uz9;
             macro_sums(f, rho9, ux9, uy9, uz9);
-            ux = (double)(ux9 + (T)a.fx / T(2.));
-            uy = (double)(uy9 + (T)a.fy / T(2.));
-            uz = (double)(uz9 + (T)a.fz / T(2.));
+            uint32_t cx, cy, cz;
+            node_coords(n, a.g, cx, cy, cz);
+            double fx, fy, fz;
+            node_force(a.f, a.g, cx, cy, cz, fx, fy, fz);
+            ux = (double)(ux9 + (T)fx / T(2.));
+            uy = (double)(uy9 + (T)fy / T(2.));
+            uz = (double)(uz9 + (T)fz / T(2.));
             rho = (double)(rho9 - (T)a.rho_off);
         }
         s[0] += ux;
@@ -625,9 +697,11 @@ __global__ void __launch_bounds__(BLOCK) k_filtered_plane_partials(PlaneArgs a)
                     pull_all<P>((const S *)a.L, a.g.N, b, f);
                     T rho9, ux9, uy9, uz9;
                     macro_sums(f, rho9, ux9, uy9, uz9);
-                    ux += (double)(ux9 + (T)a.fx / T(2.));
-                    uy += (double)(uy9 + (T)a.fy / T(2.));
-                    uz += (double)(uz9 + (T)a.fz / T(2.));
+                    double fx, fy, fz;
+                    node_force(a.f, a.g, 2 * ic + dx, 2 * cy + dy, 2 * cz + dz, fx, fy, fz);
+                    ux += (double)(ux9 + (T)fx / T(2.));
+                    uy += (double)(uy9 + (T)fy / T(2.));
+                    uz += (double)(uz9 + (T)fz / T(2.));
                     rho += (double)(rho9 - (T)a.rho_off);
                 }
         rho *= 0.125;
@@ -715,6 +789,21 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
             return hipErrorInvalidValue;
         else
             hipLaunchKernelGGL((k_step<P, 12>), grid, blk, 0, s, a);
+        break;
+    case 16:
+        hipLaunchKernelGGL((k_step<P, 16>), grid, blk, 0, s, a);
+        break;
+    case 17:
+        hipLaunchKernelGGL((k_step<P, 17>), grid, blk, 0, s, a);
+        break;
+    case 18:
+        hipLaunchKernelGGL((k_step<P, 18>), grid, blk, 0, s, a);
+        break;
+    case 24:
+        if constexpr (P == 0)
+            return hipErrorInvalidValue;
+        else
+            hipLaunchKernelGGL((k_step<P, 24>), grid, blk, 0, s, a);
         break;
     default:
         return hipErrorInvalidValue;

@@ -15,6 +15,7 @@
 // and materialises the arrays only when a call needs stored values.
 #include <hip/hip_runtime.h>
 
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -69,7 +70,9 @@ struct lbm_channel {
     double *mac[4]; // rho, ux, uy, uz (allocated on first use)
     MacroState ms;
     double rho_off;
-    double force[3];
+    ForceSpec fc;       // the force arrays' contents: FORCING (uniform) or FORCINGP
+    ForceSpec fm;       // the force that the fields FROM_CUR / FROM_PREV are formed with
+    double *ftab;       // FORCINGP factor tables (allocated on first use)
     hipStream_t stream;
     double *partials;   // 2*RED_BLOCKS doubles, then PLANE_CHUNKS*NPLANE*lx doubles
     double *red_out;    // 2 doubles + NPLANE*lx doubles of final sums
@@ -163,9 +166,9 @@ RatesD rates_of(const lbm_channel *h)
     r.omegepsl = c.omegepsl;
     r.omegepslj = c.omegepslj;
     r.omegxx = c.omegxx;
-    r.fx = h->force[0];
-    r.fy = h->force[1];
-    r.fz = h->force[2];
+    r.fx = h->fc.fx;
+    r.fy = h->fc.fy;
+    r.fz = h->fc.fz;
     return r;
 }
 
@@ -201,12 +204,25 @@ int compute_fields(lbm_channel *h, int which)
     a.ux = h->mac[1];
     a.uy = h->mac[2];
     a.uz = h->mac[3];
-    a.fx = h->force[0];
-    a.fy = h->force[1];
-    a.fz = h->force[2];
+    a.f = h->fm;
     a.rho_off = h->rho_off;
     a.errbits = nullptr;
     return dispatch_err(PREC_CALL(h, macro, a, 0, h->stream), "macrovar");
+}
+
+// macrovar of the current lattice (with the force arrays as they are now)
+// becomes what the fields hold.
+void fields_from_cur(lbm_channel *h)
+{
+    h->ms = FROM_CUR;
+    h->fm = h->fc;
+}
+
+// Do two force specifications give the same force at every node?
+bool same_force(const ForceSpec &a, const ForceSpec &b)
+{
+    return a.tab == b.tab && a.fx == b.fx && a.fy == b.fy && a.fz == b.fz &&
+           (a.tab == nullptr || (a.a == b.a && a.b == b.b && a.c == b.c));
 }
 
 // Make the arrays hold what the reference's arrays hold now.
@@ -256,6 +272,16 @@ int launch_step(lbm_channel *h)
     if (h->part) {
         mode |= 4;
         a.solid = h->part->ps.solid;
+    }
+    a.fc = h->fc;
+    a.fm = h->fm;
+    // a force that varies over the nodes, or fused fields formed with a force
+    // other than the collision's
+    if (h->fc.tab || (h->ms != ARRAYS && !same_force(h->fm, h->fc))) {
+        if (h->part)
+            return fail(LBM_ERR_STATE, "collision: particles need a uniform force that macrovar has seen "
+                                       "(FORCINGP is not combined with particles)");
+        mode |= 16;
     }
     a.mlp = h->mlp;
     a.inv_ystar = h->mlp_inv_ystar;
@@ -318,9 +344,7 @@ int plane_sums(lbm_channel *h, bool acc)
     a.ux = h->mac[1];
     a.uy = h->mac[2];
     a.uz = h->mac[3];
-    a.fx = h->force[0];
-    a.fy = h->force[1];
-    a.fz = h->force[2];
+    a.f = h->fm;
     a.rho_off = h->rho_off;
     a.solid = solid_of(h);
     a.partials = h->partials + 2 * RED_BLOCKS;
@@ -345,9 +369,7 @@ int filtered_plane_sums(lbm_channel *h)
     a.g = h->g;
     a.L = h->lat[h->ms == FROM_PREV ? 1 - h->cur : h->cur];
     a.rho = a.ux = a.uy = a.uz = nullptr;
-    a.fx = h->force[0];
-    a.fy = h->force[1];
-    a.fz = h->force[2];
+    a.f = h->fm;
     a.rho_off = h->rho_off;
     a.solid = nullptr;
     a.partials = h->partials + 2 * RED_BLOCKS;
@@ -471,7 +493,7 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         const char *k = getenv("LBM_STEP_KERNEL");
         h->runs = (k && strcmp(k, "runs") == 0) ? 1 : 0;
     }
-    h->ms = FROM_CUR;
+    fields_from_cur(h);
     h->fresh = true;
 
     auto cleanup = [&](int code) {
@@ -542,6 +564,8 @@ int lbm_destroy(lbm_channel *h)
     }
     if (h->mlp)
         (void)hipFree(h->mlp);
+    if (h->ftab)
+        (void)hipFree(h->ftab);
     if (h->ev[0])
         (void)hipEventDestroy(h->ev[0]);
     if (h->ev[1])
@@ -630,15 +654,74 @@ int lbm_download_f(lbm_channel *h, double *f)
 int lbm_set_force(lbm_channel *h, double fx, double fy, double fz)
 {
     CHECK_H(h);
-    // the arrays the reference holds were formed with the old force
-    if (h->ms != ARRAYS && !h->fresh && (fx != h->force[0] || fy != h->force[1] || fz != h->force[2])) {
-        int rc = materialize(h);
-        if (rc)
-            return rc;
+    // the fields keep the force they were formed with (fm) until the next
+    // macrovar; a fresh handle has formed none yet
+    h->fc.fx = fx;
+    h->fc.fy = fy;
+    h->fc.fz = fz;
+    h->fc.tab = nullptr;
+    h->fc.a = h->fc.b = h->fc.c = 0.0;
+    if (h->fresh)
+        h->fm = h->fc;
+    return LBM_OK;
+}
+
+int lbm_forcingp(lbm_channel *h, int istep, double force_in_y)
+{
+    CHECK_H(h);
+    if (h->part)
+        return fail(LBM_ERR_STATE, "lbm_forcingp: not for particle-laden channels (Channel-Flow/main.f90:148)");
+    const uint32_t lx = h->g.lx, ly = h->g.ly, lz = h->g.lz;
+    const double pi2 = 2.0 * (4.0 * atan(1.0)); // var_inc.f90:71-72
+    const double beta9 = 3.0, gamma9 = 2.0, phase9 = 0.25, Tpd = 2000.;
+    const int ixs0 = 2;
+    const int ihh = (int)(lx / 2) / 2; // lxh/2, var_inc.f90:53
+    if (!h->ftab) {
+        // factors of one coordinate each (collision.f90:561-599, one rank);
+        // computed with the C library's sin and cos, as the oracle does
+        std::vector<double> t(3 * (size_t)lx + 4 * (size_t)ly + 2 * (size_t)lz, 0.0);
+        for (int band = 1; band <= 2; band++) {
+            const int ixs = band == 1 ? ixs0 : (int)lx - ixs0 - ihh; // the second band is written last
+            for (int i = 1; i <= ihh; i++) {
+                const double x9 = pi2 * ((double)i - 0.5) / (double)ihh;
+                double *xt = &t[3 * (size_t)(i + ixs - 1)];
+                xt[0] = (double)band;
+                xt[1] = 1. - cos(x9);
+                xt[2] = sin(x9);
+            }
+        }
+        for (uint32_t j = 1; j <= ly; j++) {
+            const double y1 = pi2 * ((double)j - 0.5) / (double)ly;
+            const double y2 = pi2 * (((double)j - 0.5) / (double)ly + phase9);
+            double *yt = &t[3 * (size_t)lx + 4 * (size_t)(j - 1)];
+            yt[0] = cos(beta9 * y1);
+            yt[1] = sin(beta9 * y1);
+            yt[2] = cos(beta9 * y2);
+            yt[3] = sin(beta9 * y2);
+        }
+        for (uint32_t k = 1; k <= lz; k++) {
+            const double z9 = pi2 * ((double)k - 0.5) / (double)lz;
+            double *zt = &t[3 * (size_t)lx + 4 * (size_t)ly + 2 * (size_t)(k - 1)];
+            zt[0] = cos(gamma9 * z9);
+            zt[1] = sin(gamma9 * z9);
+        }
+        hipError_t e = hipMalloc((void **)&h->ftab, t.size() * sizeof(double));
+        if (e != hipSuccess)
+            return fail(LBM_ERR_NOMEM, "hipMalloc of the FORCINGP tables failed: %s", hipGetErrorString(e));
+        HIP_TRY(hipMemcpyAsync(h->ftab, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
     }
-    h->force[0] = fx;
-    h->force[1] = fy;
-    h->force[2] = fz;
+    // collision.f90:541 and the amplitude factors of :571-576
+    const double Amp0 = 40.00 * beta9 / (double)ly * sin(pi2 * (double)istep / Tpd);
+    h->fc.fx = 0.0;
+    h->fc.fy = force_in_y;
+    h->fc.fz = 0.0;
+    h->fc.tab = h->ftab;
+    h->fc.a = force_in_y * 0.5 * Amp0 * (double)ihh;
+    h->fc.b = Amp0 * (double)ly / beta9;
+    h->fc.c = force_in_y * 0.5 * Amp0 * (double)lz / gamma9;
+    if (h->fresh)
+        h->fm = h->fc;
     return LBM_OK;
 }
 
@@ -687,7 +770,7 @@ int lbm_collision_mrt(lbm_channel *h)
 int lbm_macrovar(lbm_channel *h, double *rho, double *ux, double *uy, double *uz)
 {
     CHECK_H(h);
-    h->ms = FROM_CUR;
+    fields_from_cur(h);
     h->rho_off = 0.0;
     h->fresh = false;
     double *dst[4] = {rho, ux, uy, uz};
@@ -718,9 +801,7 @@ int lbm_rhoupdat(lbm_channel *h, double *rhoerrmax)
     a.ux = h->mac[1];
     a.uy = h->mac[2];
     a.uz = h->mac[3];
-    a.fx = h->force[0];
-    a.fy = h->force[1];
-    a.fz = h->force[2];
+    a.f = h->fm;
     a.rho_off = 0.0;
     a.errbits = h->errbits;
     rc = dispatch_err(PREC_CALL(h, macro, a, 1, h->stream), "rhoupdat");
@@ -776,7 +857,7 @@ int lbm_run(lbm_channel *h, int nsteps)
         int rc = launch_step(h);
         if (rc)
             return rc;
-        h->ms = FROM_CUR; // macrovar
+        fields_from_cur(h); // macrovar
         h->rho_off = 0.0;
     }
     return LBM_OK;
@@ -924,7 +1005,7 @@ int lbm_time_steps(lbm_channel *h, int nsteps, double *avg_ms, double *total_ms)
     for (int s = 0; s < nsteps && rc == LBM_OK; s++) {
         HIP_TRY(hipEventRecord(ev[2 * s], h->stream));
         rc = launch_step(h);
-        h->ms = FROM_CUR;
+        fields_from_cur(h);
         h->rho_off = 0.0;
         HIP_TRY(hipEventRecord(ev[2 * s + 1], h->stream));
     }
@@ -1010,6 +1091,8 @@ int lbm_particles_create(lbm_channel *h, const lbm_particle_config *pc, const do
         return fail(LBM_ERR_ARG, "lbm_particles_create: null argument");
     if (h->part)
         return fail(LBM_ERR_STATE, "lbm_particles_create: particles already exist on this handle");
+    if (h->fc.tab)
+        return fail(LBM_ERR_STATE, "lbm_particles_create: particles need a uniform force (FORCING, not FORCINGP)");
     if (pc->npart < 1 || !(pc->rad > 0.0) || !(pc->rhopart > 0.0) || pc->reserved != 0)
         return fail(LBM_ERR_ARG, "lbm_particles_create: need npart >= 1, rad > 0, rhopart > 0, reserved = 0");
     const double lx = h->g.lx, ly = h->g.ly, lz = h->g.lz;
@@ -1044,9 +1127,9 @@ int lbm_particles_create(lbm_channel *h, const lbm_particle_config *pc, const do
     pg.stf = pc->stf;
     pg.stf_w = pc->stf_w;
     pg.pressure_force = pc->pressure_force;
-    pg.fx = h->force[0];
-    pg.fy = h->force[1];
-    pg.fz = h->force[2];
+    pg.fx = h->fc.fx;
+    pg.fy = h->fc.fy;
+    pg.fz = h->fc.fz;
     const size_t np = pc->npart, nl = np * pg.maxlink, nf = np * pg.maxfill;
     int rc = LBM_OK;
     auto A = [&](void **p, size_t bytes) {
@@ -1156,9 +1239,9 @@ int lbm_beads_move(lbm_channel *h)
 {
     CHECK_P(h);
     Part *pt = h->part;
-    pt->pg.fx = h->force[0];
-    pt->pg.fy = h->force[1];
-    pt->pg.fz = h->force[2];
+    pt->pg.fx = h->fc.fx;
+    pt->pg.fy = h->fc.fy;
+    pt->pg.fz = h->fc.fz;
     return dispatch_err(part_move(pt->pg, pt->ps, h->stream), "beads_move");
 }
 
@@ -1173,7 +1256,7 @@ int lbm_run_particles(lbm_channel *h, int nsteps)
             rc = launch_step(h);
         if (rc)
             return rc;
-        h->ms = FROM_CUR; // macrovar
+        fields_from_cur(h); // macrovar
         h->rho_off = 0.0;
         rc = lbm_beads_move(h);
         if (!rc)
@@ -1338,6 +1421,8 @@ int lbm_trainer_add_filtered(lbm_trainer *t, lbm_channel *fine, double nu_coarse
                     fine->g.ly, fine->g.lz);
     if (fine->part)
         return fail(LBM_ERR_ARG, "lbm_trainer_add_filtered: not for particle-laden channels");
+    if (fine->fc.tab)
+        return fail(LBM_ERR_ARG, "lbm_trainer_add_filtered: the fine channel needs a uniform force (FORCING)");
     const uint32_t lxc = fine->g.lx / 2, lyc = fine->g.ly / 2, lzc = fine->g.lz / 2;
     const long long Nc = (long long)lxc * lyc * lzc;
     if (t->count + Nc > t->capacity)
@@ -1351,9 +1436,9 @@ int lbm_trainer_add_filtered(lbm_trainer *t, lbm_channel *fine, double nu_coarse
     FilterArgs fa;
     fa.gf = fine->g;
     fa.L = fine->lat[fine->cur];
-    fa.fx = fine->force[0];
-    fa.fy = fine->force[1];
-    fa.fz = fine->force[2];
+    fa.fx = fine->fc.fx;
+    fa.fy = fine->fc.fy;
+    fa.fz = fine->fc.fz;
     fa.lxc = lxc;
     fa.lyc = lyc;
     fa.lzc = lzc;

@@ -62,6 +62,7 @@ _SIGS = {
     "lbm_upload_f": [_H, _D],
     "lbm_download_f": [_H, _D],
     "lbm_set_force": [_H, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+    "lbm_forcingp": [_H, ctypes.c_int, ctypes.c_double],
     "lbm_set_macros": [_H, _D, _D, _D, _D],
     "lbm_initpop": [_H],
     "lbm_collision_mrt": [_H],
@@ -225,6 +226,25 @@ class Channel:
 
     def set_force(self, fx, fy, fz):
         _call("lbm_set_force", self._h, float(fx), float(fy), float(fz))
+
+    def FORCINGP(self, istep, force_in_y=None):
+        """Channel-Flow/collision.f90:529-602: mean force plus the banded
+        divergence-free perturbation of time step istep (force_in_y of
+        para.f90:65 unless given)."""
+        fiy = self.p["force_in_y"] if force_in_y is None else force_in_y
+        _call("lbm_forcingp", self._h, int(istep), float(fiy))
+
+    def run_forcingp(self, istep0, nsteps, npforcing, force_in_y=None):
+        """Main loop of Channel-Flow/main.f90:142-161 with the perturbation
+        forcing switched on (the commented block at :148-153): FORCINGP before
+        the collision while istep < npforcing, FORCING at istep == npforcing."""
+        for istep in range(istep0 + 1, istep0 + nsteps + 1):
+            if istep < npforcing:
+                self.FORCINGP(istep, force_in_y)
+            elif istep == npforcing:
+                self.FORCING()
+            self.collision_MRT()
+            self.macrovar()
 
     def initvel(self):
         """Channel-Flow/initial.f90:94-115: log-law streamwise profile, u_x = u_z = 0."""

@@ -19,6 +19,7 @@
       ! run control (namelist /channel/)
       integer :: lx = 32, ly = 64, lz = 32
       integer :: nsteps = 1000, ndiag = 250, nflowout = 0, nstat = 0
+      integer :: npforcing = 0   ! last step of the perturbation forcing (para.f90:72-73)
       integer :: precision = 0, device = 0, mrttype_in = 0
       logical :: laminarflow = .false., newrun = .true., newinitflow = .true.
       logical :: savecntd = .true.
@@ -47,7 +48,7 @@
       real(8), allocatable, target :: f(:,:,:,:)
       real(8), allocatable, target :: rho(:,:,:), ux(:,:,:), uy(:,:,:), uz(:,:,:)
 
-      namelist /channel/ lx, ly, lz, nsteps, ndiag, nflowout, nstat, precision, &
+      namelist /channel/ lx, ly, lz, nsteps, ndiag, nflowout, nstat, npforcing, precision, &
                          device, mrttype_in, laminarflow, newrun, newinitflow,  &
                          savecntd, istpload, dirgenr, rhoepsl
 

@@ -9,6 +9,7 @@
 !   rhoupdat       -> lbm_rhoupdat        (collision.f90:469-480)
 !   avedensity     -> lbm_avedensity      (collision.f90:487-513)
 !   FORCING        -> lbm_set_force       (collision.f90:515-527)
+!   FORCINGP       -> lbm_forcingp        (collision.f90:529-602)
 ! macrovar leaves the fields on the device; fetch_fields copies them to the
 ! host arrays when an I/O routine needs them.
 !=============================================================================
@@ -52,3 +53,11 @@
       implicit none
       call lbm_check(lbm_set_force(lbm, 0.0d0, force_in_y*force_mag, 0.0d0), 'FORCING')
       end subroutine FORCING
+
+      ! Perturbation forcing at the current istep (mean force_in_y plus the
+      ! banded divergence-free perturbation); the device evaluates it per node.
+      subroutine FORCINGP
+      use channel_vars
+      implicit none
+      call lbm_check(lbm_forcingp(lbm, int(istep, c_int), force_in_y), 'FORCINGP')
+      end subroutine FORCINGP

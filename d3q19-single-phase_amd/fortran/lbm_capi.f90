@@ -67,6 +67,13 @@
           real(c_double), value :: fx, fy, fz
         end function
 
+        integer(c_int) function lbm_forcingp(h, istep, force_in_y) bind(C, name="lbm_forcingp")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          integer(c_int), value :: istep
+          real(c_double), value :: force_in_y
+        end function
+
         integer(c_int) function lbm_set_macros(h, rho, ux, uy, uz) bind(C, name="lbm_set_macros")
           import :: c_int, c_ptr
           type(c_ptr), value :: h, rho, ux, uy, uz

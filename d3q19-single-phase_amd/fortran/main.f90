@@ -65,6 +65,12 @@
 
       ! main.f90:142-208
       do istep = istep0 + 1, istep0 + nsteps
+        ! perturbation forcing until npforcing (the block at main.f90:148-153)
+        if (istep < npforcing) then
+          call FORCINGP
+        else if (istep == npforcing) then
+          call FORCING
+        end if
         call collision_MRT
         call macrovar
         if (ndiag > 0) then

@@ -85,8 +85,20 @@ int lbm_upload_f(lbm_channel *h, const double *f);
 int lbm_download_f(lbm_channel *h, double *f);
 
 /* FORCING, Channel-Flow/collision.f90:515-527: a uniform body force
- * (force_realx, force_realy, force_realz). */
+ * (force_realx, force_realy, force_realz).  Fields formed by an earlier
+ * macrovar keep the force they were formed with, as the reference's arrays
+ * do, until the next macrovar. */
 int lbm_set_force(lbm_channel *h, double fx, double fy, double fz);
+
+/* FORCINGP, Channel-Flow/collision.f90:529-602 (one rank): the force arrays
+ * become the mean force (0, force_in_y, 0) plus the perturbation of time step
+ * istep, a divergence-free field of 3 streamwise and 2 spanwise waves in two
+ * bands of lx/4 wall-normal nodes, amplitude 40*3/ly*sin(2 pi istep/2000).
+ * The driver calls it before collision_MRT while istep < npforcing and
+ * FORCING at istep == npforcing (main.f90:148-153).  The force is evaluated
+ * per node inside the step from tables of one coordinate each (no 3 x N
+ * force arrays).  Not combined with particles (main.f90:148). */
+int lbm_forcingp(lbm_channel *h, int istep, double force_in_y);
 
 /* Sets the macroscopic fields the next collision reads, as the driver does by
  * assigning the module arrays (initial.f90:94-115 initvel, :32 rho = 0).

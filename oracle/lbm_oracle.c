@@ -608,6 +608,61 @@ void orc_forcing(const orc_params *p, double *fx, double *fy, double *fz)
     }
 }
 
+/* FORCINGP, Channel-Flow/collision.f90:529-602, one rank (alphay = alphaz = 0,
+ * collision.f90:551-559): the mean force plus a divergence-free perturbation
+ * in two wall-parallel bands, beta = 3 streamwise and gamma = 2 spanwise
+ * waves, amplitude 40*beta/ny*sin(2 pi istep / 2000).  The second band
+ * (collision.f90:583-599) is shifted a quarter period in y and overwrites the
+ * first where the two meet.  Every expression keeps the reference's operand
+ * order (Fortran evaluates a*b/c*d left to right; a leading minus negates
+ * the whole product, which equals negating its first factor exactly). */
+void orc_forcingp(const orc_params *p, int istep, double *fx, double *fy, double *fz)
+{
+    const int lx = p->lx, ly = p->ly, lz = p->lz, nx = p->lx, ny = p->ly, nz = p->lz;
+    const double pi = 4.0 * atan(1.0); /* var_inc.f90:71-72 */
+    const double pi2 = 2.0 * pi;
+    const double Tpd = 2000., beta9 = 3.0, gamma9 = 2.0, phase9 = 0.25;
+    const int ixs0 = 2;
+    const double Amp0 = 40.00 * beta9 / (double)ny * sin(pi2 * (double)istep / Tpd); /* :541 */
+    const double force_in_y = p->force_in_y;
+    const size_t n = (size_t)lx * ly * lz;
+    for (size_t m = 0; m < n; m++) { /* :546-548 */
+        fx[m] = 0.0;
+        fy[m] = force_in_y;
+        fz[m] = 0.0;
+    }
+    const int lxh = lx / 2; /* var_inc.f90:53 */
+    int ixs = ixs0, ihh = lxh / 2;
+    for (int k = 1; k <= lz; k++) { /* :563-580 */
+        const double z9 = pi2 * ((double)k - 0.5) / (double)nz;
+        for (int j = 1; j <= ly; j++) {
+            const double y9 = pi2 * ((double)j - 0.5) / (double)ny;
+            for (int i = 1; i <= ihh; i++) {
+                const double x9 = pi2 * ((double)i - 0.5) / (double)ihh;
+                const size_t m = (size_t)(i + ixs - 1) + (size_t)lx * ((size_t)(j - 1) + (size_t)ly * (k - 1));
+                fx[m] = force_in_y * 0.5 * Amp0 * (double)ihh * (1. - cos(x9)) * cos(beta9 * y9) * cos(gamma9 * z9);
+                fy[m] = force_in_y * (1.0 - Amp0 * (double)ny / beta9 * sin(x9) * sin(beta9 * y9) * cos(gamma9 * z9));
+                fz[m] = force_in_y * 0.5 * Amp0 * (double)nz / gamma9 * sin(x9) * cos(beta9 * y9) * sin(gamma9 * z9);
+            }
+        }
+    }
+    ihh = lxh / 2;
+    ixs = nx - ixs0 - ihh;
+    for (int k = 1; k <= lz; k++) { /* :583-599 */
+        const double z9 = pi2 * ((double)k - 0.5) / (double)nz;
+        for (int j = 1; j <= ly; j++) {
+            const double y9 = pi2 * (((double)j - 0.5) / (double)ny + phase9);
+            for (int i = 1; i <= ihh; i++) {
+                const double x9 = pi2 * ((double)i - 0.5) / (double)ihh;
+                const size_t m = (size_t)(i + ixs - 1) + (size_t)lx * ((size_t)(j - 1) + (size_t)ly * (k - 1));
+                fx[m] = -(force_in_y * 0.5 * Amp0 * (double)ihh * (1. - cos(x9)) * cos(beta9 * y9) * cos(gamma9 * z9));
+                fy[m] = force_in_y * (1.0 + Amp0 * (double)ny / beta9 * sin(x9) * sin(beta9 * y9) * cos(gamma9 * z9));
+                fz[m] = -(force_in_y * 0.5 * Amp0 * (double)nz / gamma9 * sin(x9) * cos(beta9 * y9) * sin(gamma9 * z9));
+            }
+        }
+    }
+}
+
 /* Channel-Flow/initial.f90:26-44: rho = 0, f = equilibrium. */
 void orc_initpop(const orc_params *p, double *f, double *rho, const double *ux, const double *uy,
                  const double *uz)

@@ -32,6 +32,7 @@ void orc_macrovar(const orc_params *p, const double *f, double *rho, double *ux,
 void orc_rhoupdat(const orc_params *p, const double *f, double *rho);
 double orc_avedensity(const orc_params *p, double *rho);
 void orc_forcing(const orc_params *p, double *fx, double *fy, double *fz);
+void orc_forcingp(const orc_params *p, int istep, double *fx, double *fy, double *fz);
 void orc_initpop(const orc_params *p, double *f, double *rho, const double *ux, const double *uy,
                  const double *uz);
 void orc_initvel(const orc_params *p, double *ux, double *uy, double *uz);

@@ -75,6 +75,7 @@ def lib():
         L.orc_avedensity.argtypes = [P, D]
         L.orc_avedensity.restype = _DBL
         L.orc_forcing.argtypes = [P, D, D, D]
+        L.orc_forcingp.argtypes = [P, _INT, D, D, D]
         L.orc_initpop.argtypes = [P] + [D] * 5
         L.orc_initvel.argtypes = [P, D, D, D]
         L.orc_statistc2.argtypes = [P] + [D] * 5
@@ -122,6 +123,10 @@ class OracleChannel:
     # --- reference subroutines -------------------------------------------
     def FORCING(self):
         lib().orc_forcing(ctypes.byref(self.p), _ptr(self.fx), _ptr(self.fy), _ptr(self.fz))
+
+    def FORCINGP(self, istep):
+        """Perturbation forcing of collision.f90:529-602 at time step istep."""
+        lib().orc_forcingp(ctypes.byref(self.p), int(istep), _ptr(self.fx), _ptr(self.fy), _ptr(self.fz))
 
     def initvel(self):
         lib().orc_initvel(ctypes.byref(self.p), _ptr(self.ux), _ptr(self.uy), _ptr(self.uz))
@@ -178,6 +183,17 @@ class OracleChannel:
             if rhoerrmax <= rhoepsl or istep > maxiter:
                 return istep, rhoerrmax
             istep += 1
+
+    def run_forcingp(self, istep0, nsteps, npforcing):
+        """Main loop with the perturbation forcing of main.f90:148-153 switched on:
+        FORCINGP before the collision while istep < npforcing, FORCING at npforcing."""
+        for istep in range(istep0 + 1, istep0 + nsteps + 1):
+            if istep < npforcing:
+                self.FORCINGP(istep)
+            elif istep == npforcing:
+                self.FORCING()
+            self.collision_MRT()
+            self.macrovar()
 
     def run(self, nsteps, ab=False, nthreads=0):
         """Main loop body of main.f90:142-167 without I/O: collision_MRT, macrovar."""

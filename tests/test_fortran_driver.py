@@ -105,3 +105,24 @@ def test_driver_fresh_start_prerelax_bitwise(tmp_path):
     ref.run(nsteps)
     _, f = fortio.read_cntd(str(tmp_path / "cntdflow" / f"endrunflow2D16x8.{nsteps:07d}.000"), ref.f.shape)
     assert np.array_equal(f.view(np.uint64), ref.f.view(np.uint64))
+
+
+@pytest.mark.gpu
+def test_driver_perturbation_forcing_bitwise(tmp_path):
+    """npforcing > 0 switches on the perturbation forcing of main.f90:148-153:
+    FORCINGP before each collision until npforcing, then FORCING."""
+    lx, ly, lz, nsteps, npforcing = 16, 12, 8, 40, 30
+    for d in ("initflow", "cntdflow", "diag", "stat"):
+        (tmp_path / d).mkdir()
+    ref = O.OracleChannel(lx, ly, lz)
+    ref.ux[:], ref.uy[:], ref.uz[:] = workloads.laminar_parabola_noise(lx, ly, lz, seed=22)
+    ref.initpop()
+    fortio.write_finit(str(tmp_path / "initflow" / "finit.000"), ref.f)
+    _nml(tmp_path / "run.nml", lx=lx, ly=ly, lz=lz, nsteps=nsteps, npforcing=npforcing, newinitflow=".false.",
+         ndiag=0, dirgenr=f"'{tmp_path}/'")
+    _run(tmp_path / "run.nml", tmp_path)
+    ref.FORCING()
+    ref.macrovar()
+    ref.run_forcingp(0, nsteps, npforcing)
+    _, f = fortio.read_cntd(str(tmp_path / "cntdflow" / f"endrunflow2D16x8.{nsteps:07d}.000"), ref.f.shape)
+    assert np.array_equal(f.view(np.uint64), ref.f.view(np.uint64))

@@ -117,7 +117,7 @@ def test_filtered_statistics_match_box_filter_of_fields(prec):
     assert np.allclose(out[:, 0], np.arange(lx // 2) + 0.5)
 
 
-def _run_fp32(lx, ly, lz, vel, steps, visc=None, closure=None, ystar=None):
+def _run_fp32(lx, ly, lz, vel, steps, visc=None, closure=None, ystar=None, forcingp=False):
     with d3q19.Channel(lx, ly, lz, "fp32", visc=visc) as ch:
         ch.set_macros(ux=vel[0], uy=vel[1], uz=vel[2])
         ch.set_force(0.0, 2e-5, 0.0)
@@ -125,20 +125,24 @@ def _run_fp32(lx, ly, lz, vel, steps, visc=None, closure=None, ystar=None):
         ch.macrovar()
         if closure is not None:
             ch.closure_set(closure, ystar=ystar)
-        ch.run(steps)
+        if forcingp:  # perturbation forcing (force varying over the nodes) from step 400 on
+            ch.run_forcingp(400, steps, npforcing=10**9, force_in_y=2e-5)
+        else:
+            ch.run(steps)
         return ch.download_f()
 
 
-def test_constant_eddy_viscosity_equals_doubled_viscosity():
+@pytest.mark.parametrize("forcingp", [False, True])
+def test_constant_eddy_viscosity_equals_doubled_viscosity(forcingp):
     """Zero weights and b3 = log(e - 1): nu_t = nu everywhere, i.e. plain MRT
-    with viscosity 2 nu."""
+    with viscosity 2 nu (also under the perturbation forcing)."""
     lx, ly, lz = 32, 24, 16
     vel = _fine_flow(lx, ly, lz, 4)
     nu = 0.0036
     p = d3q19.closure_params(init="zero", nu=nu)
     p[M.B3] = np.log(np.e - 1.0)
-    a = _run_fp32(lx, ly, lz, vel, 100, visc=None, closure=p, ystar=1.0)
-    b = _run_fp32(lx, ly, lz, vel, 100, visc=2 * nu)
+    a = _run_fp32(lx, ly, lz, vel, 100, visc=None, closure=p, ystar=1.0, forcingp=forcingp)
+    b = _run_fp32(lx, ly, lz, vel, 100, visc=2 * nu, forcingp=forcingp)
     # the closure forms its rate in FP32, plain MRT rounds the FP64 rate once
     assert np.max(np.abs(a - b)) < 1e-5 * np.max(np.abs(b))
 
