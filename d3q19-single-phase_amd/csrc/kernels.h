@@ -96,6 +96,9 @@ constexpr int NPLANE = 12;
 template <int P>
 struct Launch {
     static hipError_t step(const StepArgs &a, int mode, hipStream_t s);
+    // closure inputs (what 0, 10 floats per node) or nu_t/nu (what 1) of nodes [n_begin, n_begin + count)
+    static hipError_t closure_eval(const StepArgs &a, uint64_t n_begin, uint64_t count, float *out, int what,
+                                   hipStream_t s);
     static hipError_t macro(const MacroArgs &a, int what, hipStream_t s);
     static hipError_t gather(const CopyArgs &a, hipStream_t s);
     static hipError_t scatter(const CopyArgs &a, hipStream_t s);

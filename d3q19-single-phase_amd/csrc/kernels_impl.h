@@ -147,16 +147,32 @@ __device__ __forceinline__ uint32_t node_index(const Geom &g)
 // changed after the last macrovar): a.fc is the collision's force, a.fm the
 // fields'.
 template <int P, int MODE>
-__global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
+__device__ __forceinline__ void step_node(const StepArgs &a)
 {
     using S = typename Prec<P>::S;
     using T = typename Prec<P>::T;
-    const uint32_t n = node_index(a.g);
-    if (n >= a.g.N)
-        return;
-    if constexpr ((MODE & 4) != 0) {
-        if (a.solid[n] >= 0)
+    // The closure's network runs on the matrix cores over whole waves: there
+    // every lane stays to the end (a lane past the lattice works on the last
+    // node, a particle node on its own slots) and only active lanes store.
+    constexpr bool WAVE = (MODE & 8) != 0;
+    const uint32_t n0 = node_index(a.g);
+    bool active = n0 < a.g.N;
+    if constexpr (!WAVE) {
+        if (!active)
             return;
+    }
+    const uint32_t n = active ? n0 : (uint32_t)(a.g.N - 1);
+    if constexpr ((MODE & 4) != 0) {
+        if (a.solid[n] >= 0) {
+            if constexpr (!WAVE)
+                return;
+            active = false;
+        }
+    }
+    __shared__ __attribute__((aligned(16))) _Float16 wl[WAVE ? MLP_LDS : 8];
+    if constexpr (WAVE) {
+        mlp_stage(a.mlp, wl, threadIdx.x, BLOCK);
+        __syncthreads();
     }
     const Nbr b = make_nbr(n, a.g);
     const S *__restrict__ src = (const S *)a.src;
@@ -241,12 +257,102 @@ __global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
             const uint32_t ix = n - fdiv(n, a.g.dlx) * a.g.lx;
             const float wd = fminf((float)ix + 0.5f, (float)a.g.lx - (float)ix - 0.5f);
             mrt_collide_delta(f, rho9, ux9, uy9, uz9, drho, djx, djy, djz, r,
-                              MlpClosure{a.mlp, wd * a.inv_ystar / a.yscale});
+                              MlpClosureWave{a.mlp, wl, wd * a.inv_ystar / a.yscale});
         } else {
             mrt_collide_delta(f, rho9, ux9, uy9, uz9, drho, djx, djy, djz, r);
         }
     }
+    if constexpr (WAVE) {
+        if (!active)
+            return;
+    }
     store_all<P>(dst, a.g.N, n, f);
+}
+
+template <int P, int MODE>
+__global__ void __launch_bounds__(BLOCK) k_step(StepArgs a)
+{
+    step_node<P, MODE>(a);
+}
+
+// The closure modes, held to 128 registers so that four waves per SIMD keep
+// the lattice traffic in flight while others run the network.
+template <int P, int MODE>
+__global__ void __launch_bounds__(BLOCK, 4) k_step_closure(StepArgs a)
+{
+    step_node<P, MODE>(a);
+}
+
+// Closure diagnostics for nodes [n_begin, n_begin + count) of the current
+// lattice, as the next fused collision (mode 0, or 16 with per-node forces)
+// sees them.  WHAT 0: the ten scaled network inputs (out[10 i + k] for node
+// n_begin + i); WHAT 1: nu_t / nu = softplus(o) from the matrix-core network.
+// The preamble is mrt_collide_delta's up to the closure call.
+template <int P, int WHAT>
+__global__ void __launch_bounds__(BLOCK) k_closure_eval(StepArgs a, uint64_t n_begin, uint64_t count, float *out)
+{
+    using S = typename Prec<P>::S;
+    using T = typename Prec<P>::T;
+    const uint64_t i0 = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const bool active = i0 < count;
+    const uint32_t n = (uint32_t)(n_begin + (active ? i0 : count - 1));
+    __shared__ __attribute__((aligned(16))) _Float16 wl[WHAT == 1 ? MLP_LDS : 8];
+    if constexpr (WHAT == 1) {
+        mlp_stage(a.mlp, wl, threadIdx.x, BLOCK);
+        __syncthreads();
+    }
+    const Nbr b = make_nbr(n, a.g);
+    T f[19];
+    pull_all<P>((const S *)a.src, a.g.N, b, f);
+    Rates<T> r;
+    r.s1 = (T)a.r.s1;
+    r.s2 = (T)a.r.s2;
+    r.s4 = (T)a.r.s4;
+    r.s9 = (T)a.r.s9;
+    r.s10 = (T)a.r.s10;
+    r.s13 = (T)a.r.s13;
+    r.s16 = (T)a.r.s16;
+    r.omegepsl = (T)a.r.omegepsl;
+    r.omegepslj = (T)a.r.omegepslj;
+    r.omegxx = (T)a.r.omegxx;
+    uint32_t ix, iy, iz;
+    node_coords(n, a.g, ix, iy, iz);
+    double x, y, z;
+    node_force(a.fc, a.g, ix, iy, iz, x, y, z);
+    r.fx = (T)x;
+    r.fy = (T)y;
+    r.fz = (T)z;
+    node_force(a.fm, a.g, ix, iy, iz, x, y, z);
+    T rho9, ux9, uy9, uz9;
+    macro_sums(f, rho9, ux9, uy9, uz9);
+    ux9 = ux9 + (T)x / T(2.);
+    uy9 = uy9 + (T)y / T(2.);
+    uz9 = uz9 + (T)z / T(2.);
+    T Fb[19];
+    guo_terms(ux9, uy9, uz9, r.fx, r.fy, r.fz, Fb);
+    T f9[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++)
+        f9[q] = f[q] + T(0.5) * Fb[q];
+    T eq[12];
+    equilibria(rho9, ux9, uy9, uz9, r, eq);
+    T m[15];
+    moments(f9, m);
+    T neq[9];
+    closure_inputs(m, eq, neq);
+    const float wd = fminf((float)ix + 0.5f, (float)a.g.lx - (float)ix - 0.5f);
+    float in[MLP_IN];
+    mlp_inputs(a.mlp, neq, wd * a.inv_ystar / a.yscale, in);
+    if constexpr (WHAT == 0) {
+        if (active)
+#pragma unroll
+            for (int k = 0; k < MLP_IN; k++)
+                out[i0 * MLP_IN + k] = in[k];
+    } else {
+        const float o = mlp_forward_wave(a.mlp, wl, in);
+        if (active)
+            out[i0] = softplus(o);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -782,13 +888,13 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
         if constexpr (P == 0)
             return hipErrorInvalidValue;
         else
-            hipLaunchKernelGGL((k_step<P, 8>), grid, blk, 0, s, a);
+            hipLaunchKernelGGL((k_step_closure<P, 8>), grid, blk, 0, s, a);
         break;
     case 12:
         if constexpr (P == 0)
             return hipErrorInvalidValue;
         else
-            hipLaunchKernelGGL((k_step<P, 12>), grid, blk, 0, s, a);
+            hipLaunchKernelGGL((k_step_closure<P, 12>), grid, blk, 0, s, a);
         break;
     case 16:
         hipLaunchKernelGGL((k_step<P, 16>), grid, blk, 0, s, a);
@@ -803,12 +909,28 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
         if constexpr (P == 0)
             return hipErrorInvalidValue;
         else
-            hipLaunchKernelGGL((k_step<P, 24>), grid, blk, 0, s, a);
+            hipLaunchKernelGGL((k_step_closure<P, 24>), grid, blk, 0, s, a);
         break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int P>
+hipError_t Launch<P>::closure_eval(const StepArgs &a, uint64_t n_begin, uint64_t count, float *out, int what,
+                                   hipStream_t s)
+{
+    if constexpr (P == 0) {
+        return hipErrorInvalidValue; // the closure runs in FP32 arithmetic only
+    } else {
+        const dim3 grid(nblocks<P>(count)), blk(BLOCK);
+        if (what == 0)
+            hipLaunchKernelGGL((k_closure_eval<P, 0>), grid, blk, 0, s, a, n_begin, count, out);
+        else
+            hipLaunchKernelGGL((k_closure_eval<P, 1>), grid, blk, 0, s, a, n_begin, count, out);
+        return hipGetLastError();
+    }
 }
 
 template <int P>

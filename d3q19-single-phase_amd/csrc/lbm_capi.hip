@@ -1337,6 +1337,52 @@ int lbm_closure_set(lbm_channel *h, const float *params, double ystar)
     return LBM_OK;
 }
 
+// Closure inputs (what 0) or nu_t/nu (what 1) of every node into host memory.
+static int closure_eval(lbm_channel *h, float *out, int what, const char *name)
+{
+    CHECK_H(h);
+    if (!out)
+        return fail(LBM_ERR_ARG, "%s: null output", name);
+    if (!h->mlp)
+        return fail(LBM_ERR_STATE, "%s: no closure set (lbm_closure_set)", name);
+    if (h->ms != FROM_CUR || h->rho_off != 0.0)
+        return fail(LBM_ERR_STATE, "%s: the closure needs fields from macrovar (call lbm_macrovar first)", name);
+    StepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = h->g;
+    a.src = h->lat[h->cur];
+    a.r = rates_of(h);
+    a.fc = h->fc;
+    a.fm = h->fm;
+    a.mlp = h->mlp;
+    a.inv_ystar = h->mlp_inv_ystar;
+    a.yscale = h->mlp_yscale;
+    const int width = what == 0 ? MLP_IN : 1;
+    float *buf = nullptr;
+    const uint64_t chunk = std::min<uint64_t>(COPY_NODES, h->g.N);
+    HIP_TRY(hipMalloc((void **)&buf, chunk * width * sizeof(float)));
+    int rc = LBM_OK;
+    for (uint64_t n0 = 0; n0 < h->g.N && rc == LBM_OK; n0 += chunk) {
+        const uint64_t cnt = std::min<uint64_t>(chunk, h->g.N - n0);
+        rc = dispatch_err(PREC_CALL(h, closure_eval, a, n0, cnt, buf, what, h->stream), name);
+        if (rc == LBM_OK) {
+            hipError_t e = hipMemcpyAsync(out + n0 * width, buf, cnt * width * sizeof(float), hipMemcpyDeviceToHost,
+                                          h->stream);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess)
+                rc = fail(LBM_ERR_HIP, "%s: copy failed: %s", name, hipGetErrorString(e));
+        }
+    }
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(buf);
+    return rc;
+}
+
+int lbm_closure_inputs(lbm_channel *h, float *feat) { return closure_eval(h, feat, 0, "lbm_closure_inputs"); }
+
+int lbm_closure_nut(lbm_channel *h, float *nut) { return closure_eval(h, nut, 1, "lbm_closure_nut"); }
+
 int lbm_closure_clear(lbm_channel *h)
 {
     CHECK_H(h);

@@ -92,6 +92,8 @@ _SIGS = {
     "lbm_particles_solid": [_H, _D],
     "lbm_closure_set": [_H, _D, ctypes.c_double],
     "lbm_closure_clear": [_H],
+    "lbm_closure_inputs": [_H, _D],
+    "lbm_closure_nut": [_H, _D],
     "lbm_trainer_create": [ctypes.c_int, ctypes.c_longlong, _D, ctypes.POINTER(ctypes.c_void_p)],
     "lbm_trainer_destroy": [_H],
     "lbm_trainer_set_target": [_H, ctypes.c_int],
@@ -433,6 +435,18 @@ class Channel:
 
     def closure_clear(self):
         _call("lbm_closure_clear", self._h)
+
+    def closure_inputs(self):
+        """The ten scaled network inputs of every node, (lz, ly, lx, 10) float32."""
+        out = np.empty((self.lz, self.ly, self.lx, 10), dtype=np.float32)
+        _call("lbm_closure_inputs", self._h, out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def closure_nut(self):
+        """nu_t / nu the fused collision applies at every node, (lz, ly, lx) float32."""
+        out = np.empty((self.lz, self.ly, self.lx), dtype=np.float32)
+        _call("lbm_closure_nut", self._h, out.ctypes.data_as(ctypes.c_void_p))
+        return out
 
     def particles_solid(self):
         s = np.empty(self.N, dtype=np.int32)

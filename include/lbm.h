@@ -261,6 +261,14 @@ int lbm_closure_set(lbm_channel *h, const float *params, double ystar);
 /* Back to plain MRT. */
 int lbm_closure_clear(lbm_channel *h);
 
+/* What the closure sees and applies at every node of the current lattice, for
+ * the next collision_MRT (the fields must come from macrovar): the ten scaled
+ * network inputs (feat[10*n + k], n = ix + lx*(iy + ly*iz)), or the eddy
+ * viscosity ratio nu_t/nu = softplus(o) (nut[n]) from the same matrix-core
+ * evaluation the fused collision uses. */
+int lbm_closure_inputs(lbm_channel *h, float *feat);
+int lbm_closure_nut(lbm_channel *h, float *nut);
+
 /* Trainer: a sample store and the network on one device.  Samples are
  * (10 features, target nu_t/nu).  lbm_trainer_add_filtered box-filters the
  * fine channel's current lattice 2x2x2 and appends one sample per coarse node
