@@ -68,11 +68,11 @@ def log(msg):
 
 
 def traffic_from_profiles(prec, workload):
-    """HBM bytes per launch measured by rocprofv3 --pmc (committed summary), or None."""
+    """L2-to-fabric bytes per step measured by rocprofv3 --pmc (committed summary), or None."""
     path = os.path.join(ROOT, "profiles", f"step_{prec}_{workload}_traffic.json")
     if os.path.exists(path):
         with open(path) as fh:
-            return json.load(fh).get("hbm_bytes_per_launch")
+            return json.load(fh).get("hbm_bytes_per_step")
     return None
 
 
@@ -188,18 +188,20 @@ def measure_particles(prec, workload, npart, rad, steps, warmup):
 
 
 COPY_KERNEL_TAG = "MulFunctor"  # torch.mul(src, 2.0, out=dst): reads and writes 8 B per element
+PROBE_STEPS = 8  # steps in the traffic probe (even: whole blocked pairs)
+STEP_KERNEL_TAGS = ("k_step<", "k_step_planes<")  # single-step kernel, blocked-sweep launches
 
 
-def traffic_probe(prec, workload, steps):
-    """A short run for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: `steps` step
-    launches after two warm-up steps, then one elementwise kernel of known
-    traffic (read 2 GiB, write 2 GiB) that calibrates the counters."""
+def traffic_probe(prec, workload):
+    """A short run for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: PROBE_STEPS
+    main-loop steps (single steps or blocked pairs, as lbm_run runs them),
+    then one elementwise kernel of known traffic (read 2 GiB, write 2 GiB)
+    that calibrates the counters."""
     import torch
     lx, ly, lz, _ = WORKLOADS[workload]
+    steps = PROBE_STEPS
     with d3q19.Channel(lx, ly, lz, prec) as ch:
         init_channel(ch, lx, ly, lz)
-        ch.run(2)
-        ch.sync()
         ch.run(steps)
         ch.sync()
     n = 1 << 28
@@ -214,7 +216,8 @@ def traffic_probe(prec, workload, steps):
 
 
 def _counter_means(csv_path, counter):
-    """Mean value of `counter` per dispatch for the step kernel and the copy kernel."""
+    """`counter` summed over the step kernels' dispatches per step, and its
+    mean per dispatch of the copy kernel."""
     import csv
     step, copy = [], []
     with open(csv_path, newline="") as fh:
@@ -222,18 +225,20 @@ def _counter_means(csv_path, counter):
             if row.get("Counter_Name") != counter:
                 continue
             name, v = row.get("Kernel_Name", ""), float(row["Counter_Value"])
-            if "k_step<" in name:
+            if any(t in name for t in STEP_KERNEL_TAGS):
                 step.append(v)
             elif COPY_KERNEL_TAG in name:
                 copy.append(v)
     if not step or not copy:
         raise SystemExit(f"{csv_path}: no {counter} rows for the step ({len(step)}) or copy ({len(copy)}) kernel")
-    return sum(step) / len(step), sum(copy) / len(copy), len(step)
+    return sum(step) / PROBE_STEPS, sum(copy) / len(copy), len(step)
 
 
 def traffic_summary(prec, workload, fetch_csv, write_csv):
-    """HBM bytes per step launch from the two counter runs, each counter
-    scaled by what it reads for the copy kernel of known traffic."""
+    """Bytes per step between L2 and the memory fabric from the two counter
+    runs, each counter scaled by what it reads for the copy kernel of known
+    traffic.  With the blocked sweep the 256 MB Infinity Cache behind this
+    point serves part of it, so the figure bounds HBM traffic from above."""
     lx, ly, lz, _ = WORKLOADS[workload]
     algo = lx * ly * lz * BYTES_PER_NODE[prec]
     f_step, f_copy, nf = _counter_means(fetch_csv, "FETCH_SIZE")
@@ -243,12 +248,12 @@ def traffic_summary(prec, workload, fetch_csv, write_csv):
     w_scale = copy_bytes / (w_copy * 1024.0)
     fetch = f_step * 1024.0 * f_scale
     write = w_step * 1024.0 * w_scale
-    return {"prec": prec, "workload": workload, "step_dispatches": [nf, nw],
-            "fetch_kib_per_launch": f_step, "write_kib_per_launch": w_step,
+    return {"prec": prec, "workload": workload, "step_dispatches": [nf, nw], "steps": PROBE_STEPS,
+            "fetch_kib_per_step": f_step, "write_kib_per_step": w_step,
             "copy_fetch_kib": f_copy, "copy_write_kib": w_copy,
             "fetch_scale": f_scale, "write_scale": w_scale,
             "fetch_bytes": fetch, "write_bytes": write,
-            "hbm_bytes_per_launch": round(fetch + write), "algorithmic_bytes": algo,
+            "hbm_bytes_per_step": round(fetch + write), "algorithmic_bytes": algo,
             "traffic_over_algorithmic": (fetch + write) / algo}
 
 
@@ -300,7 +305,7 @@ def main():
         WORKLOADS["grid"] = (lx, ly, lz, f"channel {ly}x{lz}x{lx} (streamwise x spanwise x wall-normal)")
         args.workload = "grid"
     if args.traffic_probe:
-        traffic_probe(args.traffic_probe, args.workload, 5)
+        traffic_probe(args.traffic_probe, args.workload)
         return
     if args.traffic_from:
         s = traffic_summary(args.prec, args.workload, *args.traffic_from)

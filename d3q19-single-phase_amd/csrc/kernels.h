@@ -93,6 +93,116 @@ struct PlaneArgs {
 // Plane sums per wall-normal index: 11 moments and the fluid-node count.
 constexpr int NPLANE = 12;
 
+// Where the z planes of one lattice state live during a two-step blocked
+// sweep: plane p in slot p % ring_slots of `ring` (a whole lattice when
+// ring_slots = lz), except planes head_lo .. head_lo + head_n - 1, which a
+// reader takes from `head` (and a writer also stores there).  Strides count
+// elements between consecutive populations of a buffer.
+struct PlaneStore {
+    void *ring;
+    uint64_t ring_stride;
+    uint32_t ring_slots;
+    void *head; // nullable
+    uint64_t head_stride;
+    int32_t head_lo;
+    uint32_t head_n;
+};
+
+// Plane p of store s as a reader sees it: buffer, population stride and the
+// element offset of the plane.
+struct PlaneRef {
+    void *buf;
+    uint64_t stride;
+    uint64_t off;
+};
+
+__host__ __device__ inline PlaneRef plane_read(const PlaneStore &s, uint32_t p, uint64_t plane)
+{
+    const uint32_t h = p - (uint32_t)s.head_lo; // wraps for p < head_lo
+    if (s.head != nullptr && h < s.head_n)
+        return PlaneRef{s.head, s.head_stride, (uint64_t)h * plane};
+    return PlaneRef{s.ring, s.ring_stride, (uint64_t)(p % s.ring_slots) * plane};
+}
+
+// Plane p of store s as a writer stores it: always its ring slot, and also
+// the head copy when p is a head plane (plane_write_head; buf = nullptr
+// otherwise).
+__host__ __device__ inline PlaneRef plane_write(const PlaneStore &s, uint32_t p, uint64_t plane)
+{
+    return PlaneRef{s.ring, s.ring_stride, (uint64_t)(p % s.ring_slots) * plane};
+}
+
+__host__ __device__ inline PlaneRef plane_write_head(const PlaneStore &s, uint32_t p, uint64_t plane)
+{
+    const uint32_t h = p - (uint32_t)s.head_lo;
+    if (s.head != nullptr && h < s.head_n)
+        return PlaneRef{s.head, s.head_stride, (uint64_t)h * plane};
+    return PlaneRef{nullptr, 0, 0};
+}
+
+// One launch of the blocked sweep: one time step (fused macrovar, mode 0) of
+// planes z0, z0+1, ..., z0+nz-1 (taken modulo lz), gathering from `src` and
+// storing into `dst` (non-temporal stores when nt).
+struct BlockArgs {
+    Geom g;
+    FastDiv dplane; // division by lx*ly
+    int32_t z0;
+    uint32_t nz;
+    PlaneStore src, dst;
+    RatesD r;
+    int nt;
+};
+
+// Plane (0 .. lz-1) of the launch's zl-th plane; z0 >= -lz.
+__host__ __device__ inline uint32_t block_plane(const BlockArgs &a, uint32_t zl)
+{
+    int32_t z = a.z0 + (int32_t)zl;
+    if (z < 0)
+        z += (int32_t)a.g.lz;
+    return (uint32_t)z % a.g.lz;
+}
+
+// Gather sources of one node (ix, iy, z) of a blocked launch: the planes
+// z + 1, z, z - 1 (index c_z + 1 holds plane z - c_z) and the rows
+// iy + 1, iy, iy - 1 (index c_y + 1 holds row iy - c_y, times lx).
+struct BlockNbr {
+    PlaneRef pl[3];
+    uint64_t row[3];
+    uint32_t ix, lx;
+};
+
+__host__ __device__ inline BlockNbr block_nbr(const BlockArgs &a, uint32_t ix, uint32_t iy, uint32_t z)
+{
+    BlockNbr b;
+    const uint32_t lz = a.g.lz, ly = a.g.ly;
+    b.pl[0] = plane_read(a.src, z + 1 == lz ? 0 : z + 1, a.g.plane);
+    b.pl[1] = plane_read(a.src, z, a.g.plane);
+    b.pl[2] = plane_read(a.src, z == 0 ? lz - 1 : z - 1, a.g.plane);
+    b.row[0] = (uint64_t)(iy + 1 == ly ? 0 : iy + 1) * a.g.lx;
+    b.row[1] = (uint64_t)iy * a.g.lx;
+    b.row[2] = (uint64_t)(iy == 0 ? ly - 1 : iy - 1) * a.g.lx;
+    b.ix = ix;
+    b.lx = a.g.lx;
+    return b;
+}
+
+// Buffer and element index that F_I of the node is gathered from: P(x - c_I)
+// in population I, or at a wall the node's own slot of population OPP[I]
+// (halfway bounce-back).
+template <int I>
+__host__ __device__ inline void block_src(const BlockNbr &b, void *&buf, uint64_t &idx)
+{
+    const bool walled = (CX[I] == 1 && b.ix == 0) || (CX[I] == -1 && b.ix + 1 == b.lx);
+    if (walled) {
+        buf = b.pl[1].buf;
+        idx = (uint64_t)OPP[I] * b.pl[1].stride + b.pl[1].off + b.row[1] + b.ix;
+    } else {
+        const PlaneRef &r = b.pl[CZ[I] + 1];
+        buf = r.buf;
+        idx = (uint64_t)I * r.stride + r.off + b.row[CY[I] + 1] + (b.ix - CX[I]);
+    }
+}
+
 template <int P>
 struct Launch {
     static hipError_t step(const StepArgs &a, int mode, hipStream_t s);
@@ -107,6 +217,8 @@ struct Launch {
     static hipError_t mass_partials(const SumArgs &a, uint32_t grid, hipStream_t s);
     static hipError_t plane_partials(const PlaneArgs &a, int src, uint32_t chunks, hipStream_t s);
     static hipError_t filtered_plane_partials(const PlaneArgs &a, uint32_t chunks, hipStream_t s);
+    // one launch of the two-step blocked sweep
+    static hipError_t step_planes(const BlockArgs &a, hipStream_t s);
 };
 
 extern template struct Launch<0>;

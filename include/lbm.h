@@ -131,7 +131,13 @@ int lbm_rhoupdat(lbm_channel *h, double *rhoerrmax);
 int lbm_avedensity(lbm_channel *h, double *rhomean);
 
 /* nsteps iterations of the main loop body, Channel-Flow/main.f90:157-161
- * (collision_MRT then macrovar), without host I/O. */
+ * (collision_MRT then macrovar), without host I/O.  Pairs of steps run as one
+ * blocked sweep when the fields come from macrovar, the force is uniform and
+ * there are no particles or closure: step t of a few z planes goes to a ring
+ * that the last-level cache holds, step t+1 follows one plane behind, so HBM
+ * carries one lattice read and write per two steps.  The results equal the
+ * single-step path's bit for bit.  The environment variable LBM_BLOCK_PLANES
+ * (read by lbm_create) sets the planes per chunk, 0 turns the sweep off. */
 int lbm_run(lbm_channel *h, int nsteps);
 
 /* Sum of all populations in FP64 (mass monitor; the sum is invariant under
@@ -166,9 +172,10 @@ int lbm_stats_read_filtered(lbm_channel *h, double ustar, double ystar, double *
 /* Blocks until all work on the handle's stream has finished. */
 int lbm_sync(lbm_channel *h);
 
-/* Measurement: runs nsteps of lbm_run, timing each step kernel with HIP
- * events on the handle's stream; *avg_ms is the mean kernel duration and
- * *total_ms the event time from first launch to last completion. */
+/* Measurement: runs nsteps of lbm_run, timing each step (or blocked pair of
+ * steps) with HIP events on the handle's stream; *avg_ms is the device time
+ * per step and *total_ms the event time from first launch to last
+ * completion. */
 int lbm_time_steps(lbm_channel *h, int nsteps, double *avg_ms, double *total_ms);
 
 /* Device bytes held by the handle (lattices, fields, scratch). */

@@ -1,0 +1,191 @@
+// Host-side check of the two-step blocked sweep (csrc/blocked.h): the launch
+// list is emulated on the CPU with the kernels' own addressing helpers
+// (kernels.h), every stored value being a tag (state, node, population).
+// Checked: every index lies inside its buffer; within one launch no location
+// is both read and written (its threads run concurrently); every gathered
+// value is the tag the single-step pull would read (P_s(x - c_i)_i, or
+// P_s(x)_opp(i) at a wall), so no ring slot is read after being overwritten;
+// every node and population of the output lattice holds its step-t+2 tag.
+// Launches nothing on a GPU.
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <set>
+#include <utility>
+#include <vector>
+
+#include "blocked.h"
+
+using namespace d3q19;
+
+struct Buf {
+    std::vector<uint64_t> v;
+};
+
+static Geom geom(uint32_t lx, uint32_t ly, uint32_t lz)
+{
+    Geom g;
+    g.lx = lx;
+    g.ly = ly;
+    g.lz = lz;
+    g.plane = lx * ly;
+    g.N = (uint64_t)lx * ly * lz;
+    g.dlx = make_fastdiv(lx);
+    g.dly = make_fastdiv(ly);
+    g.remap = 0;
+    return g;
+}
+
+struct Ctx {
+    Geom g;
+    Buf *bufs[4];
+    bool ok = true;
+    std::set<std::pair<const void *, uint64_t>> reads, writes; // locations touched by the current launch
+    std::vector<std::pair<uint64_t *, uint64_t>> pending;       // stores of the current launch
+    uint64_t tag(uint64_t s, uint64_t n, int i) const { return (s * g.N + n) * 19 + (uint64_t)i + 1; }
+    Buf *find(void *p)
+    {
+        for (Buf *b : bufs)
+            if ((void *)b->v.data() == p)
+                return b;
+        return nullptr;
+    }
+};
+
+template <int I>
+static void gather_one(Ctx &c, const BlockNbr &b, uint32_t ix, uint32_t iy, uint32_t z, int s)
+{
+    void *buf;
+    uint64_t idx;
+    block_src<I>(b, buf, idx);
+    Buf *B = c.find(buf);
+    if (!B || idx >= B->v.size()) {
+        fprintf(stderr, "dir %d node (%u,%u,%u): index %llu outside its buffer\n", I, ix, iy, z,
+                (unsigned long long)idx);
+        c.ok = false;
+        return;
+    }
+    const Geom &g = c.g;
+    uint64_t want;
+    const bool walled = (CX[I] == 1 && ix == 0) || (CX[I] == -1 && ix + 1 == g.lx);
+    if (walled) {
+        want = c.tag(s, ix + (uint64_t)g.lx * (iy + (uint64_t)g.ly * z), OPP[I]);
+    } else {
+        const uint32_t sx = ix - CX[I];
+        const uint32_t sy = (iy + g.ly - CY[I]) % g.ly, sz = (z + g.lz - CZ[I]) % g.lz;
+        want = c.tag(s, sx + (uint64_t)g.lx * (sy + (uint64_t)g.ly * sz), I);
+    }
+    c.reads.insert({buf, idx});
+    if (B->v[idx] != want) {
+        fprintf(stderr, "dir %d node (%u,%u,%u) state %d: read tag %llu, want %llu\n", I, ix, iy, z, s,
+                (unsigned long long)B->v[idx], (unsigned long long)want);
+        c.ok = false;
+    }
+}
+
+template <int... Is>
+static void gather_all(Ctx &c, const BlockNbr &b, uint32_t ix, uint32_t iy, uint32_t z, int s,
+                       std::integer_sequence<int, Is...>)
+{
+    (gather_one<Is>(c, b, ix, iy, z, s), ...);
+}
+
+static void store(Ctx &c, const PlaneRef &d, uint64_t xy, uint64_t n, int s)
+{
+    Buf *B = c.find(d.buf);
+    for (int i = 0; i < 19; i++) {
+        const uint64_t idx = d.off + xy + (uint64_t)i * d.stride;
+        if (!B || idx >= B->v.size()) {
+            fprintf(stderr, "store outside its buffer\n");
+            c.ok = false;
+            return;
+        }
+        c.writes.insert({d.buf, idx});
+        c.pending.push_back({&B->v[idx], c.tag(s, n, i)});
+    }
+}
+
+static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K)
+{
+    Ctx c;
+    c.g = geom(lx, ly, lz);
+    const Geom &g = c.g;
+    Buf in, out, ring, head;
+    in.v.resize(19 * g.N);
+    out.v.assign(19 * g.N, 0);
+    ring.v.assign(block_ring_elems(g, K), 0);
+    head.v.assign(block_head_elems(g), 0);
+    c.bufs[0] = &in;
+    c.bufs[1] = &out;
+    c.bufs[2] = &ring;
+    c.bufs[3] = &head;
+    for (uint64_t n = 0; n < g.N; n++)
+        for (int i = 0; i < 19; i++)
+            in.v[(uint64_t)i * g.N + n] = c.tag(0, n, i);
+    RatesD r = {};
+    const auto launches = block_schedule(g, r, K, in.v.data(), out.v.data(), ring.v.data(), head.v.data());
+    for (const BlockArgs &a : launches) {
+      c.reads.clear();
+      c.writes.clear();
+      c.pending.clear();
+      {
+        const int s = (a.src.ring == (void *)in.v.data()) ? 0 : 1; // A reads step t, B step t+1
+        if (a.nt != (a.dst.ring == (void *)out.v.data() ? 1 : 0)) {
+            fprintf(stderr, "non-temporal stores must go to the output lattice only\n");
+            return false;
+        }
+        for (uint32_t t = 0; t < g.plane * a.nz; t++) {
+            const uint32_t zl = fdiv(t, a.dplane);
+            const uint32_t xy = t - zl * g.plane;
+            const uint32_t iy = fdiv(xy, g.dlx);
+            const uint32_t ix = xy - iy * g.lx;
+            const uint32_t z = block_plane(a, zl);
+            const BlockNbr b = block_nbr(a, ix, iy, z);
+            gather_all(c, b, ix, iy, z, s, std::make_integer_sequence<int, 19>{});
+            if (!c.ok)
+                return false;
+            const uint64_t n = ix + (uint64_t)lx * (iy + (uint64_t)ly * z);
+            store(c, plane_write(a.dst, z, g.plane), xy, n, s + 1);
+            const PlaneRef hd = plane_write_head(a.dst, z, g.plane);
+            if (hd.buf != nullptr)
+                store(c, hd, xy, n, s + 1);
+            if (!c.ok)
+                return false;
+        }
+      }
+      for (const auto &w : c.writes)
+          if (c.reads.count(w)) {
+              fprintf(stderr, "%ux%ux%u K=%u: a launch reads and writes the same location\n", lx, ly, lz, K);
+              return false;
+          }
+      if (c.writes.size() != c.pending.size()) {
+          fprintf(stderr, "%ux%ux%u K=%u: a launch writes a location twice\n", lx, ly, lz, K);
+          return false;
+      }
+      for (const auto &w : c.pending)
+          *w.first = w.second;
+    }
+    for (uint64_t n = 0; n < g.N; n++)
+        for (int i = 0; i < 19; i++)
+            if (out.v[(uint64_t)i * g.N + n] != c.tag(2, n, i)) {
+                fprintf(stderr, "%ux%ux%u K=%u: output node %llu population %d not step t+2\n", lx, ly, lz, K,
+                        (unsigned long long)n, i);
+                return false;
+            }
+    printf("ok %ux%ux%u K=%u (%zu launches)\n", lx, ly, lz, K, launches.size());
+    return true;
+}
+
+int main()
+{
+    const uint32_t cases[][4] = {{4, 3, 1, 2},  {4, 3, 2, 2},  {5, 4, 3, 2},  {6, 5, 7, 2},   {5, 3, 9, 4},
+                                 {4, 4, 8, 4},  {3, 5, 10, 3}, {8, 6, 16, 1}, {8, 6, 17, 16}, {7, 5, 33, 8},
+                                 {2, 1, 5, 2},  {6, 2, 4, 1},  {9, 7, 12, 5}, {4, 3, 3, 16}};
+    bool ok = true;
+    for (auto &k : cases)
+        ok = check(k[0], k[1], k[2], k[3]) && ok;
+    if (!ok)
+        return 1;
+    printf("ok blocked sweep\n");
+    return 0;
+}

@@ -1,0 +1,78 @@
+"""The two-step blocked sweep of lbm_run (csrc/blocked.h) against single
+steps, through the C ABI.  The blocked sweep changes only where the state
+between two steps lives (a ring of planes instead of the second lattice), so
+FP64 results must be bitwise equal to single steps and to the oracle; the
+schedule itself is checked on the host by tests/host/block_check.hip."""
+import numpy as np
+import pytest
+
+import d3q19
+from test_gpu_parity import assert_bitwise, bits, seeded_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(monkeypatch, lx, ly, lz, prec, planes, seed):
+    """Two channels on the same injected start: blocked (planes per chunk) and single steps."""
+    ref, _, f0 = seeded_pair(lx, ly, lz, seed=seed, noise=1e-4, prec="fp64")
+    chans = []
+    for k in (planes, 0):
+        monkeypatch.setenv("LBM_BLOCK_PLANES", str(k))
+        ch = d3q19.Channel(lx, ly, lz, prec)
+        ch.upload_f(f0)
+        ch.FORCING()
+        ch.macrovar()
+        chans.append(ch)
+    monkeypatch.delenv("LBM_BLOCK_PLANES")
+    return ref, chans
+
+
+@pytest.mark.parametrize("shape,planes", [
+    ((8, 6, 5), 2), ((12, 10, 9), 4), ((16, 12, 17), 16), ((6, 5, 1), 2), ((6, 5, 2), 2),
+    ((10, 8, 7), 1), ((9, 7, 12), 5), ((32, 64, 32), 4),
+])
+def test_blocked_sweep_fp64_is_bitwise_single_steps_and_oracle(monkeypatch, shape, planes):
+    lx, ly, lz = shape
+    print(f"blocked {shape} planes={planes}", flush=True)
+    ref, (blk, one) = _pair(monkeypatch, lx, ly, lz, "fp64", planes, seed=51)
+    with blk, one:
+        ref.macrovar()
+        blk.run(21)  # ten blocked pairs and one single step
+        one.run(21)
+        ref.run(21)
+        fb = blk.download_f()
+        assert_bitwise(fb, one.download_f(), f"blocked vs single steps {shape}")
+        assert_bitwise(fb, ref.f, f"blocked vs oracle {shape}")
+        for name, a, b in zip(("rho", "ux", "uy", "uz"), blk.macrovar(download=True), one.macrovar(download=True)):
+            assert_bitwise(a, b, name)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_blocked_sweep_reduced_precision_matches_single_steps(monkeypatch, prec):
+    lx, ly, lz = 24, 20, 18
+    _, (blk, one) = _pair(monkeypatch, lx, ly, lz, prec, 4, seed=52)
+    with blk, one:
+        blk.run(40)
+        one.run(40)
+        a, b = blk.download_post(), one.download_post()
+    # same arithmetic; the FP32 unit may contract the two kernels' FMAs differently
+    same = np.array_equal(bits(a), bits(b))
+    assert same or np.max(np.abs(a - b)) < 1e-6 * np.max(np.abs(b))
+
+
+def test_blocked_sweep_then_other_main_loop_calls(monkeypatch):
+    """After blocked pairs every other call sees the state single steps leave:
+    collision without macrovar, avedensity (fused offset), statistics."""
+    lx, ly, lz = 16, 12, 10
+    ref, (blk, one) = _pair(monkeypatch, lx, ly, lz, "fp64", 3, seed=53)
+    with blk, one:
+        for c in (blk, one):
+            c.run(6)
+            c.collision_MRT()
+            c.collision_MRT()
+            c.macrovar()
+            c.avedensity()
+            c.run(4)
+        assert_bitwise(blk.download_f(), one.download_f(), "f")
+        np.testing.assert_array_equal(blk.statistc2(), one.statistc2())
+        assert blk.mass() == one.mass()

@@ -40,3 +40,20 @@ def test_particle_boxes_stay_inside_the_lattice():
         r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stderr + r.stdout
         assert "ok boxes" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
+                    reason="hipcc not available")
+def test_blocked_two_step_sweep_schedule():
+    """tests/host/block_check.hip: the launch list of the two-step blocked
+    sweep, emulated with the kernels' addressing on tagged values, gathers
+    exactly what the single-step pull gathers and writes every output slot."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "block_check")
+        subprocess.run([hipcc, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "d3q19-single-phase_amd", "csrc"),
+                        os.path.join(ROOT, "tests", "host", "block_check.hip"), "-o", exe],
+                       check=True, capture_output=True)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr + r.stdout
+        assert "ok blocked sweep" in r.stdout
