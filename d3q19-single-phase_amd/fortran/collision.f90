@@ -10,6 +10,7 @@
 !   avedensity     -> lbm_avedensity      (collision.f90:487-513)
 !   FORCING        -> lbm_set_force       (collision.f90:515-527)
 !   FORCINGP       -> lbm_forcingp        (collision.f90:529-602)
+!   run_steps(n)   -> lbm_run             (n x collision_MRT + macrovar)
 ! macrovar leaves the fields on the device; fetch_fields copies them to the
 ! host arrays when an I/O routine needs them.
 !=============================================================================
@@ -24,6 +25,15 @@
       implicit none
       call lbm_check(lbm_macrovar(lbm, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), 'macrovar')
       end subroutine macrovar
+
+      ! n iterations of collision_MRT followed by macrovar (main.f90:157-161)
+      ! in one library call.
+      subroutine run_steps(n)
+      use channel_vars
+      implicit none
+      integer, intent(in) :: n
+      call lbm_check(lbm_run(lbm, int(n, c_int)), 'run_steps')
+      end subroutine run_steps
 
       ! macrovar with the fields copied into the host arrays rho, ux, uy, uz.
       subroutine fetch_fields

@@ -10,7 +10,7 @@
       use channel_vars
       implicit none
       character(len=300) :: nml
-      integer :: u, ios
+      integer :: u, ios, nrun
       integer(8) :: t0, t1, rate
       real(8) :: time_max, mass
 
@@ -63,22 +63,36 @@
       call lbm_check(lbm_sync(lbm), 'sync')
       call system_clock(t0, rate)
 
-      ! main.f90:142-208
-      do istep = istep0 + 1, istep0 + nsteps
-        ! perturbation forcing until npforcing (the block at main.f90:148-153)
-        if (istep < npforcing) then
-          call FORCINGP
-        else if (istep == npforcing) then
-          call FORCING
+      ! main.f90:142-208.  Steps with nothing between them but collision_MRT
+      ! and macrovar run as one lbm_run call up to the next diag/statistc2
+      ! step (the library runs pairs of them as one blocked sweep; the
+      ! results are those of single steps).
+      istep = istep0 + 1
+      do while (istep <= istep0 + nsteps)
+        if (istep <= npforcing) then
+          ! perturbation forcing until npforcing (the block at main.f90:148-153)
+          if (istep < npforcing) then
+            call FORCINGP
+          else
+            call FORCING
+          end if
+          call collision_MRT
+          call macrovar
+          nrun = 1
+        else
+          nrun = istep0 + nsteps - istep + 1
+          if (ndiag > 0) nrun = min(nrun, ndiag - mod(istep - 1, ndiag))
+          if (nstat > 0) nrun = min(nrun, nstat - mod(istep - 1, nstat))
+          call run_steps(nrun)
         end if
-        call collision_MRT
-        call macrovar
+        istep = istep + nrun - 1
         if (ndiag > 0) then
           if (mod(istep, ndiag) == 0) call diag
         end if
         if (nstat > 0) then
           if (mod(istep, nstat) == 0) call statistc2
         end if
+        istep = istep + 1
       end do
       istep = istep0 + nsteps
 

@@ -126,3 +126,26 @@ def test_driver_perturbation_forcing_bitwise(tmp_path):
     ref.run_forcingp(0, nsteps, npforcing)
     _, f = fortio.read_cntd(str(tmp_path / "cntdflow" / f"endrunflow2D16x8.{nsteps:07d}.000"), ref.f.shape)
     assert np.array_equal(f.view(np.uint64), ref.f.view(np.uint64))
+
+
+@pytest.mark.gpu
+def test_driver_batched_steps_between_outputs_bitwise(tmp_path):
+    """The driver runs the steps between diag outputs as one lbm_run call
+    (blocked pairs plus odd single steps); the result is the oracle's."""
+    lx, ly, lz, nsteps = 12, 10, 9, 37
+    for d in ("initflow", "cntdflow", "diag", "stat"):
+        (tmp_path / d).mkdir()
+    ref = O.OracleChannel(lx, ly, lz)
+    ref.ux[:], ref.uy[:], ref.uz[:] = workloads.laminar_parabola_noise(lx, ly, lz, seed=23)
+    ref.initpop()
+    fortio.write_finit(str(tmp_path / "initflow" / "finit.000"), ref.f)
+    _nml(tmp_path / "run.nml", lx=lx, ly=ly, lz=lz, nsteps=nsteps, newinitflow=".false.", ndiag=10, nstat=15,
+         dirgenr=f"'{tmp_path}/'")
+    _run(tmp_path / "run.nml", tmp_path)
+    ref.FORCING()
+    ref.macrovar()
+    ref.run(nsteps)
+    _, f = fortio.read_cntd(str(tmp_path / "cntdflow" / f"endrunflow2D16x8.{nsteps:07d}.000"), ref.f.shape)
+    assert np.array_equal(f.view(np.uint64), ref.f.view(np.uint64))
+    diag = (tmp_path / "diag" / "diag.dat").read_text().split("\n")
+    assert len([l for l in diag if l.strip()]) == 3
