@@ -137,9 +137,9 @@ __device__ __forceinline__ uint32_t node_index(const Geom &g)
 // One time step: gather F_k, macroscopic fields, MRT collision, store P_{k+1}.
 // MODE 0: rho, u from F_k (macrovar fused); MODE 1: same with rho - rho_off
 // (avedensity's correction); MODE 2: rho, u read from the field arrays.
-// MODE + 4: nodes inside a particle are skipped (not collided, not stored);
-// fluid nodes pull the interpolated bounce-back values written into the
-// particle nodes' slots before the step.
+// MODE + 4: nodes inside a particle are not stored (their slots keep the
+// interpolated bounce-back values written before the step, which the fluid
+// nodes pull).
 // MODE + 8: the learned closure (mlp.h) sets each node's shear rates
 // (FP32 compute only).
 // MODE + 16: the body force varies over the nodes (FORCINGP), or the fused
@@ -162,13 +162,10 @@ __device__ __forceinline__ void step_node(const StepArgs &a)
             return;
     }
     const uint32_t n = active ? n0 : (uint32_t)(a.g.N - 1);
-    if constexpr ((MODE & 4) != 0) {
-        if (a.solid[n] >= 0) {
-            if constexpr (!WAVE)
-                return;
-            active = false;
-        }
-    }
+    // a node inside a particle is computed like the others (its gather
+    // issues with the flag load instead of waiting for it) but not stored
+    if constexpr ((MODE & 4) != 0)
+        active = active && a.solid[n] < 0;
     __shared__ __attribute__((aligned(16))) _Float16 wl[WAVE ? MLP_LDS : 8];
     if constexpr (WAVE) {
         mlp_stage(a.mlp, wl, threadIdx.x, BLOCK);
@@ -262,7 +259,7 @@ __device__ __forceinline__ void step_node(const StepArgs &a)
             mrt_collide_delta(f, rho9, ux9, uy9, uz9, drho, djx, djy, djz, r);
         }
     }
-    if constexpr (WAVE) {
+    if constexpr (WAVE || (MODE & 4) != 0) {
         if (!active)
             return;
     }
