@@ -143,11 +143,90 @@ void run(const char *name, Dims d, const int *Ks, int nK)
     CK(hipFree(L1));
 }
 
+// Three steps per pass: two rings (step t and step t+1 planes), launches
+// A_i (lattice -> ring 1), B_i (ring 1 -> ring 2, one plane behind), C_i
+// (ring 2 -> lattice, two planes behind); ring stores cached.
+template <typename S>
+void run3(const char *name, Dims d, const int *Ks, int nK)
+{
+    const uint64_t N = d.plane * d.lz;
+    S *L0, *L1;
+    CK(hipMalloc(&L0, N * 19 * sizeof(S)));
+    CK(hipMalloc(&L1, N * 19 * sizeof(S)));
+    CK(hipMemset(L0, 0, N * 19 * sizeof(S)));
+    CK(hipMemset(L1, 0, N * 19 * sizeof(S)));
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    const int reps = 6;
+    auto grid = [&](int nz) { return dim3((unsigned)((d.plane * (uint64_t)nz + 255) / 256)); };
+    for (int w = 0; w < 3; w++)
+        hipLaunchKernelGGL((pull_planes<S, 1>), grid(d.lz), dim3(256), 0, 0, L0, N, d.lz, L1, N, d.lz, d, 0, d.lz);
+    CK(hipDeviceSynchronize());
+    CK(hipEventRecord(e0));
+    for (int r = 0; r < reps; r++)
+        for (int q = 0; q < 3; q++)
+            hipLaunchKernelGGL((pull_planes<S, 1>), grid(d.lz), dim3(256), 0, 0, (q & 1) ? L1 : L0, N, d.lz,
+                               (q & 1) ? L0 : L1, N, d.lz, d, 0, d.lz);
+    CK(hipEventRecord(e1));
+    CK(hipEventSynchronize(e1));
+    float ms;
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    const double ref = ms / reps;
+    printf("%s three full steps         %8.3f ms\n", name, ref);
+    fflush(stdout);
+    for (int k = 0; k < nK; k++) {
+        const int K = Ks[k], slots = K + 2;
+        S *R1, *R2;
+        CK(hipMalloc(&R1, d.plane * slots * 19 * sizeof(S)));
+        CK(hipMalloc(&R2, d.plane * slots * 19 * sizeof(S)));
+        CK(hipMemset(R1, 0, d.plane * slots * 19 * sizeof(S)));
+        CK(hipMemset(R2, 0, d.plane * slots * 19 * sizeof(S)));
+        const uint64_t rs = d.plane * slots;
+        auto pass = [&]() {
+            for (int z0 = 0; z0 < d.lz; z0 += K) {
+                const int nz = (z0 + K <= d.lz) ? K : d.lz - z0;
+                hipLaunchKernelGGL((pull_planes<S, 0>), grid(nz), dim3(256), 0, 0, L0, N, d.lz, R1, rs, slots, d,
+                                   z0, nz);
+                hipLaunchKernelGGL((pull_planes<S, 0>), grid(nz), dim3(256), 0, 0, R1, rs, slots, R2, rs, slots, d,
+                                   z0 - 1, nz);
+                hipLaunchKernelGGL((pull_planes<S, 1>), grid(nz), dim3(256), 0, 0, R2, rs, slots, L1, N, d.lz, d,
+                                   z0 - 2, nz);
+            }
+        };
+        pass();
+        CK(hipGetLastError());
+        CK(hipDeviceSynchronize());
+        CK(hipEventRecord(e0));
+        for (int r = 0; r < reps; r++)
+            pass();
+        CK(hipEventRecord(e1));
+        CK(hipEventSynchronize(e1));
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        const double t = ms / reps;
+        printf("%s 3-step K=%-3d rings %6.1f MB %8.3f ms  speed-up %.2f\n", name, K,
+               2.0 * d.plane * slots * 19.0 * sizeof(S) / 1e6, t, ref / t);
+        fflush(stdout);
+        CK(hipFree(R1));
+        CK(hipFree(R2));
+    }
+    CK(hipFree(L0));
+    CK(hipFree(L1));
+}
+
 int main()
 {
     Dims d{192, 768, 384, 192ull * 768};
     const int K64[] = {1, 2, 4, 8};
     const int K32[] = {1, 2, 4, 8, 16};
+    const char *mode = getenv("TB_MODE");
+    if (mode && mode[0] == '3') {
+        const int K3a[] = {1, 2, 3};
+        const int K3b[] = {2, 4, 6, 8};
+        run3<double>("fp64", d, K3a, 3);
+        run3<float>("fp32", d, K3b, 4);
+        return 0;
+    }
     run<double>("fp64", d, K64, 4);
     run<float>("fp32", d, K32, 5);
     return 0;
