@@ -18,6 +18,10 @@ kernel's cost).
 (1.07 G); `value` is FP32 MLUPS there (FP64 lattices would need 326 GB) and
 FP16 storage is the variant.
 
+`value` runs lbm_run's two-step blocked sweep (pairs of steps through a
+ring of z planes held in the Infinity Cache, bitwise equal to single
+steps); "single_step" is the same workload with the sweep off.
+
 "forcingp": the same step under the perturbation forcing FORCINGP
 (Channel-Flow/collision.f90:529-602), its force evaluated per node inside
 the step, timed by wall clock around the host loop.
@@ -85,10 +89,23 @@ def init_channel(ch, lx, ly, lz):
     return ux, uy, uz
 
 
-def measure(prec, workload, steps, warmup):
+def measure(prec, workload, steps, warmup, single_steps=False):
+    """Main-loop steps on the workload; single_steps=True turns lbm_run's
+    two-step blocked sweep off (LBM_BLOCK_PLANES=0 while the handle is made)."""
     lx, ly, lz, _ = WORKLOADS[workload]
     n = lx * ly * lz
-    with d3q19.Channel(lx, ly, lz, prec, rstar=RSTAR.get(workload)) as ch:
+    saved = os.environ.get("LBM_BLOCK_PLANES")
+    if single_steps:
+        os.environ["LBM_BLOCK_PLANES"] = "0"
+    try:
+        ch = d3q19.Channel(lx, ly, lz, prec, rstar=RSTAR.get(workload))
+    finally:
+        if single_steps:
+            if saved is None:
+                os.environ.pop("LBM_BLOCK_PLANES", None)
+            else:
+                os.environ["LBM_BLOCK_PLANES"] = saved
+    with ch:
         init_channel(ch, lx, ly, lz)
         m0 = ch.mass()
         ch.run(warmup)
@@ -325,6 +342,8 @@ def main():
     main_m = measure(main_prec, args.workload, args.steps, args.warmup)
     log(f"bench: {main_prec} {main_m['mlups']:.1f} MLUPS, kernel {main_m['kernel_ms']:.3f} ms, "
         f"{main_m['roofline']['achieved']:.0f} GB/s")
+    single = measure(main_prec, args.workload, args.steps, args.warmup, single_steps=True)
+    log(f"bench: {main_prec} single steps {single['mlups']:.1f} MLUPS, {single['roofline']['achieved']:.0f} GB/s")
     variants = {}
     for prec in [v for v in args.variants.split(",") if v and v != main_prec]:
         log(f"bench: variant {prec}")
@@ -393,6 +412,8 @@ def main():
                    "mass_drift_per_node": main_m["mass_drift_per_node"]},
         "roofline": main_m["roofline"],
         "cpu_baseline": cpu,
+        "single_step": {"value": round(single["mlups"], 1), "unit": "MLUPS", "kernel_ms": round(single["kernel_ms"], 4),
+                        "roofline_frac": single["roofline"]["frac"]},
         "variants": variants,
         "closure_coarse": closure,
         "forcingp": forcingp,
