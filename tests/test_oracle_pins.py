@@ -302,3 +302,65 @@ def test_poiseuille_startup_matches_reference_run():
     assert round(e[100], 5) == pytest.approx(9.95e-3, abs=5e-6)
     assert round(e[300], 5) == pytest.approx(3.01e-3, abs=5e-6)
     assert round(e[1000], 6) == pytest.approx(6.73e-4, abs=5e-7)
+
+
+def _trig_fields(lx, ly, lz, seed):
+    """Fields whose plane moments are known in closed form: with
+    t_j = 2 pi j / ly and p_k = 2 pi k / lz, the plane means of cos t_j,
+    sin p_k and their products vanish and the means of cos^2, sin^2 are 1/2."""
+    rng = np.random.default_rng(seed)
+    A, B, C, D, E, F, R, G = (rng.uniform(-1e-2, 1e-2, lx) for _ in range(8))
+    t = 2 * np.pi * np.arange(ly) / ly
+    p = 2 * np.pi * np.arange(lz) / lz
+    ct = np.cos(t)[None, :, None]
+    sp = np.sin(p)[:, None, None]
+    shape = (lz, ly, lx)
+    ux = np.broadcast_to(A + B * ct, shape).copy()
+    uy = np.broadcast_to(C + D * sp, shape).copy()
+    uz = np.broadcast_to(E + F * ct, shape).copy()
+    rho = np.broadcast_to(R + G * ct, shape).copy()
+    return (A, B, C, D, E, F, R, G), (rho, ux, uy, uz)
+
+
+def test_statistc2_restatement_matches_closed_forms():
+    """saveload.f90:1382-1470: plane means, variances, covariances, pressure
+    moments and the y+ column, on fields whose moments are known exactly."""
+    lx, ly, lz = 10, 6, 8
+    o = O.OracleChannel(lx, ly, lz)
+    (A, B, C, D, E, F, R, G), (rho, ux, uy, uz) = _trig_fields(lx, ly, lz, 5)
+    o.rho[:], o.ux[:], o.uy[:], o.uz[:] = rho, ux, uy, uz
+    out = o.statistc2()
+    us, ys = o.p.ustar, o.p.ystar
+    i = np.arange(1, lx + 1)
+    yplus = np.where(i > lx // 2, (lx - i + 0.5) / ys, (i - 0.5) / ys)
+    want = np.stack([i - 0.5, yplus, A / us, C / us, E / us, B * B / 2 / us**2, D * D / 2 / us**2,
+                     F * F / 2 / us**2, B * F / 2 / us**2, 0 * A, 0 * A, R / us**2, G * G / 2 / us**4,
+                     np.ones(lx), np.zeros(lx)], axis=1)
+    scale = np.abs(want).max(axis=0)
+    scale[[9, 10]] = np.abs(want[:, 5:9]).max()  # vanishing covariances: the second moments' size
+    scale[14] = 1.0
+    assert np.all(np.abs(out - want) <= 1e-11 * scale), np.abs(out - want).max(axis=0) / scale
+
+
+def test_diag_restatement_matches_closed_forms():
+    """saveload.f90:1535-1660: volume means and rms of the velocity, the
+    largest speed and its node, and the density extrema."""
+    lx, ly, lz = 10, 6, 8
+    o = O.OracleChannel(lx, ly, lz)
+    (A, B, C, D, E, F, R, G), (rho, ux, uy, uz) = _trig_fields(lx, ly, lz, 6)
+    ux[3, 2, 7] += 0.5  # one fast node, well above every other speed
+    o.rho[:], o.ux[:], o.uy[:], o.uz[:] = rho, ux, uy, uz
+    out = o.diag()
+    us = o.p.ustar
+    n = lx * ly * lz
+    um = (ux.sum()) / n
+    ur = np.sqrt((ux**2).sum() / n - um**2)
+    vm, wm = C.mean(), E.mean()
+    vr = np.sqrt((C**2 + D**2 / 2).mean() - vm**2)
+    wr = np.sqrt((E**2 + F**2 / 2).mean() - wm**2)
+    speed = np.sqrt(ux**2 + uy**2 + uz**2)
+    assert out[0] == pytest.approx(speed[3, 2, 7], rel=1e-15)
+    assert tuple(out[1:4]) == (8, 3, 4)  # 1-based (i, j, k) of node (ix=7, iy=2, iz=3)
+    np.testing.assert_allclose(out[4:10], [um / us, vm / us, wm / us, ur / us, vr / us, wr / us], rtol=1e-10)
+    assert out[10] == 0.0
+    assert (out[11], out[12]) == (rho.max(), rho.min())
