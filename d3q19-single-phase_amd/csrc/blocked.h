@@ -1,22 +1,23 @@
 // Two time steps per pass over the lattice (host side): the launch list of
 // one blocked sweep.
 //
-// Step t of a chunk of K z planes is written to a ring of plane slots small
-// enough for the 256 MB last-level cache to keep; step t+1 of the planes one
-// behind reads that ring and writes the output lattice.  HBM then carries
-// one lattice read and one lattice write per two steps, plus what the cache
-// spills.  Per chunk i (planes [iK, iK + nz)):
-//   A_i  step t   of planes [iK, iK + nz)                   lattice -> ring
-//   B_i  step t+1 of planes [max(iK - 1, 1), iK + nz - 1)   ring -> lattice
+// The lattice is swept slab by slab along z or y (kernels.h: PlaneStore).
+// Step t of a chunk of K slabs is written to a ring small enough for the
+// 256 MB last-level cache to keep; step t+1 of the slabs one behind reads
+// that ring and writes the output lattice.  HBM then carries one lattice
+// read and one lattice write per two steps, plus what the cache spills.  Per
+// chunk i (slabs [iK, iK + n)):
+//   A_i  step t   of slabs [iK, iK + n)                    lattice -> ring
+//   B_i  step t+1 of slabs [max(iK - 1, 1), iK + n - 1)    ring -> lattice
 // The launches run in the order A_0, B_0, A_1, B_1, ...  B_i reads step t of
-// planes iK - 2 .. iK + nz - 1, the last K + 2 planes written, so a ring of
-// K + 2 slots (plane p in slot p mod (K + 2)) has overwritten none of them.
-// Planes lz - 1 and 0 of step t+1 need step t of planes lz - 2, lz - 1, 0
-// and 1, which exist together only at the end: the A launches also copy
-// planes 0 and 1 into a head buffer, and one last launch computes those two
-// planes.  Periodic wraps in z are the plane indices taken modulo lz; in y
-// and x the addressing is the single-step kernel's.  (Running A_i and
-// B_{i-1} as one launch over a ring of 2K + 2 slots measured slower.)
+// slabs iK - 2 .. iK + n - 1, the last K + 2 slabs written, so a ring of
+// K + 2 slots (slab s in slot s mod (K + 2)) has overwritten none of them.
+// Slabs ns - 1 and 0 of step t+1 need slabs of step t from both ends, which
+// exist together only at the end: the A launches also copy slabs 0 and 1
+// into a head buffer, and one last launch computes those two slabs.  The
+// periodic wraps along the sweep axis are the slab indices taken modulo ns;
+// along the rows they wrap within a slab.  (Running A_i and B_{i-1} as one
+// launch over a ring of 2K + 2 slots measured slower.)
 #pragma once
 
 #include <stdint.h>
@@ -28,46 +29,67 @@
 
 namespace d3q19 {
 
-// Ring slots and buffer sizes in elements.
-inline uint32_t block_ring_slots(uint32_t K) { return K + 2; }
-inline uint64_t block_ring_elems(const Geom &g, uint32_t K) { return 19ull * block_ring_slots(K) * g.plane; }
-inline uint64_t block_head_elems(const Geom &g) { return 19ull * 2 * g.plane; }
+// Slab geometry of a sweep along axis 2 (z) or 1 (y).
+struct BlockGeom {
+    uint32_t ns, nr;                      // slabs, rows per slab
+    uint64_t lat_slab_stride, lat_row_stride; // in the lattice (x fastest, then y, then z)
+};
 
-inline std::vector<BlockArgs> block_schedule(const Geom &g, const RatesD &r, uint32_t K, void *lat_in,
+inline BlockGeom block_geom(const Geom &g, int axis)
+{
+    if (axis == 2)
+        return BlockGeom{g.lz, g.ly, (uint64_t)g.lx * g.ly, g.lx};
+    return BlockGeom{g.ly, g.lz, g.lx, (uint64_t)g.lx * g.ly};
+}
+
+// Ring slots and buffer sizes in elements (slabs of lx * nr nodes).
+inline uint32_t block_ring_slots(uint32_t K) { return K + 2; }
+inline uint64_t block_ring_elems(const Geom &g, int axis, uint32_t K)
+{
+    return 19ull * block_ring_slots(K) * g.lx * block_geom(g, axis).nr;
+}
+inline uint64_t block_head_elems(const Geom &g, int axis) { return 19ull * 2 * g.lx * block_geom(g, axis).nr; }
+
+inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const RatesD &r, uint32_t K, void *lat_in,
                                              void *lat_out, void *ring, void *head)
 {
-    const uint32_t lz = g.lz, slots = block_ring_slots(K);
-    const PlaneStore full_in = {lat_in, g.N, lz, nullptr, 0, 0, 0};
-    const PlaneStore full_out = {lat_out, g.N, lz, nullptr, 0, 0, 0};
-    const PlaneStore ring_only = {ring, (uint64_t)slots * g.plane, slots, nullptr, 0, 0, 0};
+    const BlockGeom bg = block_geom(g, axis);
+    const uint32_t ns = bg.ns, slots = block_ring_slots(K);
+    const uint64_t slab = (uint64_t)g.lx * bg.nr;
+    const PlaneStore full_in = {lat_in, g.N, ns, bg.lat_slab_stride, bg.lat_row_stride, nullptr, 0, 0, 0};
+    const PlaneStore full_out = {lat_out, g.N, ns, bg.lat_slab_stride, bg.lat_row_stride, nullptr, 0, 0, 0};
+    const PlaneStore ring_only = {ring, slots * slab, slots, slab, g.lx, nullptr, 0, 0, 0};
     PlaneStore ring_head = ring_only;
     ring_head.head = head;
-    ring_head.head_stride = 2ull * g.plane;
+    ring_head.head_stride = 2 * slab;
     ring_head.head_lo = 0;
-    ring_head.head_n = std::min<uint32_t>(2, lz);
+    ring_head.head_n = std::min<uint32_t>(2, ns);
 
     std::vector<BlockArgs> out;
-    auto add = [&](int32_t z0, uint32_t nz, const PlaneStore &src, const PlaneStore &dst, int nt) {
+    auto add = [&](int32_t s0, uint32_t count, const PlaneStore &src, const PlaneStore &dst, int nt) {
         BlockArgs a;
         a.g = g;
-        a.dplane = make_fastdiv((uint32_t)g.plane);
+        a.ns = ns;
+        a.nr = bg.nr;
+        a.slab = (uint32_t)slab;
+        a.dslab = make_fastdiv((uint32_t)slab);
         a.r = r;
-        a.z0 = z0;
-        a.nz = nz;
+        a.s0 = s0;
+        a.count = count;
         a.src = src;
         a.dst = dst;
         a.nt = nt;
         out.push_back(a);
     };
-    for (uint32_t z0 = 0; z0 < lz; z0 += K) {
-        const uint32_t nz = std::min(K, lz - z0);
-        add((int32_t)z0, nz, full_in, ring_head, 0);
-        const uint32_t lo = std::max<uint32_t>(z0 == 0 ? 0 : z0 - 1, 1), hi = z0 + nz - 1;
+    for (uint32_t s0 = 0; s0 < ns; s0 += K) {
+        const uint32_t n = std::min(K, ns - s0);
+        add((int32_t)s0, n, full_in, ring_head, 0);
+        const uint32_t lo = std::max<uint32_t>(s0 == 0 ? 0 : s0 - 1, 1), hi = s0 + n - 1;
         if (hi > lo)
             add((int32_t)lo, hi - lo, ring_only, full_out, 1);
     }
-    // planes lz - 1 and 0 (one plane when lz == 1)
-    add((int32_t)lz - 1, std::min<uint32_t>(2, lz), ring_head, full_out, 1);
+    // slabs ns - 1 and 0 (one slab when ns == 1)
+    add((int32_t)ns - 1, std::min<uint32_t>(2, ns), ring_head, full_out, 1);
     return out;
 }
 

@@ -93,94 +93,103 @@ struct PlaneArgs {
 // Plane sums per wall-normal index: 11 moments and the fluid-node count.
 constexpr int NPLANE = 12;
 
-// Where the z planes of one lattice state live during a two-step blocked
-// sweep: plane p in slot p % ring_slots of `ring` (a whole lattice when
-// ring_slots = lz), except planes head_lo .. head_lo + head_n - 1, which a
-// reader takes from `head` (and a writer also stores there).  Strides count
-// elements between consecutive populations of a buffer.
+// Two-step blocked sweep: the lattice is swept slab by slab along one
+// periodic axis, z (AXIS 2) or y (AXIS 1).  Slab s holds the nodes with that
+// coordinate equal to s; within it a node is (ix, r), r being the other
+// periodic coordinate.
+//
+// Where the slabs of one lattice state live: slab s in slot s % ring_slots
+// of `ring`, its element (I, r, ix) at I * ring_stride + slot * slab_stride +
+// r * row_stride + ix; except slabs head_lo .. head_lo + head_n - 1, which a
+// reader takes from `head` (same slab and row strides, population stride
+// head_stride) and a writer also stores there.  A whole lattice is the store
+// with ring_slots = the number of slabs and the lattice's own strides.
 struct PlaneStore {
     void *ring;
     uint64_t ring_stride;
     uint32_t ring_slots;
+    uint64_t slab_stride, row_stride;
     void *head; // nullable
     uint64_t head_stride;
     int32_t head_lo;
     uint32_t head_n;
 };
 
-// Plane p of store s as a reader sees it: buffer, population stride and the
-// element offset of the plane.
+// Slab s of a store: buffer, population stride, element offset of the slab
+// and stride between its rows.
 struct PlaneRef {
     void *buf;
     uint64_t stride;
     uint64_t off;
+    uint64_t row_stride;
 };
 
-__host__ __device__ inline PlaneRef plane_read(const PlaneStore &s, uint32_t p, uint64_t plane)
+// Slab s as a reader sees it.
+__host__ __device__ inline PlaneRef plane_read(const PlaneStore &st, uint32_t s)
 {
-    const uint32_t h = p - (uint32_t)s.head_lo; // wraps for p < head_lo
-    if (s.head != nullptr && h < s.head_n)
-        return PlaneRef{s.head, s.head_stride, (uint64_t)h * plane};
-    return PlaneRef{s.ring, s.ring_stride, (uint64_t)(p % s.ring_slots) * plane};
+    const uint32_t h = s - (uint32_t)st.head_lo; // wraps for s < head_lo
+    if (st.head != nullptr && h < st.head_n)
+        return PlaneRef{st.head, st.head_stride, (uint64_t)h * st.slab_stride, st.row_stride};
+    return PlaneRef{st.ring, st.ring_stride, (uint64_t)(s % st.ring_slots) * st.slab_stride, st.row_stride};
 }
 
-// Plane p of store s as a writer stores it: always its ring slot, and also
-// the head copy when p is a head plane (plane_write_head; buf = nullptr
-// otherwise).
-__host__ __device__ inline PlaneRef plane_write(const PlaneStore &s, uint32_t p, uint64_t plane)
+// Slab s as a writer stores it: always its ring slot, and also the head copy
+// when s is a head slab (plane_write_head; buf = nullptr otherwise).
+__host__ __device__ inline PlaneRef plane_write(const PlaneStore &st, uint32_t s)
 {
-    return PlaneRef{s.ring, s.ring_stride, (uint64_t)(p % s.ring_slots) * plane};
+    return PlaneRef{st.ring, st.ring_stride, (uint64_t)(s % st.ring_slots) * st.slab_stride, st.row_stride};
 }
 
-__host__ __device__ inline PlaneRef plane_write_head(const PlaneStore &s, uint32_t p, uint64_t plane)
+__host__ __device__ inline PlaneRef plane_write_head(const PlaneStore &st, uint32_t s)
 {
-    const uint32_t h = p - (uint32_t)s.head_lo;
-    if (s.head != nullptr && h < s.head_n)
-        return PlaneRef{s.head, s.head_stride, (uint64_t)h * plane};
-    return PlaneRef{nullptr, 0, 0};
+    const uint32_t h = s - (uint32_t)st.head_lo;
+    if (st.head != nullptr && h < st.head_n)
+        return PlaneRef{st.head, st.head_stride, (uint64_t)h * st.slab_stride, st.row_stride};
+    return PlaneRef{nullptr, 0, 0, 0};
 }
 
 // One launch of the blocked sweep: one time step (fused macrovar, mode 0) of
-// planes z0, z0+1, ..., z0+nz-1 (taken modulo lz), gathering from `src` and
+// slabs s0, s0+1, ..., s0+count-1 (taken modulo ns), gathering from `src` and
 // storing into `dst` (non-temporal stores when nt).
 struct BlockArgs {
     Geom g;
-    FastDiv dplane; // division by lx*ly
-    int32_t z0;
-    uint32_t nz;
+    uint32_t ns, nr; // slabs along the sweep axis, rows per slab
+    uint32_t slab;   // nodes per slab, lx * nr
+    FastDiv dslab;   // division by slab
+    int32_t s0;
+    uint32_t count;
     PlaneStore src, dst;
     RatesD r;
     int nt;
 };
 
-// Plane (0 .. lz-1) of the launch's zl-th plane; z0 >= -lz.
-__host__ __device__ inline uint32_t block_plane(const BlockArgs &a, uint32_t zl)
+// Slab (0 .. ns-1) of the launch's sl-th slab; s0 >= -ns.
+__host__ __device__ inline uint32_t block_plane(const BlockArgs &a, uint32_t sl)
 {
-    int32_t z = a.z0 + (int32_t)zl;
-    if (z < 0)
-        z += (int32_t)a.g.lz;
-    return (uint32_t)z % a.g.lz;
+    int32_t s = a.s0 + (int32_t)sl;
+    if (s < 0)
+        s += (int32_t)a.ns;
+    return (uint32_t)s % a.ns;
 }
 
-// Gather sources of one node (ix, iy, z) of a blocked launch: the planes
-// z + 1, z, z - 1 (index c_z + 1 holds plane z - c_z) and the rows
-// iy + 1, iy, iy - 1 (index c_y + 1 holds row iy - c_y, times lx).
+// Gather sources of one node (ix, r, s) of a blocked launch: slabs s + 1, s,
+// s - 1 (index c + 1 holds slab s - c for the sweep-axis component c of a
+// velocity) and rows r + 1, r, r - 1 (likewise for the row-axis component).
 struct BlockNbr {
     PlaneRef pl[3];
-    uint64_t row[3];
+    uint32_t row[3];
     uint32_t ix, lx;
 };
 
-__host__ __device__ inline BlockNbr block_nbr(const BlockArgs &a, uint32_t ix, uint32_t iy, uint32_t z)
+__host__ __device__ inline BlockNbr block_nbr(const BlockArgs &a, uint32_t ix, uint32_t r, uint32_t s)
 {
     BlockNbr b;
-    const uint32_t lz = a.g.lz, ly = a.g.ly;
-    b.pl[0] = plane_read(a.src, z + 1 == lz ? 0 : z + 1, a.g.plane);
-    b.pl[1] = plane_read(a.src, z, a.g.plane);
-    b.pl[2] = plane_read(a.src, z == 0 ? lz - 1 : z - 1, a.g.plane);
-    b.row[0] = (uint64_t)(iy + 1 == ly ? 0 : iy + 1) * a.g.lx;
-    b.row[1] = (uint64_t)iy * a.g.lx;
-    b.row[2] = (uint64_t)(iy == 0 ? ly - 1 : iy - 1) * a.g.lx;
+    b.pl[0] = plane_read(a.src, s + 1 == a.ns ? 0 : s + 1);
+    b.pl[1] = plane_read(a.src, s);
+    b.pl[2] = plane_read(a.src, s == 0 ? a.ns - 1 : s - 1);
+    b.row[0] = r + 1 == a.nr ? 0 : r + 1;
+    b.row[1] = r;
+    b.row[2] = r == 0 ? a.nr - 1 : r - 1;
     b.ix = ix;
     b.lx = a.g.lx;
     return b;
@@ -189,17 +198,20 @@ __host__ __device__ inline BlockNbr block_nbr(const BlockArgs &a, uint32_t ix, u
 // Buffer and element index that F_I of the node is gathered from: P(x - c_I)
 // in population I, or at a wall the node's own slot of population OPP[I]
 // (halfway bounce-back).
-template <int I>
+template <int AXIS, int I>
 __host__ __device__ inline void block_src(const BlockNbr &b, void *&buf, uint64_t &idx)
 {
+    constexpr int cs = AXIS == 2 ? CZ[I] : CY[I]; // along the sweep axis
+    constexpr int cr = AXIS == 2 ? CY[I] : CZ[I]; // along the rows
     const bool walled = (CX[I] == 1 && b.ix == 0) || (CX[I] == -1 && b.ix + 1 == b.lx);
     if (walled) {
-        buf = b.pl[1].buf;
-        idx = (uint64_t)OPP[I] * b.pl[1].stride + b.pl[1].off + b.row[1] + b.ix;
+        const PlaneRef &p = b.pl[1];
+        buf = p.buf;
+        idx = (uint64_t)OPP[I] * p.stride + p.off + (uint64_t)b.row[1] * p.row_stride + b.ix;
     } else {
-        const PlaneRef &r = b.pl[CZ[I] + 1];
-        buf = r.buf;
-        idx = (uint64_t)I * r.stride + r.off + b.row[CY[I] + 1] + (b.ix - CX[I]);
+        const PlaneRef &p = b.pl[cs + 1];
+        buf = p.buf;
+        idx = (uint64_t)I * p.stride + p.off + (uint64_t)b.row[cr + 1] * p.row_stride + (b.ix - CX[I]);
     }
 }
 
@@ -217,8 +229,8 @@ struct Launch {
     static hipError_t mass_partials(const SumArgs &a, uint32_t grid, hipStream_t s);
     static hipError_t plane_partials(const PlaneArgs &a, int src, uint32_t chunks, hipStream_t s);
     static hipError_t filtered_plane_partials(const PlaneArgs &a, uint32_t chunks, hipStream_t s);
-    // one launch of the two-step blocked sweep
-    static hipError_t step_planes(const BlockArgs &a, hipStream_t s);
+    // one launch of the two-step blocked sweep along axis 2 (z) or 1 (y)
+    static hipError_t step_planes(const BlockArgs &a, int axis, hipStream_t s);
 };
 
 extern template struct Launch<0>;

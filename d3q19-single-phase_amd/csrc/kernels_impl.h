@@ -281,13 +281,13 @@ __global__ void __launch_bounds__(BLOCK, 4) k_step_closure(StepArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Two-step blocked sweep: one time step of a few z planes between plane
-// stores (kernels.h: PlaneStore).  The arithmetic per node is k_step<P, 0>'s,
-// so the values are the same bit for bit; only where the states live
-// differs.  NT: non-temporal stores (the whole-lattice output); otherwise
-// cached stores, so a ring of planes written here stays in the last-level
-// cache for the next launch, which reads it.
-template <int P, int... Is>
+// Two-step blocked sweep: one time step of a few slabs between slab stores
+// (kernels.h: PlaneStore).  The arithmetic per node is k_step<P, 0>'s, so the
+// values are the same bit for bit; only where the states live differs.
+// NT: non-temporal stores (the whole-lattice output); otherwise cached
+// stores, so a ring of slabs written here stays in the last-level cache for
+// the next launch, which reads it.
+template <int P, int AXIS, int... Is>
 __device__ __forceinline__ void block_gather(const BlockNbr &b, typename Prec<P>::T (&f)[19],
                                              std::integer_sequence<int, Is...>)
 {
@@ -296,75 +296,70 @@ __device__ __forceinline__ void block_gather(const BlockNbr &b, typename Prec<P>
         [&] {
             void *buf;
             uint64_t idx;
-            block_src<Is>(b, buf, idx);
+            block_src<AXIS, Is>(b, buf, idx);
             f[Is] = Prec<P>::template to_T<Is>(((const S *)buf)[idx]);
         }(),
         ...);
 }
 
 template <int P, int NT, int... Is>
-__device__ __forceinline__ void block_store(const PlaneRef &d, uint64_t xy, const typename Prec<P>::T (&f)[19],
+__device__ __forceinline__ void block_store(const PlaneRef &d, uint64_t e, const typename Prec<P>::T (&f)[19],
                                             std::integer_sequence<int, Is...>)
 {
     using S = typename Prec<P>::S;
-    S *base = (S *)d.buf + d.off + xy;
+    S *base = (S *)d.buf + e;
     if constexpr (NT)
         ((Prec<P>::template st<Is>(base + (uint64_t)Is * d.stride, f[Is])), ...);
     else
         ((base[(uint64_t)Is * d.stride] = Prec<P>::template to_S<Is>(f[Is])), ...);
 }
 
-// One node of a blocked-sweep launch (thread t of its grid).
-template <int P, int NT>
-__device__ __forceinline__ void step_planes_node(const BlockArgs &a, uint32_t t)
-{
-    using T = typename Prec<P>::T;
-    if ((uint64_t)t >= a.g.plane * a.nz)
-        return;
-    const uint32_t zl = fdiv(t, a.dplane);
-    const uint32_t xy = t - zl * a.g.plane;
-    const uint32_t iy = fdiv(xy, a.g.dlx);
-    const uint32_t ix = xy - iy * a.g.lx;
-    const uint32_t z = block_plane(a, zl);
-    const BlockNbr b = block_nbr(a, ix, iy, z);
-
-    T f[19];
-    block_gather<P>(b, f, std::make_integer_sequence<int, 19>{});
-
-    Rates<T> r;
-    r.s1 = (T)a.r.s1;
-    r.s2 = (T)a.r.s2;
-    r.s4 = (T)a.r.s4;
-    r.s9 = (T)a.r.s9;
-    r.s10 = (T)a.r.s10;
-    r.s13 = (T)a.r.s13;
-    r.s16 = (T)a.r.s16;
-    r.omegepsl = (T)a.r.omegepsl;
-    r.omegepslj = (T)a.r.omegepslj;
-    r.omegxx = (T)a.r.omegxx;
-    r.fx = (T)a.r.fx;
-    r.fy = (T)a.r.fy;
-    r.fz = (T)a.r.fz;
-    T rho9, ux9, uy9, uz9;
-    macro_sums(f, rho9, ux9, uy9, uz9);
-    ux9 = ux9 + r.fx / T(2.); // collision.f90:415-417
-    uy9 = uy9 + r.fy / T(2.);
-    uz9 = uz9 + r.fz / T(2.);
-    if constexpr (P == 0)
-        mrt_collide(f, rho9, ux9, uy9, uz9, r);
-    else
-        mrt_collide_delta(f, rho9, ux9, uy9, uz9, T(0), T(0), T(0), T(0), r);
-
-    block_store<P, NT>(plane_write(a.dst, z, a.g.plane), xy, f, std::make_integer_sequence<int, 19>{});
-    const PlaneRef hd = plane_write_head(a.dst, z, a.g.plane);
-    if (hd.buf != nullptr)
-        block_store<P, 0>(hd, xy, f, std::make_integer_sequence<int, 19>{});
-}
-
-template <int P, int NT>
+template <int P, int AXIS, int NT>
 __global__ void __launch_bounds__(BLOCK) k_step_planes(BlockArgs a)
 {
-    step_planes_node<P, NT>(a, blockIdx.x * BLOCK + threadIdx.x);
+    using T = typename Prec<P>::T;
+    const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+    if ((uint64_t)t >= (uint64_t)a.slab * a.count)
+        return;
+    const uint32_t sl = fdiv(t, a.dslab);
+    const uint32_t w = t - sl * a.slab;
+    const uint32_t r = fdiv(w, a.g.dlx);
+    const uint32_t ix = w - r * a.g.lx;
+    const uint32_t sb = block_plane(a, sl);
+    const BlockNbr b = block_nbr(a, ix, r, sb);
+
+    T f[19];
+    block_gather<P, AXIS>(b, f, std::make_integer_sequence<int, 19>{});
+
+    Rates<T> rt;
+    rt.s1 = (T)a.r.s1;
+    rt.s2 = (T)a.r.s2;
+    rt.s4 = (T)a.r.s4;
+    rt.s9 = (T)a.r.s9;
+    rt.s10 = (T)a.r.s10;
+    rt.s13 = (T)a.r.s13;
+    rt.s16 = (T)a.r.s16;
+    rt.omegepsl = (T)a.r.omegepsl;
+    rt.omegepslj = (T)a.r.omegepslj;
+    rt.omegxx = (T)a.r.omegxx;
+    rt.fx = (T)a.r.fx;
+    rt.fy = (T)a.r.fy;
+    rt.fz = (T)a.r.fz;
+    T rho9, ux9, uy9, uz9;
+    macro_sums(f, rho9, ux9, uy9, uz9);
+    ux9 = ux9 + rt.fx / T(2.); // collision.f90:415-417
+    uy9 = uy9 + rt.fy / T(2.);
+    uz9 = uz9 + rt.fz / T(2.);
+    if constexpr (P == 0)
+        mrt_collide(f, rho9, ux9, uy9, uz9, rt);
+    else
+        mrt_collide_delta(f, rho9, ux9, uy9, uz9, T(0), T(0), T(0), T(0), rt);
+
+    const PlaneRef d = plane_write(a.dst, sb);
+    block_store<P, NT>(d, d.off + (uint64_t)r * d.row_stride + ix, f, std::make_integer_sequence<int, 19>{});
+    const PlaneRef hd = plane_write_head(a.dst, sb);
+    if (hd.buf != nullptr)
+        block_store<P, 0>(hd, hd.off + (uint64_t)r * hd.row_stride + ix, f, std::make_integer_sequence<int, 19>{});
 }
 
 // Closure diagnostics for nodes [n_begin, n_begin + count) of the current
@@ -1002,13 +997,22 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
 }
 
 template <int P>
-hipError_t Launch<P>::step_planes(const BlockArgs &a, hipStream_t s)
+hipError_t Launch<P>::step_planes(const BlockArgs &a, int axis, hipStream_t s)
 {
-    const dim3 grid(nblocks<P>(a.g.plane * a.nz)), blk(BLOCK);
-    if (a.nt)
-        hipLaunchKernelGGL((k_step_planes<P, 1>), grid, blk, 0, s, a);
-    else
-        hipLaunchKernelGGL((k_step_planes<P, 0>), grid, blk, 0, s, a);
+    const dim3 grid(nblocks<P>((uint64_t)a.slab * a.count)), blk(BLOCK);
+    if (axis == 2) {
+        if (a.nt)
+            hipLaunchKernelGGL((k_step_planes<P, 2, 1>), grid, blk, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_step_planes<P, 2, 0>), grid, blk, 0, s, a);
+    } else if (axis == 1) {
+        if (a.nt)
+            hipLaunchKernelGGL((k_step_planes<P, 1, 1>), grid, blk, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_step_planes<P, 1, 0>), grid, blk, 0, s, a);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -60,6 +60,7 @@ constexpr uint64_t COPY_NODES = 1u << 20; // nodes per upload/download chunk
 // cache (the measured optimum on the 192x768x384 channel in FP64 and FP32;
 // larger rings are partly evicted by the streams through the same cache)
 constexpr double BLOCK_RING_BYTES = 160e6;
+constexpr long BLOCK_MIN_SLABS = 4;
 
 } // namespace
 
@@ -92,8 +93,9 @@ struct lbm_channel {
     float *mlp;         // learned closure parameters on the device (nullptr: plain MRT)
     float mlp_inv_ystar, mlp_yscale;
     int runs;           // step kernel: 1 = vector runs where the shape allows, 0 = one node per thread
-    uint32_t bK;        // two-step blocked sweep: planes per chunk (0 = single steps only)
-    void *bring, *bhead; // its plane ring and head buffer (allocated on first use)
+    uint32_t bK;        // two-step blocked sweep: slabs per chunk (0 = single steps only)
+    int baxis;          // its sweep axis: 2 = z (slabs lx*ly), 1 = y (slabs lx*lz)
+    void *bring, *bhead; // its slab ring and head buffer (allocated on first use)
 };
 
 struct lbm_trainer {
@@ -499,17 +501,30 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         // the 192x768x384 channel: 64.6 % vs 69.2 % of HBM in FP64)
         const char *k = getenv("LBM_STEP_KERNEL");
         h->runs = (k && strcmp(k, "runs") == 0) ? 1 : 0;
-        // two-step blocked sweep: chunks of K planes whose ring of K + 2
-        // planes stays within BLOCK_RING_BYTES; LBM_BLOCK_PLANES=K sets K (0: off).
-        // Not by default for FP16 storage, whose step is bound by arithmetic
-        // (measured 49 400 MLUPS blocked against 61 400 single-step).
-        const double plane_bytes = (double)h->g.plane * 19.0 * (double)h->elem;
-        long kb = c.precision == LBM_FP16 ? 0 : (long)(BLOCK_RING_BYTES / plane_bytes) - 2;
+        // two-step blocked sweep: chunks of K slabs whose ring of K + 2 slabs
+        // stays within BLOCK_RING_BYTES, along z, else along y, and only when
+        // a chunk holds at least BLOCK_MIN_SLABS slabs (fewer measured slower
+        // than single steps: the 2048x1024x512 channel's 40 MB y slabs allow
+        // two, 33 900 against 38 200 MLUPS).  LBM_BLOCK_AXIS=z|y sets the
+        // axis, LBM_BLOCK_PLANES=K the chunk (0: off).  Not by default for
+        // FP16 storage, whose step is bound by arithmetic (measured 49 400
+        // MLUPS blocked against 61 400 single-step).
+        auto chunk = [&](int axis) -> long {
+            const BlockGeom bg = block_geom(h->g, axis);
+            const double slab_bytes = (double)h->g.lx * bg.nr * 19.0 * (double)h->elem;
+            return (long)(BLOCK_RING_BYTES / slab_bytes) - 2;
+        };
+        const char *ka = getenv("LBM_BLOCK_AXIS");
+        h->baxis = ka ? (strcmp(ka, "y") == 0 ? 1 : 2) : (chunk(2) >= BLOCK_MIN_SLABS ? 2 : 1);
+        long kb = c.precision == LBM_FP16 ? 0 : chunk(h->baxis);
+        if (kb < BLOCK_MIN_SLABS)
+            kb = 0;
         const char *kp = getenv("LBM_BLOCK_PLANES");
         if (kp)
             kb = strtol(kp, nullptr, 10);
-        kb = std::min<long>(std::min<long>(kb, 16), (long)c.lz);
-        if (kb < 1 || (uint64_t)h->g.plane * (uint64_t)kb >= (1ull << 31))
+        const BlockGeom bg = block_geom(h->g, h->baxis);
+        kb = std::min<long>(std::min<long>(kb, 16), (long)bg.ns);
+        if (kb < 1 || (uint64_t)h->g.lx * bg.nr * (uint64_t)kb >= (1ull << 31))
             kb = 0;
         h->bK = (uint32_t)kb;
     }
@@ -885,7 +900,8 @@ static bool block_ready(const lbm_channel *h)
 static int run_pair(lbm_channel *h)
 {
     if (!h->bring) {
-        const size_t rb = block_ring_elems(h->g, h->bK) * h->elem, hb = block_head_elems(h->g) * h->elem;
+        const size_t rb = block_ring_elems(h->g, h->baxis, h->bK) * h->elem,
+                     hb = block_head_elems(h->g, h->baxis) * h->elem;
         hipError_t e = hipMalloc(&h->bring, rb);
         if (e == hipSuccess)
             e = hipMalloc(&h->bhead, hb);
@@ -898,9 +914,9 @@ static int run_pair(lbm_channel *h)
         }
     }
     const auto launches =
-        block_schedule(h->g, rates_of(h), h->bK, h->lat[h->cur], h->lat[1 - h->cur], h->bring, h->bhead);
+        block_schedule(h->g, h->baxis, rates_of(h), h->bK, h->lat[h->cur], h->lat[1 - h->cur], h->bring, h->bhead);
     for (const BlockArgs &a : launches) {
-        int rc = dispatch_err(PREC_CALL(h, step_planes, a, h->stream), "collision (blocked sweep)");
+        int rc = dispatch_err(PREC_CALL(h, step_planes, a, h->baxis, h->stream), "collision (blocked sweep)");
         if (rc)
             return rc;
     }
@@ -1118,7 +1134,7 @@ int lbm_device_bytes(lbm_channel *h, size_t *bytes)
     if (h->stage)
         b += COPY_NODES * 19 * sizeof(double);
     if (h->bring)
-        b += (block_ring_elems(h->g, h->bK) + block_head_elems(h->g)) * h->elem;
+        b += (block_ring_elems(h->g, h->baxis, h->bK) + block_head_elems(h->g, h->baxis)) * h->elem;
     b += (2 * RED_BLOCKS + (size_t)PLANE_CHUNKS * NPLANE * h->g.lx + 2 + 2 * NPLANE * (size_t)h->g.lx) * sizeof(double) + 8;
     *bytes = b;
     return LBM_OK;

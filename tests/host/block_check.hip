@@ -52,12 +52,20 @@ struct Ctx {
     }
 };
 
-template <int I>
+// Node index of (ix, r, slab s) for a sweep along axis 2 (z) or 1 (y).
+static uint64_t node(const Geom &g, int axis, uint32_t ix, uint32_t r, uint32_t sl)
+{
+    const uint32_t iy = axis == 2 ? r : sl, iz = axis == 2 ? sl : r;
+    return ix + (uint64_t)g.lx * (iy + (uint64_t)g.ly * iz);
+}
+
+template <int AXIS, int I>
 static void gather_one(Ctx &c, const BlockNbr &b, uint32_t ix, uint32_t iy, uint32_t z, int s)
 {
+    // (iy, z) here are (row, slab) of the sweep
     void *buf;
     uint64_t idx;
-    block_src<I>(b, buf, idx);
+    block_src<AXIS, I>(b, buf, idx);
     Buf *B = c.find(buf);
     if (!B || idx >= B->v.size()) {
         fprintf(stderr, "dir %d node (%u,%u,%u): index %llu outside its buffer\n", I, ix, iy, z,
@@ -66,14 +74,16 @@ static void gather_one(Ctx &c, const BlockNbr &b, uint32_t ix, uint32_t iy, uint
         return;
     }
     const Geom &g = c.g;
+    const uint32_t nr = AXIS == 2 ? g.ly : g.lz, ns = AXIS == 2 ? g.lz : g.ly;
+    const int cr = AXIS == 2 ? CY[I] : CZ[I], cs = AXIS == 2 ? CZ[I] : CY[I];
     uint64_t want;
     const bool walled = (CX[I] == 1 && ix == 0) || (CX[I] == -1 && ix + 1 == g.lx);
     if (walled) {
-        want = c.tag(s, ix + (uint64_t)g.lx * (iy + (uint64_t)g.ly * z), OPP[I]);
+        want = c.tag(s, node(g, AXIS, ix, iy, z), OPP[I]);
     } else {
         const uint32_t sx = ix - CX[I];
-        const uint32_t sy = (iy + g.ly - CY[I]) % g.ly, sz = (z + g.lz - CZ[I]) % g.lz;
-        want = c.tag(s, sx + (uint64_t)g.lx * (sy + (uint64_t)g.ly * sz), I);
+        const uint32_t sr = (iy + nr - cr) % nr, ss = (z + ns - cs) % ns;
+        want = c.tag(s, node(g, AXIS, sx, sr, ss), I);
     }
     c.reads.insert({buf, idx});
     if (B->v[idx] != want) {
@@ -83,18 +93,18 @@ static void gather_one(Ctx &c, const BlockNbr &b, uint32_t ix, uint32_t iy, uint
     }
 }
 
-template <int... Is>
+template <int AXIS, int... Is>
 static void gather_all(Ctx &c, const BlockNbr &b, uint32_t ix, uint32_t iy, uint32_t z, int s,
                        std::integer_sequence<int, Is...>)
 {
-    (gather_one<Is>(c, b, ix, iy, z, s), ...);
+    (gather_one<AXIS, Is>(c, b, ix, iy, z, s), ...);
 }
 
-static void store(Ctx &c, const PlaneRef &d, uint64_t xy, uint64_t n, int s)
+static void store(Ctx &c, const PlaneRef &d, uint32_t ix, uint32_t r, uint64_t n, int s)
 {
     Buf *B = c.find(d.buf);
     for (int i = 0; i < 19; i++) {
-        const uint64_t idx = d.off + xy + (uint64_t)i * d.stride;
+        const uint64_t idx = d.off + (uint64_t)r * d.row_stride + ix + (uint64_t)i * d.stride;
         if (!B || idx >= B->v.size()) {
             fprintf(stderr, "store outside its buffer\n");
             c.ok = false;
@@ -105,7 +115,7 @@ static void store(Ctx &c, const PlaneRef &d, uint64_t xy, uint64_t n, int s)
     }
 }
 
-static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K)
+static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
 {
     Ctx c;
     c.g = geom(lx, ly, lz);
@@ -113,8 +123,8 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K)
     Buf in, out, ring, head;
     in.v.resize(19 * g.N);
     out.v.assign(19 * g.N, 0);
-    ring.v.assign(block_ring_elems(g, K), 0);
-    head.v.assign(block_head_elems(g), 0);
+    ring.v.assign(block_ring_elems(g, axis, K), 0);
+    head.v.assign(block_head_elems(g, axis), 0);
     c.bufs[0] = &in;
     c.bufs[1] = &out;
     c.bufs[2] = &ring;
@@ -123,7 +133,7 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K)
         for (int i = 0; i < 19; i++)
             in.v[(uint64_t)i * g.N + n] = c.tag(0, n, i);
     RatesD r = {};
-    const auto launches = block_schedule(g, r, K, in.v.data(), out.v.data(), ring.v.data(), head.v.data());
+    const auto launches = block_schedule(g, axis, r, K, in.v.data(), out.v.data(), ring.v.data(), head.v.data());
     for (const BlockArgs &a : launches) {
       c.reads.clear();
       c.writes.clear();
@@ -134,21 +144,24 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K)
             fprintf(stderr, "non-temporal stores must go to the output lattice only\n");
             return false;
         }
-        for (uint32_t t = 0; t < g.plane * a.nz; t++) {
-            const uint32_t zl = fdiv(t, a.dplane);
-            const uint32_t xy = t - zl * g.plane;
-            const uint32_t iy = fdiv(xy, g.dlx);
-            const uint32_t ix = xy - iy * g.lx;
-            const uint32_t z = block_plane(a, zl);
-            const BlockNbr b = block_nbr(a, ix, iy, z);
-            gather_all(c, b, ix, iy, z, s, std::make_integer_sequence<int, 19>{});
+        for (uint32_t t = 0; t < a.slab * a.count; t++) {
+            const uint32_t sl = fdiv(t, a.dslab);
+            const uint32_t w = t - sl * a.slab;
+            const uint32_t rr = fdiv(w, g.dlx);
+            const uint32_t ix = w - rr * g.lx;
+            const uint32_t sb = block_plane(a, sl);
+            const BlockNbr b = block_nbr(a, ix, rr, sb);
+            if (axis == 2)
+                gather_all<2>(c, b, ix, rr, sb, s, std::make_integer_sequence<int, 19>{});
+            else
+                gather_all<1>(c, b, ix, rr, sb, s, std::make_integer_sequence<int, 19>{});
             if (!c.ok)
                 return false;
-            const uint64_t n = ix + (uint64_t)lx * (iy + (uint64_t)ly * z);
-            store(c, plane_write(a.dst, z, g.plane), xy, n, s + 1);
-            const PlaneRef hd = plane_write_head(a.dst, z, g.plane);
+            const uint64_t n = node(g, axis, ix, rr, sb);
+            store(c, plane_write(a.dst, sb), ix, rr, n, s + 1);
+            const PlaneRef hd = plane_write_head(a.dst, sb);
             if (hd.buf != nullptr)
-                store(c, hd, xy, n, s + 1);
+                store(c, hd, ix, rr, n, s + 1);
             if (!c.ok)
                 return false;
         }
@@ -168,11 +181,11 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K)
     for (uint64_t n = 0; n < g.N; n++)
         for (int i = 0; i < 19; i++)
             if (out.v[(uint64_t)i * g.N + n] != c.tag(2, n, i)) {
-                fprintf(stderr, "%ux%ux%u K=%u: output node %llu population %d not step t+2\n", lx, ly, lz, K,
-                        (unsigned long long)n, i);
+                fprintf(stderr, "%ux%ux%u K=%u axis %d: output node %llu population %d not step t+2\n", lx, ly,
+                        lz, K, axis, (unsigned long long)n, i);
                 return false;
             }
-    printf("ok %ux%ux%u K=%u (%zu launches)\n", lx, ly, lz, K, launches.size());
+    printf("ok %ux%ux%u K=%u axis %c (%zu launches)\n", lx, ly, lz, K, axis == 2 ? 'z' : 'y', launches.size());
     return true;
 }
 
@@ -183,7 +196,8 @@ int main()
                                  {2, 1, 5, 2},  {6, 2, 4, 1},  {9, 7, 12, 5}, {4, 3, 3, 16}};
     bool ok = true;
     for (auto &k : cases)
-        ok = check(k[0], k[1], k[2], k[3]) && ok;
+        for (int axis : {2, 1})
+            ok = check(k[0], k[1], k[2], k[3], axis) && ok;
     if (!ok)
         return 1;
     printf("ok blocked sweep\n");

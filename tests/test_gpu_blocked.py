@@ -12,10 +12,12 @@ from test_gpu_parity import assert_bitwise, bits, seeded_pair
 pytestmark = pytest.mark.gpu
 
 
-def _pair(monkeypatch, lx, ly, lz, prec, planes, seed):
-    """Two channels on the same injected start: blocked (planes per chunk) and single steps."""
+def _pair(monkeypatch, lx, ly, lz, prec, planes, seed, axis="z"):
+    """Two channels on the same injected start: blocked (slabs per chunk, sweep
+    axis) and single steps."""
     ref, _, f0 = seeded_pair(lx, ly, lz, seed=seed, noise=1e-4, prec="fp64")
     chans = []
+    monkeypatch.setenv("LBM_BLOCK_AXIS", axis)
     for k in (planes, 0):
         monkeypatch.setenv("LBM_BLOCK_PLANES", str(k))
         ch = d3q19.Channel(lx, ly, lz, prec)
@@ -24,17 +26,19 @@ def _pair(monkeypatch, lx, ly, lz, prec, planes, seed):
         ch.macrovar()
         chans.append(ch)
     monkeypatch.delenv("LBM_BLOCK_PLANES")
+    monkeypatch.delenv("LBM_BLOCK_AXIS")
     return ref, chans
 
 
+@pytest.mark.parametrize("axis", ["z", "y"])
 @pytest.mark.parametrize("shape,planes", [
     ((8, 6, 5), 2), ((12, 10, 9), 4), ((16, 12, 17), 16), ((6, 5, 1), 2), ((6, 5, 2), 2),
-    ((10, 8, 7), 1), ((9, 7, 12), 5), ((32, 64, 32), 4),
+    ((10, 8, 7), 1), ((9, 7, 12), 5), ((32, 64, 32), 4), ((6, 1, 5), 2),
 ])
-def test_blocked_sweep_fp64_is_bitwise_single_steps_and_oracle(monkeypatch, shape, planes):
+def test_blocked_sweep_fp64_is_bitwise_single_steps_and_oracle(monkeypatch, shape, planes, axis):
     lx, ly, lz = shape
-    print(f"blocked {shape} planes={planes}", flush=True)
-    ref, (blk, one) = _pair(monkeypatch, lx, ly, lz, "fp64", planes, seed=51)
+    print(f"blocked {shape} planes={planes} axis={axis}", flush=True)
+    ref, (blk, one) = _pair(monkeypatch, lx, ly, lz, "fp64", planes, seed=51, axis=axis)
     with blk, one:
         ref.macrovar()
         blk.run(21)  # ten blocked pairs and one single step
@@ -47,10 +51,11 @@ def test_blocked_sweep_fp64_is_bitwise_single_steps_and_oracle(monkeypatch, shap
             assert_bitwise(a, b, name)
 
 
+@pytest.mark.parametrize("axis", ["z", "y"])
 @pytest.mark.parametrize("prec", ["fp32", "fp16"])
-def test_blocked_sweep_reduced_precision_matches_single_steps(monkeypatch, prec):
+def test_blocked_sweep_reduced_precision_matches_single_steps(monkeypatch, prec, axis):
     lx, ly, lz = 24, 20, 18
-    _, (blk, one) = _pair(monkeypatch, lx, ly, lz, prec, 4, seed=52)
+    _, (blk, one) = _pair(monkeypatch, lx, ly, lz, prec, 4, seed=52, axis=axis)
     with blk, one:
         blk.run(40)
         one.run(40)
@@ -64,7 +69,7 @@ def test_blocked_sweep_then_other_main_loop_calls(monkeypatch):
     """After blocked pairs every other call sees the state single steps leave:
     collision without macrovar, avedensity (fused offset), statistics."""
     lx, ly, lz = 16, 12, 10
-    ref, (blk, one) = _pair(monkeypatch, lx, ly, lz, "fp64", 3, seed=53)
+    ref, (blk, one) = _pair(monkeypatch, lx, ly, lz, "fp64", 3, seed=53, axis="y")
     with blk, one:
         for c in (blk, one):
             c.run(6)
