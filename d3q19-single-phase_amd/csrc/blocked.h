@@ -72,7 +72,8 @@ inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const Rate
         a.ns = ns;
         a.nr = bg.nr;
         a.slab = (uint32_t)slab;
-        a.dslab = make_fastdiv((uint32_t)slab);
+        a.slabp = slab_padded((uint32_t)slab);
+        a.dslabp = make_fastdiv(a.slabp);
         a.r = r;
         a.s0 = s0;
         a.count = count;

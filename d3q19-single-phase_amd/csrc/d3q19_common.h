@@ -497,38 +497,74 @@ __host__ __device__ __forceinline__ void mrt_collide(T (&f)[19], T rho9, T ux9, 
         f[i] = f9[i] + T(0.5) * Fb[i];
 }
 
-// The same collision written as an increment, for FP32 arithmetic:
+// Moments of the Guo term Fbar_i = w_i (3 c.F + 9 (c.F)(c.u) - 3 u.F)
+// (guo_terms), sum_i M_ki Fbar_i in moments() order for k < 12 (the third-
+// order moments mx, my, mz of Fbar vanish; sum_i Fbar_i = 0 and
+// sum_i c_i Fbar_i = F).  Lattice sums of w_i times even powers of c_i give
+//   e 38 u.F, eps -11 u.F, q -2F/3, 3pxx 2 G, 3pixx -G, pww 2 H, piww -H,
+//   pxy ux Fy + uy Fx, pyz uy Fz + uz Fy, pxz ux Fz + uz Fx
+// with G = 2 ux Fx - uy Fy - uz Fz and H = uy Fy - uz Fz.
+template <typename T>
+__host__ __device__ __forceinline__ void guo_moments(T ux9, T uy9, T uz9, T fx9, T fy9, T fz9, T (&s)[12])
+{
+    const T gx = ux9 * fx9, gy = uy9 * fy9, gz = uz9 * fz9;
+    const T uf = gx + gy + gz;
+    const T G = T(2.0) * gx - gy - gz;
+    const T H = gy - gz;
+    constexpr T coef1 = T(-(2.0 / 3.0));
+    s[0] = T(38.0) * uf;
+    s[1] = T(-11.0) * uf;
+    s[2] = coef1 * fx9;
+    s[3] = coef1 * fy9;
+    s[4] = coef1 * fz9;
+    s[5] = T(2.0) * G;
+    s[6] = -G;
+    s[7] = T(2.0) * H;
+    s[8] = -H;
+    s[9] = ux9 * fy9 + uy9 * fx9;
+    s[10] = uy9 * fz9 + uz9 * fy9;
+    s[11] = ux9 * fz9 + uz9 * fx9;
+}
+
+// The moments a collision relaxes, for the increment form below: m = the 15
+// non-conserved moments of f9 = f + Fbar/2, formed as moments(f) + S/2 with S
+// the moments of Fbar (guo_moments), and the equilibria.
+template <typename T>
+__host__ __device__ __forceinline__ void delta_moments(const T (&f)[19], T rho9, T ux9, T uy9, T uz9, const Rates<T> &r,
+                                                       T (&m)[15], T (&sf)[12], T (&eq)[12])
+{
+    moments(f, m);
+    guo_moments(ux9, uy9, uz9, r.fx, r.fy, r.fz, sf);
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+        m[k] = m[k] + T(0.5) * sf[k];
+    equilibria(rho9, ux9, uy9, uz9, r, eq);
+}
+
+// The same collision written as an increment, in moment space, for FP32
+// arithmetic:
 //
-//     P_i = f_i + Fbar_i + [M^-1 (m* - m)]_i
+//     P_i = f_i + [M^-1 dm]_i,   dm = (drho, dj + F, S - s (m - m_eq))
 //
-// with m the moments of f9 = f + Fbar/2, m* their relaxed values and the
-// conserved moments' changes (drho, djx, djy, djz) = field value - moment of
-// f9.  This equals the reference form f9 + M^-1 (m* - m) + Fbar/2 in exact
-// arithmetic.  In FP32 the reference form rebuilds every population from
-// moments of size |u| and adds F/2 ~ 1e-7 to sums of size 0.1, which
-// rounds a few per cent of the per-step momentum input away, the same way on
-// every node of a flow that is uniform along the walls: the mean profile
-// then drifts by ~1e-4 of U per thousand steps.  Here the small terms are
-// formed and summed first and f_i is touched by one addition, so momentum
-// enters exactly up to one rounding of each stored value.  When the fields
-// are macrovar of the same f, the conserved changes vanish exactly
-// (sum_i Fbar_i = 0, sum_i c_i Fbar_i = F) and the caller passes zeros.
+// with m the moments of f9 = f + Fbar/2, S the moments of Fbar, s the rates
+// and (drho, djx, djy, djz) = field value - conserved moment of f9.  Since
+// Fbar = M^-1 (0, F, S), this equals the reference form
+// f9 + M^-1 (m* - m) + Fbar/2 in exact arithmetic, and the 19 Guo terms are
+// never formed.  In FP32 the reference form rebuilds every population from
+// moments of size |u| and adds F/2 ~ 1e-7 to sums of size 0.1, which rounds
+// a few per cent of the per-step momentum input away, the same way on every
+// node of a flow that is uniform along the walls: the mean profile then
+// drifts by ~1e-4 of U per thousand steps.  Here the small increments are
+// formed first and f_i is touched by one addition, so momentum enters
+// exactly up to one rounding of each stored value.  When the fields are
+// macrovar of the same f the conserved changes vanish and the caller passes
+// zeros.
 template <typename T, typename Closure = NoClosure>
 __host__ __device__ __forceinline__ void mrt_collide_delta(T (&f)[19], T rho9, T ux9, T uy9, T uz9, T drho, T djx, T djy,
                                                   T djz, const Rates<T> &r, const Closure &closure = Closure())
 {
-    T Fb[19];
-    guo_terms(ux9, uy9, uz9, r.fx, r.fy, r.fz, Fb);
-
-    T f9[19];
-#pragma unroll
-    for (int i = 0; i < 19; i++)
-        f9[i] = f[i] + T(0.5) * Fb[i];
-
-    T eq[12];
-    equilibria(rho9, ux9, uy9, uz9, r, eq);
-    T m[15];
-    moments(f9, m);
+    T m[15], sf[12], eq[12];
+    delta_moments(f, rho9, ux9, uy9, uz9, r, m, sf, eq);
 
     T s9 = r.s9, s13 = r.s13;
     if constexpr (!std::is_same<Closure, NoClosure>::value) {
@@ -540,16 +576,16 @@ __host__ __device__ __forceinline__ void mrt_collide_delta(T (&f)[19], T rho9, T
     T dm[15];
 #pragma unroll
     for (int k = 0; k < 12; k++)
-        dm[k] = -(rate_of(k, r, s9, s13) * (m[k] - eq[k]));
+        dm[k] = sf[k] - rate_of(k, r, s9, s13) * (m[k] - eq[k]);
 #pragma unroll
     for (int k = 12; k < 15; k++)
         dm[k] = -(r.s16 * m[k]);
 
     T d[19];
-    inverse(drho, djx, djy, djz, dm, d);
+    inverse(drho, djx + r.fx, djy + r.fy, djz + r.fz, dm, d);
 #pragma unroll
     for (int i = 0; i < 19; i++)
-        f[i] = f[i] + (Fb[i] + d[i]);
+        f[i] = f[i] + d[i];
 }
 
 } // namespace d3q19

@@ -8,13 +8,35 @@
 
 namespace d3q19 {
 
-// Relaxation rates and force as passed by the driver (FP64; the FP32 kernels
-// round them once).
+// Relaxation rates and force as passed by the driver (FP64), with their FP32
+// roundings for the kernels that compute in FP32 (set by with_f32 on the
+// host, so no kernel converts them per thread).
 struct RatesD {
     double s1, s2, s4, s9, s10, s13, s16;
     double omegepsl, omegepslj, omegxx;
     double fx, fy, fz;
+    Rates<float> f32;
 };
+
+inline RatesD with_f32(RatesD r)
+{
+    r.f32 = Rates<float>{(float)r.s1,       (float)r.s2,        (float)r.s4,     (float)r.s9, (float)r.s10,
+                         (float)r.s13,      (float)r.s16,       (float)r.omegepsl, (float)r.omegepslj,
+                         (float)r.omegxx,   (float)r.fx,        (float)r.fy,     (float)r.fz};
+    return r;
+}
+
+// The rates in the kernel's arithmetic type.
+template <typename T>
+__host__ __device__ __forceinline__ Rates<T> rates_as(const RatesD &r)
+{
+    if constexpr (std::is_same<T, float>::value) {
+        return r.f32;
+    } else {
+        return Rates<T>{r.s1, r.s2, r.s4, r.s9, r.s10, r.s13, r.s16, r.omegepsl, r.omegepslj, r.omegxx,
+                        r.fx, r.fy, r.fz};
+    }
+}
 
 // Body force at the nodes: the force_realx/y/z arrays of
 // Channel-Flow/var_inc.f90 as FORCING (collision.f90:515-527) and FORCINGP
@@ -150,18 +172,26 @@ __host__ __device__ inline PlaneRef plane_write_head(const PlaneStore &st, uint3
 
 // One launch of the blocked sweep: one time step (fused macrovar, mode 0) of
 // slabs s0, s0+1, ..., s0+count-1 (taken modulo ns), gathering from `src` and
-// storing into `dst` (non-temporal stores when nt).
+// storing into `dst` (non-temporal stores when nt).  Thread t works on node
+// t % slabp of the launch's slab t / slabp: slabs are padded to whole waves
+// (slabp = slab rounded up to a multiple of 64), so every wave lies in one
+// slab and the slab's buffers and offsets are the same for all its lanes.
 struct BlockArgs {
     Geom g;
-    uint32_t ns, nr; // slabs along the sweep axis, rows per slab
-    uint32_t slab;   // nodes per slab, lx * nr
-    FastDiv dslab;   // division by slab
+    uint32_t ns, nr;     // slabs along the sweep axis, rows per slab
+    uint32_t slab;       // nodes per slab, lx * nr
+    uint32_t slabp;      // slab rounded up to a whole number of waves
+    FastDiv dslabp;      // division by slabp
     int32_t s0;
     uint32_t count;
     PlaneStore src, dst;
     RatesD r;
     int nt;
 };
+
+constexpr uint32_t WAVE_LANES = 64;
+
+inline uint32_t slab_padded(uint32_t slab) { return (slab + WAVE_LANES - 1) / WAVE_LANES * WAVE_LANES; }
 
 // Slab (0 .. ns-1) of the launch's sl-th slab; s0 >= -ns.
 __host__ __device__ inline uint32_t block_plane(const BlockArgs &a, uint32_t sl)
@@ -172,46 +202,53 @@ __host__ __device__ inline uint32_t block_plane(const BlockArgs &a, uint32_t sl)
     return (uint32_t)s % a.ns;
 }
 
-// Gather sources of one node (ix, r, s) of a blocked launch: slabs s + 1, s,
-// s - 1 (index c + 1 holds slab s - c for the sweep-axis component c of a
-// velocity) and rows r + 1, r, r - 1 (likewise for the row-axis component).
-struct BlockNbr {
-    PlaneRef pl[3];
-    uint32_t row[3];
+// The source slabs of slab s: index c + 1 holds slab s - c, for the sweep-axis
+// component c of a velocity (slabs s + 1, s, s - 1).
+__host__ __device__ inline void block_planes(const BlockArgs &a, uint32_t s, PlaneRef (&pl)[3])
+{
+    pl[0] = plane_read(a.src, s + 1 == a.ns ? 0 : s + 1);
+    pl[1] = plane_read(a.src, s);
+    pl[2] = plane_read(a.src, s == 0 ? a.ns - 1 : s - 1);
+}
+
+// A node's own part of its gather addresses: element offsets within a slab
+// of column ix of rows r + 1, r, r - 1 (index c + 1 for the row-axis
+// component c of a velocity, rows wrapping), with the slab's row stride.  The
+// offsets are below nr * row_stride <= the lattice size, so 32 bits hold them.
+struct BlockLane {
+    uint32_t o[3];
     uint32_t ix, lx;
 };
 
-__host__ __device__ inline BlockNbr block_nbr(const BlockArgs &a, uint32_t ix, uint32_t r, uint32_t s)
+__host__ __device__ inline BlockLane block_lane(uint32_t ix, uint32_t r, uint32_t nr, uint32_t row_stride, uint32_t lx)
 {
-    BlockNbr b;
-    b.pl[0] = plane_read(a.src, s + 1 == a.ns ? 0 : s + 1);
-    b.pl[1] = plane_read(a.src, s);
-    b.pl[2] = plane_read(a.src, s == 0 ? a.ns - 1 : s - 1);
-    b.row[0] = r + 1 == a.nr ? 0 : r + 1;
-    b.row[1] = r;
-    b.row[2] = r == 0 ? a.nr - 1 : r - 1;
-    b.ix = ix;
-    b.lx = a.g.lx;
-    return b;
+    BlockLane l;
+    const uint32_t o = r * row_stride + ix;
+    l.o[0] = r + 1 == nr ? ix : o + row_stride;
+    l.o[1] = o;
+    l.o[2] = r == 0 ? (nr - 1) * row_stride + ix : o - row_stride;
+    l.ix = ix;
+    l.lx = lx;
+    return l;
 }
 
 // Buffer and element index that F_I of the node is gathered from: P(x - c_I)
 // in population I, or at a wall the node's own slot of population OPP[I]
 // (halfway bounce-back).
 template <int AXIS, int I>
-__host__ __device__ inline void block_src(const BlockNbr &b, void *&buf, uint64_t &idx)
+__host__ __device__ inline void block_src(const PlaneRef (&pl)[3], const BlockLane &l, void *&buf, uint64_t &idx)
 {
     constexpr int cs = AXIS == 2 ? CZ[I] : CY[I]; // along the sweep axis
     constexpr int cr = AXIS == 2 ? CY[I] : CZ[I]; // along the rows
-    const bool walled = (CX[I] == 1 && b.ix == 0) || (CX[I] == -1 && b.ix + 1 == b.lx);
-    if (walled) {
-        const PlaneRef &p = b.pl[1];
+    const PlaneRef &p = pl[cs + 1];
+    if constexpr (CX[I] == 0) {
         buf = p.buf;
-        idx = (uint64_t)OPP[I] * p.stride + p.off + (uint64_t)b.row[1] * p.row_stride + b.ix;
+        idx = (uint64_t)I * p.stride + p.off + l.o[cr + 1];
     } else {
-        const PlaneRef &p = b.pl[cs + 1];
-        buf = p.buf;
-        idx = (uint64_t)I * p.stride + p.off + (uint64_t)b.row[cr + 1] * p.row_stride + (b.ix - CX[I]);
+        const bool walled = CX[I] == 1 ? l.ix == 0 : l.ix + 1 == l.lx;
+        buf = walled ? pl[1].buf : p.buf;
+        idx = walled ? (uint64_t)OPP[I] * pl[1].stride + pl[1].off + l.o[1]
+                     : (uint64_t)I * p.stride + p.off + (l.o[cr + 1] - (uint32_t)CX[I]);
     }
 }
 

@@ -24,6 +24,26 @@ namespace d3q19 {
 
 constexpr int BLOCK = 256;
 
+// Element `node` of a population row, addressed as the row base plus a 32-bit
+// byte offset when that cannot wrap: FP16 storage, whose rows hold fewer than
+// 2^31 nodes (lbm_create's limit), i.e. fewer than 2^32 bytes.  A wave-uniform
+// row base then stays in scalar registers and the load or store takes the
+// lane's offset directly, with no 64-bit address arithmetic per population.
+template <int P>
+__device__ __forceinline__ typename Prec<P>::S *elem(typename Prec<P>::S *row, uint32_t node)
+{
+    if constexpr (P == 2)
+        return (typename Prec<P>::S *)((char *)row + node * (uint32_t)sizeof(typename Prec<P>::S));
+    else
+        return row + node;
+}
+
+template <int P>
+__device__ __forceinline__ const typename Prec<P>::S *elem(const typename Prec<P>::S *row, uint32_t node)
+{
+    return elem<P>(const_cast<typename Prec<P>::S *>(row), node);
+}
+
 // F_I(x): streamed population of direction I at node x.  The bounce-back
 // choice is made with masks, not branches, so every load issues back to back.
 // Loads are plain (cached): the +-x shifted reads of neighbouring waves share
@@ -34,14 +54,20 @@ __device__ __forceinline__ typename Prec<P>::T pull(const typename Prec<P>::S *_
 {
     if constexpr (CX[I] != 0) {
         // a walled pull reads the node's own opposite slot (stored with the
-        // opposite direction's scale, which equals this direction's)
+        // opposite direction's scale, which equals this direction's); the
+        // two row bases are wave-uniform, so the lane selects between them
         static_assert(Prec<P>::template scale_class<I>() == Prec<P>::template scale_class<OPP[I]>());
-        const uint32_t w = walled<I>(b) ? 0xffffffffu : 0u;
-        const uint32_t node = (b.n & w) | (src_of<I>(b) & ~w);
-        const uint64_t row = w ? (uint64_t)OPP[I] * N : (uint64_t)I * N;
-        return Prec<P>::template to_T<I>(L[row + node]);
+        const bool w = walled<I>(b);
+        const uint32_t node = w ? b.n : src_of<I>(b);
+        uint64_t row_i = (uint64_t)I * N, row_o = (uint64_t)OPP[I] * N;
+        // opaque to the optimiser, which would otherwise turn the select of
+        // two scalar row offsets into a per-lane 64-bit multiply of a
+        // selected index
+        __asm__("" : "+s"(row_i));
+        __asm__("" : "+s"(row_o));
+        return Prec<P>::template to_T<I>(*elem<P>(L + (w ? row_o : row_i), node));
     } else {
-        return Prec<P>::template to_T<I>(L[(uint64_t)I * N + src_of<I>(b)]);
+        return Prec<P>::template to_T<I>(*elem<P>(L + (uint64_t)I * N, src_of<I>(b)));
     }
 }
 
@@ -63,7 +89,7 @@ template <int P, int... Is>
 __device__ __forceinline__ void store_all_impl(typename Prec<P>::S *__restrict__ L, uint64_t N, uint32_t n,
                                                const typename Prec<P>::T (&f)[19], std::integer_sequence<int, Is...>)
 {
-    ((Prec<P>::template st<Is>(L + (uint64_t)Is * N + n, f[Is])), ...);
+    ((Prec<P>::template st<Is>(elem<P>(L + (uint64_t)Is * N, n), f[Is])), ...);
 }
 
 template <int P>
@@ -178,20 +204,7 @@ __device__ __forceinline__ void step_node(const StepArgs &a)
     T f[19];
     pull_all<P>(src, a.g.N, b, f);
 
-    Rates<T> r;
-    r.s1 = (T)a.r.s1;
-    r.s2 = (T)a.r.s2;
-    r.s4 = (T)a.r.s4;
-    r.s9 = (T)a.r.s9;
-    r.s10 = (T)a.r.s10;
-    r.s13 = (T)a.r.s13;
-    r.s16 = (T)a.r.s16;
-    r.omegepsl = (T)a.r.omegepsl;
-    r.omegepslj = (T)a.r.omegepslj;
-    r.omegxx = (T)a.r.omegxx;
-    r.fx = (T)a.r.fx;
-    r.fy = (T)a.r.fy;
-    r.fz = (T)a.r.fz;
+    Rates<T> r = rates_as<T>(a.r);
     T fmx = r.fx, fmy = r.fy, fmz = r.fz; // force in u = j + F/2 of the fused fields
     if constexpr ((MODE & 16) != 0) {
         uint32_t ix, iy, iz;
@@ -288,30 +301,32 @@ __global__ void __launch_bounds__(BLOCK, 4) k_step_closure(StepArgs a)
 // stores, so a ring of slabs written here stays in the last-level cache for
 // the next launch, which reads it.
 template <int P, int AXIS, int... Is>
-__device__ __forceinline__ void block_gather(const BlockNbr &b, typename Prec<P>::T (&f)[19],
-                                             std::integer_sequence<int, Is...>)
+__device__ __forceinline__ void block_gather(const PlaneRef (&pl)[3], const BlockLane &l,
+                                             typename Prec<P>::T (&f)[19], std::integer_sequence<int, Is...>)
 {
     using S = typename Prec<P>::S;
     (
         [&] {
             void *buf;
             uint64_t idx;
-            block_src<AXIS, Is>(b, buf, idx);
+            block_src<AXIS, Is>(pl, l, buf, idx);
             f[Is] = Prec<P>::template to_T<Is>(((const S *)buf)[idx]);
         }(),
         ...);
 }
 
+// Store the node's 19 values at element e of slab reference d (population I
+// at I * d.stride + e).
 template <int P, int NT, int... Is>
-__device__ __forceinline__ void block_store(const PlaneRef &d, uint64_t e, const typename Prec<P>::T (&f)[19],
+__device__ __forceinline__ void block_store(const PlaneRef &d, uint32_t e, const typename Prec<P>::T (&f)[19],
                                             std::integer_sequence<int, Is...>)
 {
     using S = typename Prec<P>::S;
-    S *base = (S *)d.buf + e;
+    S *base = (S *)d.buf + d.off;
     if constexpr (NT)
-        ((Prec<P>::template st<Is>(base + (uint64_t)Is * d.stride, f[Is])), ...);
+        ((Prec<P>::template st<Is>(base + (uint64_t)Is * d.stride + e, f[Is])), ...);
     else
-        ((base[(uint64_t)Is * d.stride] = Prec<P>::template to_S<Is>(f[Is])), ...);
+        ((base[(uint64_t)Is * d.stride + e] = Prec<P>::template to_S<Is>(f[Is])), ...);
 }
 
 template <int P, int AXIS, int NT>
@@ -319,32 +334,23 @@ __global__ void __launch_bounds__(BLOCK) k_step_planes(BlockArgs a)
 {
     using T = typename Prec<P>::T;
     const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
-    if ((uint64_t)t >= (uint64_t)a.slab * a.count)
+    // the wave's slab, uniform across its lanes (slabs are padded to whole
+    // waves), so the slab buffers and offsets below stay in scalar registers
+    const uint32_t sl = __builtin_amdgcn_readfirstlane(fdiv(t, a.dslabp));
+    const uint32_t w = t - sl * a.slabp;
+    if (sl >= a.count || w >= a.slab)
         return;
-    const uint32_t sl = fdiv(t, a.dslab);
-    const uint32_t w = t - sl * a.slab;
     const uint32_t r = fdiv(w, a.g.dlx);
     const uint32_t ix = w - r * a.g.lx;
     const uint32_t sb = block_plane(a, sl);
-    const BlockNbr b = block_nbr(a, ix, r, sb);
+    PlaneRef pl[3];
+    block_planes(a, sb, pl);
+    const BlockLane l = block_lane(ix, r, a.nr, (uint32_t)a.src.row_stride, a.g.lx);
 
     T f[19];
-    block_gather<P, AXIS>(b, f, std::make_integer_sequence<int, 19>{});
+    block_gather<P, AXIS>(pl, l, f, std::make_integer_sequence<int, 19>{});
 
-    Rates<T> rt;
-    rt.s1 = (T)a.r.s1;
-    rt.s2 = (T)a.r.s2;
-    rt.s4 = (T)a.r.s4;
-    rt.s9 = (T)a.r.s9;
-    rt.s10 = (T)a.r.s10;
-    rt.s13 = (T)a.r.s13;
-    rt.s16 = (T)a.r.s16;
-    rt.omegepsl = (T)a.r.omegepsl;
-    rt.omegepslj = (T)a.r.omegepslj;
-    rt.omegxx = (T)a.r.omegxx;
-    rt.fx = (T)a.r.fx;
-    rt.fy = (T)a.r.fy;
-    rt.fz = (T)a.r.fz;
+    Rates<T> rt = rates_as<T>(a.r);
     T rho9, ux9, uy9, uz9;
     macro_sums(f, rho9, ux9, uy9, uz9);
     ux9 = ux9 + rt.fx / T(2.); // collision.f90:415-417
@@ -356,10 +362,11 @@ __global__ void __launch_bounds__(BLOCK) k_step_planes(BlockArgs a)
         mrt_collide_delta(f, rho9, ux9, uy9, uz9, T(0), T(0), T(0), T(0), rt);
 
     const PlaneRef d = plane_write(a.dst, sb);
-    block_store<P, NT>(d, d.off + (uint64_t)r * d.row_stride + ix, f, std::make_integer_sequence<int, 19>{});
+    const uint32_t e = r * (uint32_t)d.row_stride + ix; // head and ring rows share the stride
+    block_store<P, NT>(d, e, f, std::make_integer_sequence<int, 19>{});
     const PlaneRef hd = plane_write_head(a.dst, sb);
     if (hd.buf != nullptr)
-        block_store<P, 0>(hd, hd.off + (uint64_t)r * hd.row_stride + ix, f, std::make_integer_sequence<int, 19>{});
+        block_store<P, 0>(hd, e, f, std::make_integer_sequence<int, 19>{});
 }
 
 // Closure diagnostics for nodes [n_begin, n_begin + count) of the current
@@ -383,17 +390,7 @@ __global__ void __launch_bounds__(BLOCK) k_closure_eval(StepArgs a, uint64_t n_b
     const Nbr b = make_nbr(n, a.g);
     T f[19];
     pull_all<P>((const S *)a.src, a.g.N, b, f);
-    Rates<T> r;
-    r.s1 = (T)a.r.s1;
-    r.s2 = (T)a.r.s2;
-    r.s4 = (T)a.r.s4;
-    r.s9 = (T)a.r.s9;
-    r.s10 = (T)a.r.s10;
-    r.s13 = (T)a.r.s13;
-    r.s16 = (T)a.r.s16;
-    r.omegepsl = (T)a.r.omegepsl;
-    r.omegepslj = (T)a.r.omegepslj;
-    r.omegxx = (T)a.r.omegxx;
+    Rates<T> r = rates_as<T>(a.r);
     uint32_t ix, iy, iz;
     node_coords(n, a.g, ix, iy, iz);
     double x, y, z;
@@ -407,16 +404,8 @@ __global__ void __launch_bounds__(BLOCK) k_closure_eval(StepArgs a, uint64_t n_b
     ux9 = ux9 + (T)x / T(2.);
     uy9 = uy9 + (T)y / T(2.);
     uz9 = uz9 + (T)z / T(2.);
-    T Fb[19];
-    guo_terms(ux9, uy9, uz9, r.fx, r.fy, r.fz, Fb);
-    T f9[19];
-#pragma unroll
-    for (int q = 0; q < 19; q++)
-        f9[q] = f[q] + T(0.5) * Fb[q];
-    T eq[12];
-    equilibria(rho9, ux9, uy9, uz9, r, eq);
-    T m[15];
-    moments(f9, m);
+    T m[15], sf[12], eq[12];
+    delta_moments(f, rho9, ux9, uy9, uz9, r, m, sf, eq);
     T neq[9];
     closure_inputs(m, eq, neq);
     const float wd = fminf((float)ix + 0.5f, (float)a.g.lx - (float)ix - 0.5f);
@@ -558,20 +547,7 @@ __global__ void __launch_bounds__(BLOCK) k_step_run(StepArgs a)
     if (!active)
         return;
 
-    Rates<T> r;
-    r.s1 = (T)a.r.s1;
-    r.s2 = (T)a.r.s2;
-    r.s4 = (T)a.r.s4;
-    r.s9 = (T)a.r.s9;
-    r.s10 = (T)a.r.s10;
-    r.s13 = (T)a.r.s13;
-    r.s16 = (T)a.r.s16;
-    r.omegepsl = (T)a.r.omegepsl;
-    r.omegepslj = (T)a.r.omegepslj;
-    r.omegxx = (T)a.r.omegxx;
-    r.fx = (T)a.r.fx;
-    r.fy = (T)a.r.fy;
-    r.fz = (T)a.r.fz;
+    Rates<T> r = rates_as<T>(a.r);
 
 #pragma unroll
     for (int k = 0; k < VW; k++) {
@@ -999,7 +975,7 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
 template <int P>
 hipError_t Launch<P>::step_planes(const BlockArgs &a, int axis, hipStream_t s)
 {
-    const dim3 grid(nblocks<P>((uint64_t)a.slab * a.count)), blk(BLOCK);
+    const dim3 grid(nblocks<P>((uint64_t)a.slabp * a.count)), blk(BLOCK);
     if (axis == 2) {
         if (a.nt)
             hipLaunchKernelGGL((k_step_planes<P, 2, 1>), grid, blk, 0, s, a);

@@ -178,7 +178,7 @@ RatesD rates_of(const lbm_channel *h)
     r.fx = h->fc.fx;
     r.fy = h->fc.fy;
     r.fz = h->fc.fz;
-    return r;
+    return with_f32(r);
 }
 
 int ensure_fields(lbm_channel *h)
@@ -524,7 +524,7 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
             kb = strtol(kp, nullptr, 10);
         const BlockGeom bg = block_geom(h->g, h->baxis);
         kb = std::min<long>(std::min<long>(kb, 16), (long)bg.ns);
-        if (kb < 1 || (uint64_t)h->g.lx * bg.nr * (uint64_t)kb >= (1ull << 31))
+        if (kb < 1 || (uint64_t)slab_padded(h->g.lx * bg.nr) * (uint64_t)kb >= (1ull << 31))
             kb = 0;
         h->bK = (uint32_t)kb;
     }

@@ -60,12 +60,13 @@ static uint64_t node(const Geom &g, int axis, uint32_t ix, uint32_t r, uint32_t 
 }
 
 template <int AXIS, int I>
-static void gather_one(Ctx &c, const BlockNbr &b, uint32_t ix, uint32_t iy, uint32_t z, int s)
+static void gather_one(Ctx &c, const PlaneRef (&pl)[3], const BlockLane &l, uint32_t ix, uint32_t iy, uint32_t z,
+                       int s)
 {
     // (iy, z) here are (row, slab) of the sweep
     void *buf;
     uint64_t idx;
-    block_src<AXIS, I>(b, buf, idx);
+    block_src<AXIS, I>(pl, l, buf, idx);
     Buf *B = c.find(buf);
     if (!B || idx >= B->v.size()) {
         fprintf(stderr, "dir %d node (%u,%u,%u): index %llu outside its buffer\n", I, ix, iy, z,
@@ -94,17 +95,20 @@ static void gather_one(Ctx &c, const BlockNbr &b, uint32_t ix, uint32_t iy, uint
 }
 
 template <int AXIS, int... Is>
-static void gather_all(Ctx &c, const BlockNbr &b, uint32_t ix, uint32_t iy, uint32_t z, int s,
-                       std::integer_sequence<int, Is...>)
+static void gather_all(Ctx &c, const PlaneRef (&pl)[3], const BlockLane &l, uint32_t ix, uint32_t iy, uint32_t z,
+                       int s, std::integer_sequence<int, Is...>)
 {
-    (gather_one<AXIS, Is>(c, b, ix, iy, z, s), ...);
+    (gather_one<AXIS, Is>(c, pl, l, ix, iy, z, s), ...);
 }
 
+// the kernel's store addressing: element (uint32) r * row_stride + ix past the
+// slab offset, population stride d.stride
 static void store(Ctx &c, const PlaneRef &d, uint32_t ix, uint32_t r, uint64_t n, int s)
 {
     Buf *B = c.find(d.buf);
+    const uint32_t e = r * (uint32_t)d.row_stride + ix;
     for (int i = 0; i < 19; i++) {
-        const uint64_t idx = d.off + (uint64_t)r * d.row_stride + ix + (uint64_t)i * d.stride;
+        const uint64_t idx = d.off + (uint64_t)i * d.stride + e;
         if (!B || idx >= B->v.size()) {
             fprintf(stderr, "store outside its buffer\n");
             c.ok = false;
@@ -144,17 +148,31 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
             fprintf(stderr, "non-temporal stores must go to the output lattice only\n");
             return false;
         }
-        for (uint32_t t = 0; t < a.slab * a.count; t++) {
-            const uint32_t sl = fdiv(t, a.dslab);
-            const uint32_t w = t - sl * a.slab;
+        if (a.slabp % WAVE_LANES != 0 || a.slabp < a.slab || a.slabp - a.slab >= WAVE_LANES) {
+            fprintf(stderr, "padded slab %u of slab %u is not the next multiple of a wave\n", a.slabp, a.slab);
+            return false;
+        }
+        const uint32_t nthreads = (a.slabp * a.count + 255u) / 256u * 256u; // whole blocks of the launch
+        for (uint32_t t = 0; t < nthreads; t++) {
+            // the kernel takes the slab of the wave's first lane
+            const uint32_t sl = fdiv(t / WAVE_LANES * WAVE_LANES, a.dslabp);
+            if (sl != fdiv(t, a.dslabp)) {
+                fprintf(stderr, "thread %u: its wave spans two slabs\n", t);
+                return false;
+            }
+            const uint32_t w = t - sl * a.slabp;
+            if (sl >= a.count || w >= a.slab)
+                continue;
             const uint32_t rr = fdiv(w, g.dlx);
             const uint32_t ix = w - rr * g.lx;
             const uint32_t sb = block_plane(a, sl);
-            const BlockNbr b = block_nbr(a, ix, rr, sb);
+            PlaneRef pl[3];
+            block_planes(a, sb, pl);
+            const BlockLane l = block_lane(ix, rr, a.nr, (uint32_t)a.src.row_stride, g.lx);
             if (axis == 2)
-                gather_all<2>(c, b, ix, rr, sb, s, std::make_integer_sequence<int, 19>{});
+                gather_all<2>(c, pl, l, ix, rr, sb, s, std::make_integer_sequence<int, 19>{});
             else
-                gather_all<1>(c, b, ix, rr, sb, s, std::make_integer_sequence<int, 19>{});
+                gather_all<1>(c, pl, l, ix, rr, sb, s, std::make_integer_sequence<int, 19>{});
             if (!c.ok)
                 return false;
             const uint64_t n = node(g, axis, ix, rr, sb);
