@@ -176,7 +176,8 @@ def _asymmetry(p):
     return compare(a, b)
 
 
-def _coarse_run(lx, ly, lz, nu_c, steps, stats_from, stats_every, perturb, seed, closure=None, log=None):
+def _coarse_run(lx, ly, lz, nu_c, steps, stats_from, stats_every, perturb, seed, closure=None, log=None,
+                half_at=None):
     with Channel(lx, ly, lz, "fp32", mrttype=3, visc=nu_c) as ch:
         _start(ch, perturb, seed)
         if closure is not None:
@@ -185,17 +186,21 @@ def _coarse_run(lx, ly, lz, nu_c, steps, stats_from, stats_every, perturb, seed,
         ch.sync()
         t0 = time.perf_counter()
         done = 0
+        first_half = None
         while done < steps:
             ch.run(stats_every)
             done += stats_every
             if done > stats_from:
                 ch.stats_accumulate()
+            if done == half_at:
+                first_half, _ = ch.stats_read()
             if log and done % max(steps // 5, 1) == 0:
                 log(f"coarse{' + closure' if closure is not None else ''}: step {done}/{steps}")
         ch.sync()
         wall = time.perf_counter() - t0
         stats, n = ch.stats_read()
-        return {"stats": stats, "samples": n, "wall_s": wall, "mlups": lx * ly * lz * steps / wall / 1e6}
+        return {"stats": stats, "first_half": first_half, "samples": n, "wall_s": wall,
+                "mlups": lx * ly * lz * steps / wall / 1e6}
 
 
 def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_every=100, snapshots=8,
@@ -217,6 +222,8 @@ def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_ev
     from . import Trainer, closure_params
     say = log or (lambda m: None)
     total = spinup + window
+    # end of the first half of the statistics window (a multiple of stats_every)
+    half_fine = spinup + (window // 2) // stats_every * stats_every
     snap_at = sorted({int(round(s / stats_every)) * stats_every
                       for s in np.linspace(spinup // 2, total, snapshots)})
     Nc = lx * ly * lz // 8
@@ -243,6 +250,8 @@ def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_ev
                 if done > spinup:
                     fine.stats_accumulate()
                     fine.stats_accumulate_filtered()
+                if done == half_fine:
+                    fine_half, _ = fine.stats_read_filtered()
                 if done in snap_at:
                     fine.sync()
                     ts = time.perf_counter()
@@ -273,6 +282,8 @@ def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_ev
                           "mlups": round(lx * ly * lz * total / wall_fine / 1e6, 1),
                           # the two walls of the filtered fine statistics against each other
                           "wall_to_wall_asymmetry": _asymmetry(fine_filt),
+                          # the first half of the window against the whole: its sampling drift
+                          "first_half_vs_window": _stat_errors(fine_half, fine_filt),
                           # filtered against plane-pair means of the unfiltered statistics:
                           # zero for U+ (the filter is linear), the subfilter share for the rest
                           "filter_effect": _stat_errors(fine_filt, 0.5 * (fine_stats[0::2] + fine_stats[1::2]))}
@@ -309,9 +320,12 @@ def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_ev
     runs = [("coarse_plain", None)] + [("coarse_closure_" + m, trained[m]) for m in modes]
     for name, closure in runs:
         r = _coarse_run(lx // 2, ly // 2, lz // 2, nu_c, total // 2, spinup // 2, stats_every // 2, perturb, seed,
-                        closure=closure, log=say)
+                        closure=closure, log=say, half_at=half_fine // 2)
         report[name] = {"wall_s": round(r["wall_s"], 2), "samples": r["samples"], "mlups": round(r["mlups"], 1),
                         "errors_vs_filtered_fine": _stat_errors(r["stats"], ref),
+                        # the same comparison for the first half of both windows
+                        "first_half_errors_vs_filtered_fine": _stat_errors(r["first_half"], fine_half),
+                        "errors_vs_fine": _stat_errors(r["stats"], 0.5 * (fine_stats[0::2] + fine_stats[1::2])),
                         "wall_to_wall_asymmetry": _asymmetry(r["stats"]),
                         "time_to_stats_speedup": round(wall_fine / r["wall_s"], 2)}
         report[name + "_profile"] = r["stats"]
