@@ -56,9 +56,11 @@ inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const Rate
     const BlockGeom bg = block_geom(g, axis);
     const uint32_t ns = bg.ns, slots = block_ring_slots(K);
     const uint64_t slab = (uint64_t)g.lx * bg.nr;
-    const PlaneStore full_in = {lat_in, g.N, ns, bg.lat_slab_stride, bg.lat_row_stride, nullptr, 0, 0, 0};
-    const PlaneStore full_out = {lat_out, g.N, ns, bg.lat_slab_stride, bg.lat_row_stride, nullptr, 0, 0, 0};
-    const PlaneStore ring_only = {ring, slots * slab, slots, slab, g.lx, nullptr, 0, 0, 0};
+    const PlaneStore full_in = {lat_in, g.N, ns, make_fastdiv(ns), bg.lat_slab_stride, bg.lat_row_stride, nullptr, 0,
+                                0, 0};
+    const PlaneStore full_out = {lat_out, g.N, ns, make_fastdiv(ns), bg.lat_slab_stride, bg.lat_row_stride, nullptr, 0,
+                                 0, 0};
+    const PlaneStore ring_only = {ring, slots * slab, slots, make_fastdiv(slots), slab, g.lx, nullptr, 0, 0, 0};
     PlaneStore ring_head = ring_only;
     ring_head.head = head;
     ring_head.head_stride = 2 * slab;

@@ -130,6 +130,7 @@ struct PlaneStore {
     void *ring;
     uint64_t ring_stride;
     uint32_t ring_slots;
+    FastDiv dslots; // division by ring_slots
     uint64_t slab_stride, row_stride;
     void *head; // nullable
     uint64_t head_stride;
@@ -146,20 +147,26 @@ struct PlaneRef {
     uint64_t row_stride;
 };
 
+// Ring slot of slab s (s < 2^31), by multiply-high rather than a division.
+__host__ __device__ inline uint32_t ring_slot(const PlaneStore &st, uint32_t s)
+{
+    return s - fdiv(s, st.dslots) * st.ring_slots;
+}
+
 // Slab s as a reader sees it.
 __host__ __device__ inline PlaneRef plane_read(const PlaneStore &st, uint32_t s)
 {
     const uint32_t h = s - (uint32_t)st.head_lo; // wraps for s < head_lo
     if (st.head != nullptr && h < st.head_n)
         return PlaneRef{st.head, st.head_stride, (uint64_t)h * st.slab_stride, st.row_stride};
-    return PlaneRef{st.ring, st.ring_stride, (uint64_t)(s % st.ring_slots) * st.slab_stride, st.row_stride};
+    return PlaneRef{st.ring, st.ring_stride, (uint64_t)ring_slot(st, s) * st.slab_stride, st.row_stride};
 }
 
 // Slab s as a writer stores it: always its ring slot, and also the head copy
 // when s is a head slab (plane_write_head; buf = nullptr otherwise).
 __host__ __device__ inline PlaneRef plane_write(const PlaneStore &st, uint32_t s)
 {
-    return PlaneRef{st.ring, st.ring_stride, (uint64_t)(s % st.ring_slots) * st.slab_stride, st.row_stride};
+    return PlaneRef{st.ring, st.ring_stride, (uint64_t)ring_slot(st, s) * st.slab_stride, st.row_stride};
 }
 
 __host__ __device__ inline PlaneRef plane_write_head(const PlaneStore &st, uint32_t s)
@@ -193,13 +200,16 @@ constexpr uint32_t WAVE_LANES = 64;
 
 inline uint32_t slab_padded(uint32_t slab) { return (slab + WAVE_LANES - 1) / WAVE_LANES * WAVE_LANES; }
 
-// Slab (0 .. ns-1) of the launch's sl-th slab; s0 >= -ns.
+// Slab (0 .. ns-1) of the launch's sl-th slab; -ns <= s0 < ns and
+// sl < count <= ns, so one wrap either way suffices.
 __host__ __device__ inline uint32_t block_plane(const BlockArgs &a, uint32_t sl)
 {
     int32_t s = a.s0 + (int32_t)sl;
     if (s < 0)
         s += (int32_t)a.ns;
-    return (uint32_t)s % a.ns;
+    if (s >= (int32_t)a.ns)
+        s -= (int32_t)a.ns;
+    return (uint32_t)s;
 }
 
 // The source slabs of slab s: index c + 1 holds slab s - c, for the sweep-axis

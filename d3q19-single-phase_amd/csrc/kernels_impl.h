@@ -59,13 +59,18 @@ __device__ __forceinline__ typename Prec<P>::T pull(const typename Prec<P>::S *_
         static_assert(Prec<P>::template scale_class<I>() == Prec<P>::template scale_class<OPP[I]>());
         const bool w = walled<I>(b);
         const uint32_t node = w ? b.n : src_of<I>(b);
-        uint64_t row_i = (uint64_t)I * N, row_o = (uint64_t)OPP[I] * N;
-        // opaque to the optimiser, which would otherwise turn the select of
-        // two scalar row offsets into a per-lane 64-bit multiply of a
-        // selected index
-        __asm__("" : "+s"(row_i));
-        __asm__("" : "+s"(row_o));
-        return Prec<P>::template to_T<I>(*elem<P>(L + (w ? row_o : row_i), node));
+        if constexpr (P == 0) {
+            // memory-bound: this form measured 1.4 % faster than the one below
+            return Prec<P>::template to_T<I>(L[(w ? (uint64_t)OPP[I] * N : (uint64_t)I * N) + node]);
+        } else {
+            uint64_t row_i = (uint64_t)I * N, row_o = (uint64_t)OPP[I] * N;
+            // opaque to the optimiser, which would otherwise turn the select
+            // of two scalar row offsets into a per-lane 64-bit multiply of a
+            // selected index (quarter-rate; FP16 storage is issue-bound)
+            __asm__("" : "+s"(row_i));
+            __asm__("" : "+s"(row_o));
+            return Prec<P>::template to_T<I>(*elem<P>(L + (w ? row_o : row_i), node));
+        }
     } else {
         return Prec<P>::template to_T<I>(*elem<P>(L + (uint64_t)I * N, src_of<I>(b)));
     }

@@ -148,6 +148,10 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
             fprintf(stderr, "non-temporal stores must go to the output lattice only\n");
             return false;
         }
+        if (a.s0 < -(int32_t)a.ns || a.s0 >= (int32_t)a.ns || a.count > a.ns) {
+            fprintf(stderr, "launch s0 %d count %u outside the range block_plane wraps once\n", a.s0, a.count);
+            return false;
+        }
         if (a.slabp % WAVE_LANES != 0 || a.slabp < a.slab || a.slabp - a.slab >= WAVE_LANES) {
             fprintf(stderr, "padded slab %u of slab %u is not the next multiple of a wave\n", a.slabp, a.slab);
             return false;
