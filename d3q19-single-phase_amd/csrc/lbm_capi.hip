@@ -507,8 +507,8 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         // than single steps: the 2048x1024x512 channel's 40 MB y slabs allow
         // two, 33 900 against 38 200 MLUPS).  LBM_BLOCK_AXIS=z|y sets the
         // axis, LBM_BLOCK_PLANES=K the chunk (0: off).  Not by default for
-        // FP16 storage, whose step is bound by arithmetic (measured 49 400
-        // MLUPS blocked against 61 400 single-step).
+        // FP16 storage, whose step is bound by arithmetic (measured 60 800
+        // MLUPS blocked against 68 100 single-step).
         auto chunk = [&](int axis) -> long {
             const BlockGeom bg = block_geom(h->g, axis);
             const double slab_bytes = (double)h->g.lx * bg.nr * 19.0 * (double)h->elem;
