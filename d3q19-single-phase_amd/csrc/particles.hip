@@ -49,22 +49,35 @@ __device__ inline uint32_t node_of(const Geom &g, int ix, int iy, int iz)
     return (uint32_t)ix + g.lx * ((uint32_t)iy + g.ly * (uint32_t)iz);
 }
 
-// Block-wide exclusive scan of one int per thread (PBLOCK threads).
+// Block-wide exclusive scan of one int per thread (PBLOCK threads, thread
+// order), and the block total: an inclusive scan within each wave by lane
+// shifts, then the wave totals through LDS.  Integer sums, so the result is
+// exact and independent of the scheme.
 __device__ inline int block_scan(int v, int &total)
 {
-    __shared__ int s[PBLOCK];
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < PBLOCK; o <<= 1) {
-        int t = (threadIdx.x >= (unsigned)o) ? s[threadIdx.x - o] : 0;
-        __syncthreads();
-        s[threadIdx.x] += t;
-        __syncthreads();
+    constexpr int NW = PBLOCK / 64;
+    __shared__ int wsum[NW];
+    const int lane = (int)(threadIdx.x & 63u), wid = (int)(threadIdx.x >> 6);
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(x, (unsigned)o, 64);
+        if (lane >= o)
+            x += t;
     }
-    total = s[PBLOCK - 1];
-    const int ex = s[threadIdx.x] - v;
+    if (lane == 63)
+        wsum[wid] = x;
     __syncthreads();
-    return ex;
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        const int sw = wsum[w];
+        off += w < wid ? sw : 0;
+        tot += sw;
+    }
+    __syncthreads(); // wsum is rewritten by the next call
+    total = tot;
+    return off + x - v;
 }
 
 // Phase 1: nodes of particle p's previous box still marked p but no longer
@@ -147,14 +160,29 @@ __global__ void __launch_bounds__(PBLOCK) k_p_links(PartGeom pg, PartState ps)
             double ox, oy, oz;
             node_offset(pg, ix, iy, iz, X, ox, oy, oz);
             const double reach = pg.R + 1.5;
-            if (ox * ox + oy * oy + oz * oz < reach * reach && ps.solid[n] < 0) {
+            if (ox * ox + oy * oy + oz * oz < reach * reach) {
+                // the node's flag and its 18 neighbours' load together; a
+                // neighbour beyond a wall reads the node itself (in bounds)
+                // and is masked out
+                const int ym = wrap(iy - 1, pg.g.ly), yp = wrap(iy + 1, pg.g.ly);
+                const int zm = wrap(iz - 1, pg.g.lz), zp = wrap(iz + 1, pg.g.lz);
+                const int own = ps.solid[n];
+                int sv[19];
+#pragma unroll
                 for (int i = 1; i < 19; i++) {
                     const int jx = ix + CX[i];
-                    if (jx < 0 || jx >= (int)pg.g.lx)
-                        continue;
-                    const int jy = wrap(iy + CY[i], pg.g.ly), jz = wrap(iz + CZ[i], pg.g.lz);
-                    if (ps.solid[node_of(pg.g, jx, jy, jz)] == p)
-                        mask |= 1u << i;
+                    const bool in = jx >= 0 && jx < (int)pg.g.lx;
+                    const int jy = CY[i] == 0 ? iy : (CY[i] > 0 ? yp : ym);
+                    const int jz = CZ[i] == 0 ? iz : (CZ[i] > 0 ? zp : zm);
+                    sv[i] = ps.solid[node_of(pg.g, in ? jx : ix, jy, jz)];
+                }
+                if (own < 0) {
+#pragma unroll
+                    for (int i = 1; i < 19; i++) {
+                        const int jx = ix + CX[i];
+                        if (jx >= 0 && jx < (int)pg.g.lx && sv[i] == p)
+                            mask |= 1u << i;
+                    }
                 }
             }
         }
@@ -244,25 +272,27 @@ __device__ inline void ibb_link(const PartGeom &pg, const PartState &ps, typenam
     const double uwz = V[2] + (W[0] * ry - W[1] * rx);
     const double cu = cx * uwx + cy * uwy + cz * uwz;
 
+    // Every value either branch may need loads at once, so one memory round
+    // trip serves the link: f_i at x, f_opp(i) at x, and flag and f_i of the
+    // second fluid node x - c_i behind x (x itself when that lies beyond a
+    // wall, so the address stays in bounds; the branch below then ignores it).
+    // When x - c_i is a solid node, another link of this launch may be storing
+    // into its slots; the value read there is then discarded.
+    const int bx_ = ix - CX[i];
+    const bool behind_in = (bx_ >= 0 && bx_ < (int)pg.g.lx);
+    const uint32_t nb =
+        behind_in ? node_of(pg.g, bx_, wrap(iy - CY[i], pg.g.ly), wrap(iz - CZ[i], pg.g.lz)) : n;
     const double fi = ld_rt<P>(L, N, i, n);
+    const double fib = ld_rt<P>(L, N, ib, n);
+    const int solid_b = ps.solid[nb];
+    const double fb = ld_rt<P>(L, N, i, nb);
     double g;
     if (q < 0.5) {
-        // second fluid node behind x along -c_i
-        const int bx_ = ix - CX[i];
-        bool ok = (bx_ >= 0 && bx_ < (int)pg.g.lx);
-        uint32_t nb = 0;
-        if (ok) {
-            nb = node_of(pg.g, bx_, wrap(iy - CY[i], pg.g.ly), wrap(iz - CZ[i], pg.g.lz));
-            ok = ps.solid[nb] < 0;
-        }
-        if (ok) {
-            const double fb = ld_rt<P>(L, N, i, nb);
+        if (behind_in && solid_b < 0)
             g = (2.0 * q) * fi + (1.0 - 2.0 * q) * fb - 6.0 * w * cu;
-        } else {
+        else
             g = fi - 6.0 * w * cu;
-        }
     } else {
-        const double fib = ld_rt<P>(L, N, ib, n);
         g = (1.0 / (2.0 * q)) * fi + ((2.0 * q - 1.0) / (2.0 * q)) * fib - (3.0 / q) * w * cu;
     }
     // the solid node x + c_i: its slot ib is what x pulls for direction ib
