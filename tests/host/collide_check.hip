@@ -9,7 +9,12 @@
 //      for unrelated field values (the array mode);
 //   3. mrt_collide_delta<float> stays within FP32 round-off of the FP64
 //      result, and its momentum increment is F up to the rounding of the
-//      stored populations.
+//      stored populations;
+//   4. the closed-form moments of the Guo term that the increment form uses
+//      (guo_moments) equal the moments of the 19 Guo terms of the reference
+//      (guo_terms, then moments()), with sum Fbar = 0, sum c Fbar = F and
+//      vanishing third-order moments.  Dropping a term or a factor of the
+//      closed form, or swapping a velocity and a force component, fails it.
 //
 // Built with hipcc -ffp-contract=off and run on the CPU; launches nothing on a
 // GPU.  Prints one "ok" line per check, exits non-zero on a failure.
@@ -140,7 +145,33 @@ int main()
                 worst_mom32 = fmax(worst_mom32, fabs(dj[a] - (double)((&r32.fx)[a])));
         }
     }
+    // 4. closed-form Guo moments vs the moments of the 19 Guo terms
+    double worst_guo = 0.0;
+    for (int t = 0; t < trials; t++) {
+        double u[3], F[3];
+        for (int a = 0; a < 3; a++) {
+            u[a] = sym(0.15);
+            F[a] = sym(1e-3);
+        }
+        double Fb[19], m[15], sf[12];
+        guo_terms(u[0], u[1], u[2], F[0], F[1], F[2], Fb);
+        moments(Fb, m);
+        guo_moments(u[0], u[1], u[2], F[0], F[1], F[2], sf);
+        double r, jx, jy, jz;
+        macro_sums(Fb, r, jx, jy, jz);
+        for (int k = 0; k < 12; k++)
+            worst_guo = fmax(worst_guo, fabs(m[k] - sf[k]));
+        for (int k = 12; k < 15; k++)
+            worst_guo = fmax(worst_guo, fabs(m[k]));
+        worst_guo = fmax(worst_guo, fmax(fabs(r), fmax(fabs(jx - F[0]), fmax(fabs(jy - F[1]), fabs(jz - F[2])))));
+    }
     int fail = 0;
+    // the moments are of size |u||F| ~ 1e-4: a dropped term is >= 1e-6
+    printf("Guo-term moments, closed form vs moments of the 19 terms: max |diff| %.3e\n", worst_guo);
+    if (worst_guo > 1e-17)
+        fail = 1;
+    else
+        printf("ok guo_moments\n");
     printf("reference form vs oracle: %ld of %d states differ\n", mismatches, 3 * trials);
     if (mismatches != 0)
         fail = 1;

@@ -1,7 +1,8 @@
 """Host-side check of the node collision arithmetic (tests/host/collide_check.hip):
 the FP64 kernel's collision equals the oracle bit for bit on random states for
 all three rate sets, the increment form used for FP32/FP16 storage agrees with
-it, and its FP32 arithmetic stays within FP32 round-off.  The kernels' own
+it, its FP32 arithmetic stays within FP32 round-off, and the closed-form
+moments of the Guo term it uses equal the moments of the 19 Guo terms.  The kernels' own
 source is compiled for the CPU with hipcc; no GPU involved."""
 import os
 import shutil
@@ -29,5 +30,5 @@ def test_collision_arithmetic_on_host():
                        check=True, capture_output=True)
         r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr + r.stdout
-    for tag in ("ok bitwise", "ok increment64", "ok increment32"):
+    for tag in ("ok guo_moments", "ok bitwise", "ok increment64", "ok increment32"):
         assert tag in r.stdout, r.stdout
