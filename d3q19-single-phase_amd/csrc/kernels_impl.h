@@ -66,7 +66,9 @@ __device__ __forceinline__ typename Prec<P>::T pull(const typename Prec<P>::S *_
             uint64_t row_i = (uint64_t)I * N, row_o = (uint64_t)OPP[I] * N;
             // opaque to the optimiser, which would otherwise turn the select
             // of two scalar row offsets into a per-lane 64-bit multiply of a
-            // selected index (quarter-rate; FP16 storage is issue-bound)
+            // selected index (quarter-rate; FP16 storage is issue-bound).
+            // The "s" operands require N to be wave-uniform: every caller
+            // passes the kernel argument Geom::N.
             __asm__("" : "+s"(row_i));
             __asm__("" : "+s"(row_o));
             return Prec<P>::template to_T<I>(*elem<P>(L + (w ? row_o : row_i), node));
