@@ -106,6 +106,7 @@ def measure(prec, workload, steps, warmup, single_steps=False):
             else:
                 os.environ["LBM_BLOCK_PLANES"] = saved
     with ch:
+        axis, slabs = ch.block_info()
         init_channel(ch, lx, ly, lz)
         m0 = ch.mass()
         ch.run(warmup)
@@ -129,6 +130,8 @@ def measure(prec, workload, steps, warmup, single_steps=False):
                      "algorithmic_bytes": algo},
         # change of the mean density fluctuation (sum of shifted populations / N)
         "mass_drift_per_node": (m1 - m0) / n,
+        # lbm_run's pairing of steps: sweep axis and slabs per chunk (0: single steps)
+        "blocked_sweep": {"axis": axis, "slabs_per_chunk": slabs},
     }
 
 
@@ -351,7 +354,7 @@ def main():
         variants[prec] = {"value": round(v["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(v["ms_per_step"], 4),
                           "kernel_ms": round(v["kernel_ms"], 4), "roofline": v["roofline"],
                           f"speedup_vs_{main_prec}": round(v["mlups"] / main_m["mlups"], 3),
-                          "mass_drift_per_node": v["mass_drift_per_node"]}
+                          "mass_drift_per_node": v["mass_drift_per_node"], "blocked_sweep": v["blocked_sweep"]}
         log(f"bench: {prec} {v['mlups']:.1f} MLUPS, {v['roofline']['achieved']:.0f} GB/s")
 
     forcingp = None
@@ -409,7 +412,8 @@ def main():
         "data": "synthetic (seeded log-law + divergence-free perturbation)",
         "config": {"workload": desc, "lx": lx, "ly": ly, "lz": lz, "nodes": lx * ly * lz,
                    "kernel_ms": round(main_m["kernel_ms"], 4),
-                   "mass_drift_per_node": main_m["mass_drift_per_node"]},
+                   "mass_drift_per_node": main_m["mass_drift_per_node"],
+                   "blocked_sweep": main_m["blocked_sweep"]},
         "roofline": main_m["roofline"],
         "cpu_baseline": cpu,
         "single_step": {"value": round(single["mlups"], 1), "unit": "MLUPS", "kernel_ms": round(single["kernel_ms"], 4),

@@ -56,11 +56,13 @@ enum MacroState { FROM_CUR = 0, FROM_PREV = 1, ARRAYS = 2 };
 constexpr uint32_t RED_BLOCKS = 2048;   // partial sums per reduction
 constexpr uint32_t PLANE_CHUNKS = 512;  // row chunks for plane sums
 constexpr uint64_t COPY_NODES = 1u << 20; // nodes per upload/download chunk
-// Ring of the two-step blocked sweep: about 60 % of the 256 MB last-level
-// cache (the measured optimum on the 192x768x384 channel in FP64 and FP32;
-// larger rings are partly evicted by the streams through the same cache)
-constexpr double BLOCK_RING_BYTES = 160e6;
+// Ring of the two-step blocked sweep: about 150 MB of the 256 MB last-level
+// cache (13 slots of 11.2 MB measured best on the 192x768x384 channel in
+// FP64 and FP32; larger rings are partly evicted by the streams through the
+// same cache)
+constexpr double BLOCK_RING_BYTES = 150e6;
 constexpr long BLOCK_MIN_SLABS = 4;
+constexpr long BLOCK_PREF_SLABS = 8;
 
 } // namespace
 
@@ -502,20 +504,31 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         const char *k = getenv("LBM_STEP_KERNEL");
         h->runs = (k && strcmp(k, "runs") == 0) ? 1 : 0;
         // two-step blocked sweep: chunks of K slabs whose ring of K + 2 slabs
-        // stays within BLOCK_RING_BYTES, along z, else along y, and only when
-        // a chunk holds at least BLOCK_MIN_SLABS slabs (fewer measured slower
-        // than single steps: the 2048x1024x512 channel's 40 MB y slabs allow
-        // two, 33 900 against 38 200 MLUPS).  LBM_BLOCK_AXIS=z|y sets the
-        // axis, LBM_BLOCK_PLANES=K the chunk (0: off).  Not by default for
-        // FP16 storage, whose step is bound by arithmetic (measured 60 800
-        // MLUPS blocked against 68 100 single-step).
+        // stays within BLOCK_RING_BYTES.  The axis with the larger slabs is
+        // taken when its chunk holds at least BLOCK_PREF_SLABS slabs, else
+        // the other one on the same condition, else the one whose chunk
+        // holds at least BLOCK_MIN_SLABS; with fewer the sweep is off (it
+        // measured slower than single steps: the 2048x1024x512 channel's
+        // 40 MB y slabs allow two, 33 900 against 38 200 MLUPS).  On the
+        // 192x768x384 channel this picks 11.2 MB slabs, 11 to a chunk, in
+        // both FP64 (y: 19 770 MLUPS against 19 110 for z slabs, 5 to a
+        // chunk) and FP32 (z: 39 880 against 39 460 with 12 to a chunk).
+        // LBM_BLOCK_AXIS=z|y sets the axis, LBM_BLOCK_PLANES=K the chunk (0:
+        // off).  Not by default for FP16 storage, whose step is bound by
+        // arithmetic (measured 60 800 MLUPS blocked against 68 100
+        // single-step).
         auto chunk = [&](int axis) -> long {
             const BlockGeom bg = block_geom(h->g, axis);
             const double slab_bytes = (double)h->g.lx * bg.nr * 19.0 * (double)h->elem;
             return (long)(BLOCK_RING_BYTES / slab_bytes) - 2;
         };
+        // z slabs (lx * ly nodes) are the larger ones when ly >= lz
+        const int big = h->g.ly >= h->g.lz ? 2 : 1, small = 3 - big;
+        int axis = big;
+        if (chunk(big) < BLOCK_PREF_SLABS)
+            axis = chunk(small) >= BLOCK_PREF_SLABS ? small : (chunk(big) >= BLOCK_MIN_SLABS ? big : small);
         const char *ka = getenv("LBM_BLOCK_AXIS");
-        h->baxis = ka ? (strcmp(ka, "y") == 0 ? 1 : 2) : (chunk(2) >= BLOCK_MIN_SLABS ? 2 : 1);
+        h->baxis = ka ? (strcmp(ka, "y") == 0 ? 1 : 2) : axis;
         long kb = c.precision == LBM_FP16 ? 0 : chunk(h->baxis);
         if (kb < BLOCK_MIN_SLABS)
             kb = 0;
@@ -1137,6 +1150,16 @@ int lbm_device_bytes(lbm_channel *h, size_t *bytes)
         b += (block_ring_elems(h->g, h->baxis, h->bK) + block_head_elems(h->g, h->baxis)) * h->elem;
     b += (2 * RED_BLOCKS + (size_t)PLANE_CHUNKS * NPLANE * h->g.lx + 2 + 2 * NPLANE * (size_t)h->g.lx) * sizeof(double) + 8;
     *bytes = b;
+    return LBM_OK;
+}
+
+int lbm_block_info(lbm_channel *h, int *axis, int *slabs)
+{
+    CHECK_H(h);
+    if (axis)
+        *axis = h->baxis == 2 ? 'z' : 'y';
+    if (slabs)
+        *slabs = (int)h->bK;
     return LBM_OK;
 }
 

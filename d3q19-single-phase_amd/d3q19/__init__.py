@@ -80,6 +80,7 @@ _SIGS = {
     "lbm_sync": [_H],
     "lbm_time_steps": [_H, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "lbm_device_bytes": [_H, ctypes.POINTER(ctypes.c_size_t)],
+    "lbm_block_info": [_H, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "lbm_download_post": [_H, _D],
     "lbm_particles_create": [_H, _D, _D, _D, _D],
     "lbm_beads_links": [_H],
@@ -372,6 +373,12 @@ class Channel:
         v = ctypes.c_size_t()
         _call("lbm_device_bytes", self._h, ctypes.byref(v))
         return v.value
+
+    def block_info(self):
+        """The blocked sweep run() uses for pairs of steps: (axis 'z'|'y', slabs per chunk; 0 = off)."""
+        a, k = ctypes.c_int(), ctypes.c_int()
+        _call("lbm_block_info", self._h, ctypes.byref(a), ctypes.byref(k))
+        return chr(a.value), k.value
 
     def download_post(self):
         """Stored post-collision lattice, FP64, shape (19, N) (node = ix + lx*(iy + ly*iz))."""

@@ -133,11 +133,13 @@ int lbm_avedensity(lbm_channel *h, double *rhomean);
 /* nsteps iterations of the main loop body, Channel-Flow/main.f90:157-161
  * (collision_MRT then macrovar), without host I/O.  Pairs of steps run as one
  * blocked sweep when the fields come from macrovar, the force is uniform and
- * there are no particles or closure: step t of a few z planes goes to a ring
- * that the last-level cache holds, step t+1 follows one plane behind, so HBM
- * carries one lattice read and write per two steps.  The results equal the
- * single-step path's bit for bit.  The environment variable LBM_BLOCK_PLANES
- * (read by lbm_create) sets the planes per chunk, 0 turns the sweep off. */
+ * there are no particles or closure: step t of a few z or y slabs goes to a
+ * ring that the last-level cache holds, step t+1 follows one slab behind, so
+ * HBM carries one lattice read and write per two steps.  The results equal
+ * the single-step path's bit for bit.  The environment variables
+ * LBM_BLOCK_AXIS=z|y and LBM_BLOCK_PLANES (read by lbm_create) set the axis
+ * and the slabs per chunk, 0 turns the sweep off; lbm_block_info reports the
+ * choice. */
 int lbm_run(lbm_channel *h, int nsteps);
 
 /* Sum of all populations in FP64 (mass monitor; the sum is invariant under
@@ -180,6 +182,12 @@ int lbm_time_steps(lbm_channel *h, int nsteps, double *avg_ms, double *total_ms)
 
 /* Device bytes held by the handle (lattices, fields, scratch). */
 int lbm_device_bytes(lbm_channel *h, size_t *bytes);
+
+/* The two-step blocked sweep lbm_run uses for pairs of steps, as lbm_create
+ * chose it: *axis = 'z' (slabs of lx*ly nodes) or 'y' (lx*lz), *slabs = slabs
+ * per chunk (0: single steps only).  Either pointer may be null.  No
+ * reference counterpart (measurement and tests). */
+int lbm_block_info(lbm_channel *h, int *axis, int *slabs);
 
 /* The stored post-collision lattice as FP64 structure-of-arrays (19*N values,
  * population-major, node index ix + lx*(iy + ly*iz)), for checkpoint-free

@@ -81,3 +81,20 @@ def test_blocked_sweep_then_other_main_loop_calls(monkeypatch):
         assert_bitwise(blk.download_f(), one.download_f(), "f")
         np.testing.assert_array_equal(blk.statistc2(), one.statistc2())
         assert blk.mass() == one.mass()
+
+
+@pytest.mark.parametrize("prec,want", [("fp64", ("y", 11)), ("fp32", ("z", 11)), ("fp16", ("z", 0))])
+def test_default_sweep_on_the_turbulent_channel(monkeypatch, prec, want):
+    """lbm_create's choice for the 192x768x384 channel: 11.2 MB slabs, eleven
+    to a chunk (a 146 MB ring), along y in FP64 and z in FP32; FP16 storage
+    runs single steps.  Only the handle is made (lattices allocated and
+    cleared; no step runs)."""
+    monkeypatch.delenv("LBM_BLOCK_PLANES", raising=False)
+    monkeypatch.delenv("LBM_BLOCK_AXIS", raising=False)
+    print(f"default sweep {prec}", flush=True)
+    with d3q19.Channel(192, 768, 384, prec) as ch:
+        assert ch.block_info() == want
+    # and the sweep stays off for the 2048x1024x512 channel's 80 MB / 40 MB slabs in FP32
+    if prec == "fp32":
+        with d3q19.Channel(512, 2048, 1024, prec) as ch:
+            assert ch.block_info()[1] == 0
