@@ -335,8 +335,8 @@ def main():
         print(json.dumps(s, indent=1))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU slab decomposition is a design, not part of this run", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
+              "the slab decomposition over several GPUs is designed, not implemented", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 

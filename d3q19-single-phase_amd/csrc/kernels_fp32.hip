@@ -1,6 +1,6 @@
 // FP32 and FP16-storage lattice kernels (FP32 arithmetic).  Contraction into
-// fused multiply-add is allowed here; these variants are judged by their
-// error against the FP64 lattice, not bit for bit.
+// fused multiply-add is allowed here; these variants are held to error
+// budgets against the FP64 lattice, not to bit equality.
 #include <utility>
 
 #include "kernels_impl.h"

@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the D3Q19 MRT channel-flow hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker, never the product:
+ * TEST INFRASTRUCTURE ONLY.  It checks the GPU results and is never the product:
  * only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
  * may load it.  The product path (liblbm.so, HIP kernels) never links or
  * calls it.

@@ -1,5 +1,5 @@
-"""CPU oracle for the learned closure -- TEST INFRASTRUCTURE ONLY (tests/ use it
-as the checker).
+"""CPU oracle for the learned closure -- TEST INFRASTRUCTURE ONLY (tests/ check
+the GPU results against it).
 
 Float64 numpy restatements of:
   * the 10-32-32-1 tanh network and its loss mean((softplus(o) - y)^2),

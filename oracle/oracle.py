@@ -1,7 +1,7 @@
 """CPU oracle for the D3Q19 MRT channel-flow hot path (ctypes wrapper).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product (liblbm.so and its
+cpu_baseline leg of bench.py, to check the GPU results.  The product (liblbm.so and its
 Python/Fortran hosts) never imports this module.
 
 `OracleChannel` holds the same module state as Channel-Flow/var_inc.f90:107-139

@@ -1,6 +1,6 @@
 /*
- * CPU oracle for the rigid-particle path -- TEST INFRASTRUCTURE ONLY (used by
- * tests/, as the checker).
+ * CPU oracle for the rigid-particle path -- TEST INFRASTRUCTURE ONLY (tests/
+ * check the GPU results against it).
  *
  * The reference has no particle source (only its data structures and routine
  * names: Channel-Flow/var_inc.f90:30-38,120-150,166-168), so this file states
