@@ -336,11 +336,17 @@ __device__ __forceinline__ void block_store(const PlaneRef &d, uint32_t e, const
         ((base[(uint64_t)Is * d.stride + e] = Prec<P>::template to_S<Is>(f[Is])), ...);
 }
 
+// Threads per block of the sweep: 512 in FP32 (measured 1.4 % faster than
+// 256 on the 192x768x384 channel), 256 otherwise (FP64: 512 measured 1 %
+// slower, 128 3 % slower).
+template <int P>
+constexpr int plane_block() { return P == 1 ? 512 : 256; }
+
 template <int P, int AXIS, int NT>
-__global__ void __launch_bounds__(BLOCK) k_step_planes(BlockArgs a)
+__global__ void __launch_bounds__(plane_block<P>()) k_step_planes(BlockArgs a)
 {
     using T = typename Prec<P>::T;
-    const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t t = blockIdx.x * plane_block<P>() + threadIdx.x;
     // the wave's slab, uniform across its lanes (slabs are padded to whole
     // waves), so the slab buffers and offsets below stay in scalar registers
     const uint32_t sl = __builtin_amdgcn_readfirstlane(fdiv(t, a.dslabp));
@@ -982,7 +988,8 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
 template <int P>
 hipError_t Launch<P>::step_planes(const BlockArgs &a, int axis, hipStream_t s)
 {
-    const dim3 grid(nblocks<P>((uint64_t)a.slabp * a.count)), blk(BLOCK);
+    constexpr uint32_t PB = plane_block<P>();
+    const dim3 grid((uint32_t)(((uint64_t)a.slabp * a.count + PB - 1) / PB)), blk(PB);
     if (axis == 2) {
         if (a.nt)
             hipLaunchKernelGGL((k_step_planes<P, 2, 1>), grid, blk, 0, s, a);

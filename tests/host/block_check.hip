@@ -156,7 +156,8 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
             fprintf(stderr, "padded slab %u of slab %u is not the next multiple of a wave\n", a.slabp, a.slab);
             return false;
         }
-        const uint32_t nthreads = (a.slabp * a.count + 255u) / 256u * 256u; // whole blocks of the launch
+        // whole blocks of the launch (256 or 512 threads; the larger covers both)
+        const uint32_t nthreads = (a.slabp * a.count + 511u) / 512u * 512u;
         for (uint32_t t = 0; t < nthreads; t++) {
             // the kernel takes the slab of the wave's first lane
             const uint32_t sl = fdiv(t / WAVE_LANES * WAVE_LANES, a.dslabp);
