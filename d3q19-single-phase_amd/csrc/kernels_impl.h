@@ -23,9 +23,11 @@
 namespace d3q19 {
 
 constexpr int BLOCK = 256;
-// Threads per block of the plain step kernel k_step: 512 measured faster
-// than 256 on the 192x768x384 channel in FP16 storage (71 600 against 68 100
-// MLUPS) and FP64 (18 320 against 18 150), and 128 slower in both.
+// Threads per block of the step kernels k_step and k_step_closure: 512
+// measured faster than 256 on the 192x768x384 channel in FP16 storage
+// (71 600 against 68 100 MLUPS) and FP64 (18 320 against 18 150), and with
+// the fused closure on the 96x384x192 channel (22 000 against 21 200); 128
+// slower.
 constexpr int STEP_BLOCK = 512;
 
 // Element `node` of a population row, addressed as the row base plus a 32-bit
@@ -193,8 +195,7 @@ __device__ __forceinline__ void step_node(const StepArgs &a)
     // every lane stays to the end (a lane past the lattice works on the last
     // node, a particle node on its own slots) and only active lanes store.
     constexpr bool WAVE = (MODE & 8) != 0;
-    constexpr int NB = WAVE ? BLOCK : STEP_BLOCK; // k_step_closure's or k_step's block
-    const uint32_t n0 = node_index<NB>(a.g);
+    const uint32_t n0 = node_index<STEP_BLOCK>(a.g);
     bool active = n0 < a.g.N;
     if constexpr (!WAVE) {
         if (!active)
@@ -207,7 +208,7 @@ __device__ __forceinline__ void step_node(const StepArgs &a)
         active = active && a.solid[n] < 0;
     __shared__ __attribute__((aligned(16))) _Float16 wl[WAVE ? MLP_LDS : 8];
     if constexpr (WAVE) {
-        mlp_stage(a.mlp, wl, threadIdx.x, NB);
+        mlp_stage(a.mlp, wl, threadIdx.x, STEP_BLOCK);
         __syncthreads();
     }
     const Nbr b = make_nbr(n, a.g);
@@ -301,7 +302,7 @@ __global__ void __launch_bounds__(STEP_BLOCK) k_step(StepArgs a)
 // The closure modes, held to 128 registers so that four waves per SIMD keep
 // the lattice traffic in flight while others run the network.
 template <int P, int MODE>
-__global__ void __launch_bounds__(BLOCK, 4) k_step_closure(StepArgs a)
+__global__ void __launch_bounds__(STEP_BLOCK, 4) k_step_closure(StepArgs a)
 {
     step_node<P, MODE>(a);
 }
@@ -938,9 +939,7 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
             return hipGetLastError();
         }
     }
-    // k_step runs STEP_BLOCK threads per block, k_step_closure BLOCK
     const dim3 grid((uint32_t)((a.g.N + STEP_BLOCK - 1) / STEP_BLOCK)), blk(STEP_BLOCK);
-    const dim3 gridc(nblocks<P>(a.g.N)), blkc(BLOCK);
     switch (mode) {
     case 0:
         hipLaunchKernelGGL((k_step<P, 0>), grid, blk, 0, s, a);
@@ -964,13 +963,13 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
         if constexpr (P == 0)
             return hipErrorInvalidValue;
         else
-            hipLaunchKernelGGL((k_step_closure<P, 8>), gridc, blkc, 0, s, a);
+            hipLaunchKernelGGL((k_step_closure<P, 8>), grid, blk, 0, s, a);
         break;
     case 12:
         if constexpr (P == 0)
             return hipErrorInvalidValue;
         else
-            hipLaunchKernelGGL((k_step_closure<P, 12>), gridc, blkc, 0, s, a);
+            hipLaunchKernelGGL((k_step_closure<P, 12>), grid, blk, 0, s, a);
         break;
     case 16:
         hipLaunchKernelGGL((k_step<P, 16>), grid, blk, 0, s, a);
@@ -985,7 +984,7 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
         if constexpr (P == 0)
             return hipErrorInvalidValue;
         else
-            hipLaunchKernelGGL((k_step_closure<P, 24>), gridc, blkc, 0, s, a);
+            hipLaunchKernelGGL((k_step_closure<P, 24>), grid, blk, 0, s, a);
         break;
     default:
         return hipErrorInvalidValue;
