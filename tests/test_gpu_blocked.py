@@ -98,3 +98,38 @@ def test_default_sweep_on_the_turbulent_channel(monkeypatch, prec, want):
     if prec == "fp32":
         with d3q19.Channel(512, 2048, 1024, prec) as ch:
             assert ch.block_info()[1] == 0
+
+
+def test_default_sweep_bitwise_at_full_size(monkeypatch):
+    """The FP64 default (y slabs, eleven to a chunk) on the full 192x768x384
+    channel from the bench's start (log-law + divergence-free perturbation):
+    two blocked pairs equal four single steps bit for bit, populations and
+    the FP64 mass."""
+    from d3q19 import workloads
+    lx, ly, lz = 192, 768, 384
+    monkeypatch.delenv("LBM_BLOCK_AXIS", raising=False)
+    print("full-size blocked vs single steps, fp64", flush=True)
+    chans = []
+    for planes in (None, "0"):
+        if planes is None:
+            monkeypatch.delenv("LBM_BLOCK_PLANES", raising=False)
+        else:
+            monkeypatch.setenv("LBM_BLOCK_PLANES", planes)
+        chans.append(d3q19.Channel(lx, ly, lz, "fp64"))
+    monkeypatch.delenv("LBM_BLOCK_PLANES", raising=False)
+    blk, one = chans
+    with blk, one:
+        assert blk.block_info() == ("y", 11) and one.block_info()[1] == 0
+        ux, uy, uz = workloads.loglaw_divfree(lx, ly, lz, blk.p["ustar"], blk.p["ystar"], seed=2)
+        for c in (blk, one):
+            c.set_macros(ux=ux, uy=uy, uz=uz)
+            c.FORCING()
+            c.initpop()
+            c.macrovar()
+        del ux, uy, uz
+        blk.run(4)
+        one.run(4)
+        assert blk.mass() == one.mass()
+        a = blk.download_post()
+        b = one.download_post()
+        assert np.array_equal(bits(a), bits(b)), "blocked sweep differs from single steps at full size"
