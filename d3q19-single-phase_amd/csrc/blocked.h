@@ -50,8 +50,18 @@ inline uint64_t block_ring_elems(const Geom &g, int axis, uint32_t K)
 }
 inline uint64_t block_head_elems(const Geom &g, int axis) { return 19ull * 2 * g.lx * block_geom(g, axis).nr; }
 
-inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const RatesD &r, uint32_t K, void *lat_in,
-                                             void *lat_out, void *ring, void *head)
+// The force of one of the two steps of a pair: rates with the uniform force
+// (r.fx, r.fy, r.fz), or per_node with the collision's force fc and the one
+// the fused fields were formed with, fm (BlockArgs::per_node).
+struct StepForce {
+    RatesD r;
+    ForceSpec fc, fm;
+    int per_node;
+};
+
+// Launch list of one pair: step t with force f0, step t+1 with force f1.
+inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const StepForce &f0, const StepForce &f1,
+                                             uint32_t K, void *lat_in, void *lat_out, void *ring, void *head)
 {
     const BlockGeom bg = block_geom(g, axis);
     const uint32_t ns = bg.ns, slots = block_ring_slots(K);
@@ -68,7 +78,8 @@ inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const Rate
     ring_head.head_n = std::min<uint32_t>(2, ns);
 
     std::vector<BlockArgs> out;
-    auto add = [&](int32_t s0, uint32_t count, const PlaneStore &src, const PlaneStore &dst, int nt) {
+    auto add = [&](int32_t s0, uint32_t count, const PlaneStore &src, const PlaneStore &dst, int nt,
+                   const StepForce &f) {
         BlockArgs a;
         a.g = g;
         a.ns = ns;
@@ -76,7 +87,10 @@ inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const Rate
         a.slab = (uint32_t)slab;
         a.slabp = slab_padded((uint32_t)slab);
         a.dslabp = make_fastdiv(a.slabp);
-        a.r = r;
+        a.r = f.r;
+        a.fc = f.fc;
+        a.fm = f.fm;
+        a.per_node = f.per_node;
         a.s0 = s0;
         a.count = count;
         a.src = src;
@@ -86,14 +100,23 @@ inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const Rate
     };
     for (uint32_t s0 = 0; s0 < ns; s0 += K) {
         const uint32_t n = std::min(K, ns - s0);
-        add((int32_t)s0, n, full_in, ring_head, 0);
+        add((int32_t)s0, n, full_in, ring_head, 0, f0);
         const uint32_t lo = std::max<uint32_t>(s0 == 0 ? 0 : s0 - 1, 1), hi = s0 + n - 1;
         if (hi > lo)
-            add((int32_t)lo, hi - lo, ring_only, full_out, 1);
+            add((int32_t)lo, hi - lo, ring_only, full_out, 1, f1);
     }
     // slabs ns - 1 and 0 (one slab when ns == 1)
-    add((int32_t)ns - 1, std::min<uint32_t>(2, ns), ring_head, full_out, 1);
+    add((int32_t)ns - 1, std::min<uint32_t>(2, ns), ring_head, full_out, 1, f1);
     return out;
+}
+
+// A pair under one uniform force (r.fx, r.fy, r.fz).
+inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const RatesD &r, uint32_t K, void *lat_in,
+                                             void *lat_out, void *ring, void *head)
+{
+    const ForceSpec u = {r.fx, r.fy, r.fz, nullptr, 0.0, 0.0, 0.0};
+    const StepForce f = {r, u, u, 0};
+    return block_schedule(g, axis, f, f, K, lat_in, lat_out, ring, head);
 }
 
 } // namespace d3q19

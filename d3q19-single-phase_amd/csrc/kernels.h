@@ -183,6 +183,9 @@ __host__ __device__ inline PlaneRef plane_write_head(const PlaneStore &st, uint3
 // t % slabp of the launch's slab t / slabp: slabs are padded to whole waves
 // (slabp = slab rounded up to a multiple of 64), so every wave lies in one
 // slab and the slab's buffers and offsets are the same for all its lanes.
+// per_node: the body force is evaluated at every node as in step mode 16
+// (fc for the collision, fm for the fused fields u = j + F/2); otherwise it
+// is r.fx, r.fy, r.fz for both.
 struct BlockArgs {
     Geom g;
     uint32_t ns, nr;     // slabs along the sweep axis, rows per slab
@@ -193,6 +196,8 @@ struct BlockArgs {
     uint32_t count;
     PlaneStore src, dst;
     RatesD r;
+    ForceSpec fc, fm;
+    int per_node;
     int nt;
 };
 

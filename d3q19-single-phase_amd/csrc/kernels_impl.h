@@ -151,17 +151,24 @@ __device__ __forceinline__ void node_force(const ForceSpec &s, const Geom &g, ui
     fz = s.fz;
     if (s.tab == nullptr)
         return;
+    // every table value loads at once (both bands' y factors), so no load
+    // waits for the band flag; the selects pick what the band needs
     const double *xt = s.tab + 3 * ix;
-    const double band = xt[0];
-    if (band == 0.0)
-        return;
-    const double sg = band == 1.0 ? 1.0 : -1.0;
-    const double *yt = s.tab + 3 * g.lx + 4 * iy + (band == 1.0 ? 0 : 2);
+    const double *yt = s.tab + 3 * g.lx + 4 * iy;
     const double *zt = s.tab + 3 * g.lx + 4 * g.ly + 2 * iz;
-    const double cy = yt[0], sy = yt[1], cz = zt[0], sz = zt[1];
-    fx = (((sg * s.a) * xt[1]) * cy) * cz;
-    fy = s.fy * (1.0 + (((-sg * s.b) * xt[2]) * sy) * cz);
-    fz = (((sg * s.c) * xt[2]) * cy) * sz;
+    const double band = xt[0], x1 = xt[1], x2 = xt[2];
+    const double c1 = yt[0], s1 = yt[1], c2 = yt[2], s2 = yt[3];
+    const double cz = zt[0], sz = zt[1];
+    const bool first = band == 1.0;
+    const double sg = first ? 1.0 : -1.0;
+    const double cy = first ? c1 : c2, sy = first ? s1 : s2;
+    const double gx = (((sg * s.a) * x1) * cy) * cz;
+    const double gy = s.fy * (1.0 + (((-sg * s.b) * x2) * sy) * cz);
+    const double gz = (((sg * s.c) * x2) * cy) * sz;
+    const bool in_band = band != 0.0;
+    fx = in_band ? gx : fx;
+    fy = in_band ? gy : fy;
+    fz = in_band ? gz : fz;
 }
 
 template <int NB>
@@ -349,7 +356,9 @@ __device__ __forceinline__ void block_store(const PlaneRef &d, uint32_t e, const
 template <int P>
 constexpr int plane_block() { return P == 1 ? 512 : 256; }
 
-template <int P, int AXIS, int NT>
+// FP: the body force per node (mode 16 of k_step: a.fc for the collision,
+// a.fm for the fused fields), with the same arithmetic as k_step<P, 16>.
+template <int P, int AXIS, int NT, int FP>
 __global__ void __launch_bounds__(plane_block<P>()) k_step_planes(BlockArgs a)
 {
     using T = typename Prec<P>::T;
@@ -371,15 +380,37 @@ __global__ void __launch_bounds__(plane_block<P>()) k_step_planes(BlockArgs a)
     block_gather<P, AXIS>(pl, l, f, std::make_integer_sequence<int, 19>{});
 
     Rates<T> rt = rates_as<T>(a.r);
+    T fmx = rt.fx, fmy = rt.fy, fmz = rt.fz; // force in u = j + F/2 of the fused fields
+    if constexpr (FP != 0) {
+        const uint32_t iy = AXIS == 2 ? r : sb, iz = AXIS == 2 ? sb : r;
+        double x, y, z;
+        node_force(a.fc, a.g, ix, iy, iz, x, y, z);
+        rt.fx = (T)x;
+        rt.fy = (T)y;
+        rt.fz = (T)z;
+        node_force(a.fm, a.g, ix, iy, iz, x, y, z);
+        fmx = (T)x;
+        fmy = (T)y;
+        fmz = (T)z;
+    }
     T rho9, ux9, uy9, uz9;
     macro_sums(f, rho9, ux9, uy9, uz9);
-    ux9 = ux9 + rt.fx / T(2.); // collision.f90:415-417
-    uy9 = uy9 + rt.fy / T(2.);
-    uz9 = uz9 + rt.fz / T(2.);
-    if constexpr (P == 0)
+    const T jx = ux9, jy = uy9, jz = uz9;
+    ux9 = ux9 + fmx / T(2.); // collision.f90:415-417
+    uy9 = uy9 + fmy / T(2.);
+    uz9 = uz9 + fmz / T(2.);
+    if constexpr (P == 0) {
         mrt_collide(f, rho9, ux9, uy9, uz9, rt);
-    else
-        mrt_collide_delta(f, rho9, ux9, uy9, uz9, T(0), T(0), T(0), T(0), rt);
+    } else {
+        T djx = T(0), djy = T(0), djz = T(0);
+        if constexpr (FP != 0) {
+            // fields formed with fm, collision with fc
+            djx = ux9 - (jx + rt.fx / T(2.));
+            djy = uy9 - (jy + rt.fy / T(2.));
+            djz = uz9 - (jz + rt.fz / T(2.));
+        }
+        mrt_collide_delta(f, rho9, ux9, uy9, uz9, T(0), djx, djy, djz, rt);
+    }
 
     const PlaneRef d = plane_write(a.dst, sb);
     const uint32_t e = r * (uint32_t)d.row_stride + ix; // head and ring rows share the stride
@@ -997,18 +1028,17 @@ hipError_t Launch<P>::step_planes(const BlockArgs &a, int axis, hipStream_t s)
 {
     constexpr uint32_t PB = plane_block<P>();
     const dim3 grid((uint32_t)(((uint64_t)a.slabp * a.count + PB - 1) / PB)), blk(PB);
-    if (axis == 2) {
-        if (a.nt)
-            hipLaunchKernelGGL((k_step_planes<P, 2, 1>), grid, blk, 0, s, a);
-        else
-            hipLaunchKernelGGL((k_step_planes<P, 2, 0>), grid, blk, 0, s, a);
-    } else if (axis == 1) {
-        if (a.nt)
-            hipLaunchKernelGGL((k_step_planes<P, 1, 1>), grid, blk, 0, s, a);
-        else
-            hipLaunchKernelGGL((k_step_planes<P, 1, 0>), grid, blk, 0, s, a);
-    } else {
-        return hipErrorInvalidValue;
+    const int v = (axis == 2 ? 4 : axis == 1 ? 0 : -8) + (a.nt ? 2 : 0) + (a.per_node ? 1 : 0);
+    switch (v) {
+    case 0: hipLaunchKernelGGL((k_step_planes<P, 1, 0, 0>), grid, blk, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_step_planes<P, 1, 0, 1>), grid, blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_step_planes<P, 1, 1, 0>), grid, blk, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_step_planes<P, 1, 1, 1>), grid, blk, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_step_planes<P, 2, 0, 0>), grid, blk, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((k_step_planes<P, 2, 0, 1>), grid, blk, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((k_step_planes<P, 2, 1, 0>), grid, blk, 0, s, a); break;
+    case 7: hipLaunchKernelGGL((k_step_planes<P, 2, 1, 1>), grid, blk, 0, s, a); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

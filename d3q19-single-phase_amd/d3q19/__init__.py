@@ -63,6 +63,7 @@ _SIGS = {
     "lbm_download_f": [_H, _D],
     "lbm_set_force": [_H, ctypes.c_double, ctypes.c_double, ctypes.c_double],
     "lbm_forcingp": [_H, ctypes.c_int, ctypes.c_double],
+    "lbm_run_forcingp": [_H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double],
     "lbm_set_macros": [_H, _D, _D, _D, _D],
     "lbm_initpop": [_H],
     "lbm_collision_mrt": [_H],
@@ -240,7 +241,15 @@ class Channel:
     def run_forcingp(self, istep0, nsteps, npforcing, force_in_y=None):
         """Main loop of Channel-Flow/main.f90:142-161 with the perturbation
         forcing switched on (the commented block at :148-153): FORCINGP before
-        the collision while istep < npforcing, FORCING at istep == npforcing."""
+        the collision while istep < npforcing, FORCING at istep == npforcing.
+        One library call (lbm_run_forcingp), pairs of steps as blocked sweeps."""
+        fiy = self.p["force_in_y"] if force_in_y is None else force_in_y
+        _call("lbm_run_forcingp", self._h, int(istep0), int(nsteps), int(npforcing), float(fiy),
+              float(self.p["force_in_y"] * self.p["force_mag"]))
+
+    def run_forcingp_stepwise(self, istep0, nsteps, npforcing, force_in_y=None):
+        """The same loop as separate FORCINGP / FORCING, collision_MRT and
+        macrovar calls, one step at a time (the reference's call sequence)."""
         for istep in range(istep0 + 1, istep0 + nsteps + 1):
             if istep < npforcing:
                 self.FORCINGP(istep, force_in_y)

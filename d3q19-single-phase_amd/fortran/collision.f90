@@ -35,6 +35,17 @@
       call lbm_check(lbm_run(lbm, int(n, c_int)), 'run_steps')
       end subroutine run_steps
 
+      ! n main-loop steps from istep with the perturbation forcing switched on
+      ! (FORCINGP before each collision while istep < npforcing, FORCING at
+      ! npforcing; main.f90:148-153), as one library call.
+      subroutine run_forcingp_steps(n)
+      use channel_vars
+      implicit none
+      integer, intent(in) :: n
+      call lbm_check(lbm_run_forcingp(lbm, int(istep - 1, c_int), int(n, c_int), int(npforcing, c_int), &
+                     force_in_y, force_in_y*force_mag), 'run_forcingp_steps')
+      end subroutine run_forcingp_steps
+
       ! macrovar with the fields copied into the host arrays rho, ux, uy, uz.
       subroutine fetch_fields
       use channel_vars

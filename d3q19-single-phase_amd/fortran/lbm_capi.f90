@@ -74,6 +74,14 @@
           real(c_double), value :: force_in_y
         end function
 
+        integer(c_int) function lbm_run_forcingp(h, istep0, nsteps, npforcing, force_in_y, fy_forcing) &
+            bind(C, name="lbm_run_forcingp")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          integer(c_int), value :: istep0, nsteps, npforcing
+          real(c_double), value :: force_in_y, fy_forcing
+        end function
+
         integer(c_int) function lbm_set_macros(h, rho, ux, uy, uz) bind(C, name="lbm_set_macros")
           import :: c_int, c_ptr
           type(c_ptr), value :: h, rho, ux, uy, uz

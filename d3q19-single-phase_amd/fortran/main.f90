@@ -70,15 +70,13 @@
       istep = istep0 + 1
       do while (istep <= istep0 + nsteps)
         if (istep <= npforcing) then
-          ! perturbation forcing until npforcing (the block at main.f90:148-153)
-          if (istep < npforcing) then
-            call FORCINGP
-          else
-            call FORCING
-          end if
-          call collision_MRT
-          call macrovar
-          nrun = 1
+          ! perturbation forcing until npforcing (the block at main.f90:148-153):
+          ! FORCINGP or FORCING, collision_MRT and macrovar per step, batched
+          ! up to npforcing and the next diag/statistc2 step
+          nrun = min(npforcing, istep0 + nsteps) - istep + 1
+          if (ndiag > 0) nrun = min(nrun, ndiag - mod(istep - 1, ndiag))
+          if (nstat > 0) nrun = min(nrun, nstat - mod(istep - 1, nstat))
+          call run_forcingp_steps(nrun)
         else
           nrun = istep0 + nsteps - istep + 1
           if (ndiag > 0) nrun = min(nrun, ndiag - mod(istep - 1, ndiag))

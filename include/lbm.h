@@ -100,6 +100,19 @@ int lbm_set_force(lbm_channel *h, double fx, double fy, double fz);
  * force arrays).  Not combined with particles (main.f90:148). */
 int lbm_forcingp(lbm_channel *h, int istep, double force_in_y);
 
+/* nsteps iterations of the main loop with the perturbation forcing switched
+ * on (main.f90:142-161 with the block at :148-153), for istep = istep0 + 1 ..
+ * istep0 + nsteps: lbm_forcingp(istep, force_in_y) while istep < npforcing,
+ * lbm_set_force(0, fy_forcing, 0) at istep == npforcing (FORCING, with
+ * fy_forcing = force_in_y*force_mag), then collision_MRT and macrovar.  Pairs
+ * of steps under one uniform force run as one blocked sweep (see lbm_run);
+ * FORCINGP steps run one by one, or also in pairs with the force evaluated
+ * per node when LBM_BLOCK_FORCED=1 at lbm_create (measured slower).  Results
+ * equal those calls one by one, bit for bit.  No reference counterpart as
+ * one call: it batches the driver's loop. */
+int lbm_run_forcingp(lbm_channel *h, int istep0, int nsteps, int npforcing, double force_in_y,
+                     double fy_forcing);
+
 /* Sets the macroscopic fields the next collision reads, as the driver does by
  * assigning the module arrays (initial.f90:94-115 initvel, :32 rho = 0).
  * Any pointer may be NULL to keep that field. */

@@ -125,3 +125,43 @@ def test_forcingp_not_combined_with_particles():
         ch.FORCINGP(10)
         with pytest.raises(d3q19.LbmError, match="uniform force"):
             ch.particles_create(np.array([[16.0, 12.0, 12.0]]), rad=4.0)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16"])
+@pytest.mark.parametrize("axis,forced", [("z", True), ("y", True), ("y", False)])
+def test_batched_forcingp_loop_equals_call_by_call(monkeypatch, prec, axis, forced):
+    """lbm_run_forcingp against FORCINGP / FORCING, collision_MRT and macrovar
+    called one step at a time: bit for bit in FP64; the same arithmetic in
+    FP32/FP16 storage, whose FMA contraction may differ between kernels.  With
+    LBM_BLOCK_FORCED=1 the FORCINGP steps run as blocked pairs with the force
+    evaluated per node; by default they run one by one and the steps after
+    npforcing in pairs.  The range starts on an odd step, switches to FORCING
+    inside a pair (npforcing = 35) and ends on a single step."""
+    lx, ly, lz = 20, 18, 14
+    print(f"batched forcingp {prec} axis={axis} forced={forced}", flush=True)
+    _, _, f0 = seeded_pair(lx, ly, lz, seed=35, noise=1e-4)
+    monkeypatch.setenv("LBM_BLOCK_AXIS", axis)
+    monkeypatch.setenv("LBM_BLOCK_PLANES", "4")
+    if forced:
+        monkeypatch.setenv("LBM_BLOCK_FORCED", "1")
+    chans = [d3q19.Channel(lx, ly, lz, prec) for _ in range(2)]
+    for k in ("LBM_BLOCK_PLANES", "LBM_BLOCK_AXIS", "LBM_BLOCK_FORCED"):
+        monkeypatch.delenv(k, raising=False)
+    a, b = chans
+    with a, b:
+        assert a.block_info() == (axis, 4)
+        for c in (a, b):
+            c.upload_f(f0)
+            c.FORCING()
+            c.macrovar()
+        a.run_forcingp(20, 25, 35)
+        b.run_forcingp_stepwise(20, 25, 35)
+        pa, pb = a.download_post(), b.download_post()
+        if prec == "fp64":
+            assert_bitwise(pa, pb, "batched vs call-by-call FORCINGP loop")
+            assert a.mass() == b.mass()
+        else:
+            # a contraction difference moves a stored value by an ulp: 6e-8
+            # relative in FP32, 5e-4 in FP16; a wrong gather moves it by O(1)
+            tol = 1e-6 if prec == "fp32" else 2e-3
+            assert np.max(np.abs(pa - pb)) <= tol * np.max(np.abs(pb))
