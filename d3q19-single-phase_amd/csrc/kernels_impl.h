@@ -23,12 +23,14 @@
 namespace d3q19 {
 
 constexpr int BLOCK = 256;
-// Threads per block of the step kernels k_step and k_step_closure: 512
-// measured faster than 256 on the 192x768x384 channel in FP16 storage
-// (71 600 against 68 100 MLUPS) and FP64 (18 320 against 18 150), and with
-// the fused closure on the 96x384x192 channel (22 000 against 21 200); 128
-// slower.
-constexpr int STEP_BLOCK = 512;
+// Threads per block of the step kernels k_step and k_step_closure, measured
+// against each other on one box at a time: 512 for FP64 (18 320 against
+// 18 150 MLUPS with 256 on the 192x768x384 channel), FP16 storage (71 600
+// against 68 100) and the fused closure (22 000 against 21 200 on the
+// 96x384x192 channel); 256 for the plain FP32 step (39 220-39 330 against
+// 37 990-38 800 with 512 on the 2048x1024x512 channel).  128 was slower.
+template <int P, int MODE>
+constexpr int step_block() { return (P == 1 && (MODE & 8) == 0) ? 256 : 512; }
 
 // Element `node` of a population row, addressed as the row base plus a 32-bit
 // byte offset when that cannot wrap: FP16 storage, whose rows hold fewer than
@@ -202,7 +204,8 @@ __device__ __forceinline__ void step_node(const StepArgs &a)
     // every lane stays to the end (a lane past the lattice works on the last
     // node, a particle node on its own slots) and only active lanes store.
     constexpr bool WAVE = (MODE & 8) != 0;
-    const uint32_t n0 = node_index<STEP_BLOCK>(a.g);
+    constexpr int NB = step_block<P, MODE>();
+    const uint32_t n0 = node_index<NB>(a.g);
     bool active = n0 < a.g.N;
     if constexpr (!WAVE) {
         if (!active)
@@ -215,7 +218,7 @@ __device__ __forceinline__ void step_node(const StepArgs &a)
         active = active && a.solid[n] < 0;
     __shared__ __attribute__((aligned(16))) _Float16 wl[WAVE ? MLP_LDS : 8];
     if constexpr (WAVE) {
-        mlp_stage(a.mlp, wl, threadIdx.x, STEP_BLOCK);
+        mlp_stage(a.mlp, wl, threadIdx.x, NB);
         __syncthreads();
     }
     const Nbr b = make_nbr(n, a.g);
@@ -301,7 +304,7 @@ __device__ __forceinline__ void step_node(const StepArgs &a)
 }
 
 template <int P, int MODE>
-__global__ void __launch_bounds__(STEP_BLOCK) k_step(StepArgs a)
+__global__ void __launch_bounds__((step_block<P, MODE>())) k_step(StepArgs a)
 {
     step_node<P, MODE>(a);
 }
@@ -309,7 +312,7 @@ __global__ void __launch_bounds__(STEP_BLOCK) k_step(StepArgs a)
 // The closure modes, held to 128 registers so that four waves per SIMD keep
 // the lattice traffic in flight while others run the network.
 template <int P, int MODE>
-__global__ void __launch_bounds__(STEP_BLOCK, 4) k_step_closure(StepArgs a)
+__global__ void __launch_bounds__((step_block<P, MODE>()), 4) k_step_closure(StepArgs a)
 {
     step_node<P, MODE>(a);
 }
@@ -970,52 +973,53 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
             return hipGetLastError();
         }
     }
-    const dim3 grid((uint32_t)((a.g.N + STEP_BLOCK - 1) / STEP_BLOCK)), blk(STEP_BLOCK);
+    // one launch geometry per kernel's block size (step_block)
+    auto geo = [&](int nb) { return std::make_pair(dim3((uint32_t)((a.g.N + nb - 1) / nb)), dim3(nb)); };
     switch (mode) {
     case 0:
-        hipLaunchKernelGGL((k_step<P, 0>), grid, blk, 0, s, a);
+        hipLaunchKernelGGL((k_step<P, 0>), geo(step_block<P, 0>()).first, geo(step_block<P, 0>()).second, 0, s, a);
         break;
     case 1:
-        hipLaunchKernelGGL((k_step<P, 1>), grid, blk, 0, s, a);
+        hipLaunchKernelGGL((k_step<P, 1>), geo(step_block<P, 1>()).first, geo(step_block<P, 1>()).second, 0, s, a);
         break;
     case 2:
-        hipLaunchKernelGGL((k_step<P, 2>), grid, blk, 0, s, a);
+        hipLaunchKernelGGL((k_step<P, 2>), geo(step_block<P, 2>()).first, geo(step_block<P, 2>()).second, 0, s, a);
         break;
     case 4:
-        hipLaunchKernelGGL((k_step<P, 4>), grid, blk, 0, s, a);
+        hipLaunchKernelGGL((k_step<P, 4>), geo(step_block<P, 4>()).first, geo(step_block<P, 4>()).second, 0, s, a);
         break;
     case 5:
-        hipLaunchKernelGGL((k_step<P, 5>), grid, blk, 0, s, a);
+        hipLaunchKernelGGL((k_step<P, 5>), geo(step_block<P, 5>()).first, geo(step_block<P, 5>()).second, 0, s, a);
         break;
     case 6:
-        hipLaunchKernelGGL((k_step<P, 6>), grid, blk, 0, s, a);
+        hipLaunchKernelGGL((k_step<P, 6>), geo(step_block<P, 6>()).first, geo(step_block<P, 6>()).second, 0, s, a);
         break;
     case 8:
         if constexpr (P == 0)
             return hipErrorInvalidValue;
         else
-            hipLaunchKernelGGL((k_step_closure<P, 8>), grid, blk, 0, s, a);
+            hipLaunchKernelGGL((k_step_closure<P, 8>), geo(step_block<P, 8>()).first, geo(step_block<P, 8>()).second, 0, s, a);
         break;
     case 12:
         if constexpr (P == 0)
             return hipErrorInvalidValue;
         else
-            hipLaunchKernelGGL((k_step_closure<P, 12>), grid, blk, 0, s, a);
+            hipLaunchKernelGGL((k_step_closure<P, 12>), geo(step_block<P, 12>()).first, geo(step_block<P, 12>()).second, 0, s, a);
         break;
     case 16:
-        hipLaunchKernelGGL((k_step<P, 16>), grid, blk, 0, s, a);
+        hipLaunchKernelGGL((k_step<P, 16>), geo(step_block<P, 16>()).first, geo(step_block<P, 16>()).second, 0, s, a);
         break;
     case 17:
-        hipLaunchKernelGGL((k_step<P, 17>), grid, blk, 0, s, a);
+        hipLaunchKernelGGL((k_step<P, 17>), geo(step_block<P, 17>()).first, geo(step_block<P, 17>()).second, 0, s, a);
         break;
     case 18:
-        hipLaunchKernelGGL((k_step<P, 18>), grid, blk, 0, s, a);
+        hipLaunchKernelGGL((k_step<P, 18>), geo(step_block<P, 18>()).first, geo(step_block<P, 18>()).second, 0, s, a);
         break;
     case 24:
         if constexpr (P == 0)
             return hipErrorInvalidValue;
         else
-            hipLaunchKernelGGL((k_step_closure<P, 24>), grid, blk, 0, s, a);
+            hipLaunchKernelGGL((k_step_closure<P, 24>), geo(step_block<P, 24>()).first, geo(step_block<P, 24>()).second, 0, s, a);
         break;
     default:
         return hipErrorInvalidValue;
