@@ -22,6 +22,8 @@
 
 #include <stdint.h>
 
+#include <stdlib.h>
+
 #include <algorithm>
 #include <vector>
 
@@ -108,6 +110,57 @@ inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const Step
     // slabs ns - 1 and 0 (one slab when ns == 1)
     add((int32_t)ns - 1, std::min<uint32_t>(2, ns), ring_head, full_out, 1, f1);
     return out;
+}
+
+// Ring of the two-step blocked sweep: about 150 MB of the 256 MB last-level
+// cache (13 slots of 11.2 MB measured best on the 192x768x384 channel in
+// FP64 and FP32; larger rings are partly evicted by the streams through the
+// same cache)
+constexpr double BLOCK_RING_BYTES = 150e6;
+constexpr long BLOCK_MIN_SLABS = 4;
+constexpr long BLOCK_PREF_SLABS = 8;
+
+// The sweep lbm_create chooses for a lattice of elem-byte values: axis 2 (z)
+// or 1 (y) and K slabs per chunk (0: single steps only).  K is the number of
+// slabs whose ring of K + 2 fits ring_bytes (at most 16 and the number of
+// slabs).  The axis with the larger slabs is taken when its chunk holds at
+// least pref_slabs, else the other one on that condition, else whichever
+// holds at least min_slabs, else none.  axis_env / planes_env (may be null)
+// are the values of LBM_BLOCK_AXIS ("z" or "y") and LBM_BLOCK_PLANES (K, 0
+// turns the sweep off), which override the choice; blocked = false (FP16
+// storage, whose step is bound by arithmetic) makes the default K zero.
+struct BlockChoice {
+    int axis;
+    uint32_t K;
+};
+
+inline BlockChoice block_choice(const Geom &g, uint32_t elem, bool blocked, double ring_bytes, long min_slabs,
+                                long pref_slabs, const char *axis_env, const char *planes_env)
+{
+    auto chunk = [&](int axis) -> long {
+        const BlockGeom bg = block_geom(g, axis);
+        const double slab_bytes = (double)g.lx * bg.nr * 19.0 * (double)elem;
+        return (long)(ring_bytes / slab_bytes) - 2;
+    };
+    // z slabs (lx * ly nodes) are the larger ones when ly >= lz
+    const int big = g.ly >= g.lz ? 2 : 1, small = 3 - big;
+    int axis = big;
+    if (chunk(big) < pref_slabs)
+        axis = chunk(small) >= pref_slabs ? small : (chunk(big) >= min_slabs ? big : small);
+    if (axis_env)
+        axis = (axis_env[0] == 'y' && axis_env[1] == 0) ? 1 : 2;
+    long kb = blocked ? chunk(axis) : 0;
+    if (kb < min_slabs)
+        kb = 0;
+    if (planes_env) {
+        char *end = nullptr;
+        kb = strtol(planes_env, &end, 10);
+    }
+    const BlockGeom bg = block_geom(g, axis);
+    kb = std::min<long>(std::min<long>(kb, 16), (long)bg.ns);
+    if (kb < 1 || (uint64_t)slab_padded(g.lx * bg.nr) * (uint64_t)kb >= (1ull << 31))
+        kb = 0;
+    return BlockChoice{axis, (uint32_t)kb};
 }
 
 // A pair under one uniform force (r.fx, r.fy, r.fz).

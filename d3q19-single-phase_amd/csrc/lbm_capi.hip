@@ -56,13 +56,6 @@ enum MacroState { FROM_CUR = 0, FROM_PREV = 1, ARRAYS = 2 };
 constexpr uint32_t RED_BLOCKS = 2048;   // partial sums per reduction
 constexpr uint32_t PLANE_CHUNKS = 512;  // row chunks for plane sums
 constexpr uint64_t COPY_NODES = 1u << 20; // nodes per upload/download chunk
-// Ring of the two-step blocked sweep: about 150 MB of the 256 MB last-level
-// cache (13 slots of 11.2 MB measured best on the 192x768x384 channel in
-// FP64 and FP32; larger rings are partly evicted by the streams through the
-// same cache)
-constexpr double BLOCK_RING_BYTES = 150e6;
-constexpr long BLOCK_MIN_SLABS = 4;
-constexpr long BLOCK_PREF_SLABS = 8;
 
 } // namespace
 
@@ -504,13 +497,12 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         // the 192x768x384 channel: 64.6 % vs 69.2 % of HBM in FP64)
         const char *k = getenv("LBM_STEP_KERNEL");
         h->runs = (k && strcmp(k, "runs") == 0) ? 1 : 0;
-        // two-step blocked sweep: chunks of K slabs whose ring of K + 2 slabs
-        // stays within BLOCK_RING_BYTES.  The axis with the larger slabs is
-        // taken when its chunk holds at least BLOCK_PREF_SLABS slabs, else
-        // the other one on the same condition, else the one whose chunk
-        // holds at least BLOCK_MIN_SLABS; with fewer the sweep is off (it
-        // measured slower than single steps: the 2048x1024x512 channel's
-        // 40 MB y slabs allow two, 33 900 against 38 200 MLUPS).  On the
+        // two-step blocked sweep (blocked.h, block_choice): chunks of K slabs
+        // whose ring of K + 2 slabs stays within BLOCK_RING_BYTES, along the
+        // axis with the larger slabs when a chunk of those holds at least
+        // BLOCK_PREF_SLABS, else the other; below BLOCK_MIN_SLABS the sweep
+        // is off (it measured slower than single steps: the 2048x1024x512
+        // channel's 40 MB y slabs allow two, 33 900 against 38 200 MLUPS).  On the
         // 192x768x384 channel this picks 11.2 MB slabs, 11 to a chunk, in
         // both FP64 (y: 19 770 MLUPS against 19 110 for z slabs, 5 to a
         // chunk) and FP32 (z: 39 880 against 39 460 with 12 to a chunk).
@@ -518,35 +510,17 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         // off).  Not by default for FP16 storage, whose step is bound by
         // arithmetic (measured 60 800 MLUPS blocked against 68 100
         // single-step).
-        auto chunk = [&](int axis) -> long {
-            const BlockGeom bg = block_geom(h->g, axis);
-            const double slab_bytes = (double)h->g.lx * bg.nr * 19.0 * (double)h->elem;
-            return (long)(BLOCK_RING_BYTES / slab_bytes) - 2;
-        };
-        // z slabs (lx * ly nodes) are the larger ones when ly >= lz
-        const int big = h->g.ly >= h->g.lz ? 2 : 1, small = 3 - big;
-        int axis = big;
-        if (chunk(big) < BLOCK_PREF_SLABS)
-            axis = chunk(small) >= BLOCK_PREF_SLABS ? small : (chunk(big) >= BLOCK_MIN_SLABS ? big : small);
-        const char *ka = getenv("LBM_BLOCK_AXIS");
-        h->baxis = ka ? (strcmp(ka, "y") == 0 ? 1 : 2) : axis;
-        long kb = c.precision == LBM_FP16 ? 0 : chunk(h->baxis);
-        if (kb < BLOCK_MIN_SLABS)
-            kb = 0;
-        const char *kp = getenv("LBM_BLOCK_PLANES");
-        if (kp)
-            kb = strtol(kp, nullptr, 10);
+        const BlockChoice bc = block_choice(h->g, h->elem, c.precision != LBM_FP16, BLOCK_RING_BYTES,
+                                            BLOCK_MIN_SLABS, BLOCK_PREF_SLABS, getenv("LBM_BLOCK_AXIS"),
+                                            getenv("LBM_BLOCK_PLANES"));
+        h->baxis = bc.axis;
+        h->bK = bc.K;
         // LBM_BLOCK_FORCED=1: pairs under a force evaluated per node too.
         // Off by default: on the 192x768x384 channel such pairs took 3.33 ms
         // per FP64 step against 3.10 ms for single steps (the per-node force
         // costs the latency-bound sweep what the HBM-bound step hides).
         const char *kf = getenv("LBM_BLOCK_FORCED");
         h->bforced = (kf && strcmp(kf, "1") == 0) ? 1 : 0;
-        const BlockGeom bg = block_geom(h->g, h->baxis);
-        kb = std::min<long>(std::min<long>(kb, 16), (long)bg.ns);
-        if (kb < 1 || (uint64_t)slab_padded(h->g.lx * bg.nr) * (uint64_t)kb >= (1ull << 31))
-            kb = 0;
-        h->bK = (uint32_t)kb;
     }
     fields_from_cur(h);
     h->fresh = true;

@@ -6,7 +6,8 @@
 // value is the tag the single-step pull would read (P_s(x - c_i)_i, or
 // P_s(x)_opp(i) at a wall), so no ring slot is read after being overwritten;
 // every node and population of the output lattice holds its step-t+2 tag.
-// Launches nothing on a GPU.
+// Also: the default sweep (block_choice) for the tuned workloads and under
+// the environment overrides.  Launches nothing on a GPU.
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -212,8 +213,41 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
     return true;
 }
 
+// lbm_create's default sweep (block_choice) for the workloads it was tuned on.
+static bool check_choice(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t elem, bool blocked, const char *axis_env,
+                         const char *planes_env, int want_axis, uint32_t want_K)
+{
+    const Geom g = geom(lx, ly, lz);
+    const BlockChoice c = block_choice(g, elem, blocked, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, BLOCK_PREF_SLABS,
+                                       axis_env, planes_env);
+    if (c.axis != want_axis || c.K != want_K) {
+        fprintf(stderr, "%ux%ux%u elem %u: sweep axis %d K %u, want axis %d K %u\n", lx, ly, lz, elem, c.axis, c.K,
+                want_axis, want_K);
+        return false;
+    }
+    return true;
+}
+
 int main()
 {
+    // 192x768x384: y slabs, 11 to a chunk, in FP64; z slabs, 11, in FP32;
+    // FP16 storage single steps; the environment overrides
+    bool choice = check_choice(192, 768, 384, 8, true, nullptr, nullptr, 1, 11) &&
+                  check_choice(192, 768, 384, 4, true, nullptr, nullptr, 2, 11) &&
+                  check_choice(192, 768, 384, 2, false, nullptr, nullptr, 2, 0) &&
+                  check_choice(192, 768, 384, 8, true, "z", nullptr, 2, 4) &&
+                  check_choice(192, 768, 384, 8, true, nullptr, "0", 1, 0) &&
+                  check_choice(192, 768, 384, 2, false, "y", "12", 1, 12) &&
+                  // 2048x1024x512 in FP32: 80 MB z and 40 MB y slabs, no sweep
+                  check_choice(512, 2048, 1024, 4, true, nullptr, nullptr, 1, 0) &&
+                  // small grids: the larger slabs, at most 16 and at most the slab count
+                  check_choice(32, 64, 32, 8, true, nullptr, nullptr, 2, 16) &&
+                  check_choice(12, 10, 6, 8, true, nullptr, nullptr, 2, 6) &&
+                  check_choice(12, 6, 10, 8, true, nullptr, nullptr, 1, 6);
+    if (!choice)
+        return 1;
+    printf("ok sweep choice\n");
+
     const uint32_t cases[][4] = {{4, 3, 1, 2},  {4, 3, 2, 2},  {5, 4, 3, 2},  {6, 5, 7, 2},   {5, 3, 9, 4},
                                  {4, 4, 8, 4},  {3, 5, 10, 3}, {8, 6, 16, 1}, {8, 6, 17, 16}, {7, 5, 33, 8},
                                  {2, 1, 5, 2},  {6, 2, 4, 1},  {9, 7, 12, 5}, {4, 3, 3, 16}};
