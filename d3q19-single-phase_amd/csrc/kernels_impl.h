@@ -353,19 +353,20 @@ __device__ __forceinline__ void block_store(const PlaneRef &d, uint32_t e, const
         ((base[(uint64_t)Is * d.stride + e] = Prec<P>::template to_S<Is>(f[Is])), ...);
 }
 
-// Threads per block of the sweep: 512 in FP32 (measured 1.4 % faster than
-// 256 on the 192x768x384 channel), 256 otherwise (FP64: 512 measured 1 %
-// slower, 128 3 % slower).
-template <int P>
-constexpr int plane_block() { return P == 1 ? 512 : 256; }
+// Threads per block of the sweep, measured on the 192x768x384 channel: 512
+// in FP32 (1.4 % faster than 256); in FP64 256 for the step-t launches (NT
+// = 0) and 512 for the step-t+1 launches (0.7 % faster than 256 for both;
+// 512 for both 1 % slower, 128 3 % slower).
+template <int P, int NT>
+constexpr int plane_block() { return P == 1 ? 512 : (NT ? 512 : 256); }
 
 // FP: the body force per node (mode 16 of k_step: a.fc for the collision,
 // a.fm for the fused fields), with the same arithmetic as k_step<P, 16>.
 template <int P, int AXIS, int NT, int FP>
-__global__ void __launch_bounds__(plane_block<P>()) k_step_planes(BlockArgs a)
+__global__ void __launch_bounds__((plane_block<P, NT>())) k_step_planes(BlockArgs a)
 {
     using T = typename Prec<P>::T;
-    const uint32_t t = blockIdx.x * plane_block<P>() + threadIdx.x;
+    const uint32_t t = blockIdx.x * plane_block<P, NT>() + threadIdx.x;
     // the wave's slab, uniform across its lanes (slabs are padded to whole
     // waves), so the slab buffers and offsets below stay in scalar registers
     const uint32_t sl = __builtin_amdgcn_readfirstlane(fdiv(t, a.dslabp));
@@ -1030,7 +1031,7 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
 template <int P>
 hipError_t Launch<P>::step_planes(const BlockArgs &a, int axis, hipStream_t s)
 {
-    constexpr uint32_t PB = plane_block<P>();
+    const uint32_t PB = a.nt ? plane_block<P, 1>() : plane_block<P, 0>();
     const dim3 grid((uint32_t)(((uint64_t)a.slabp * a.count + PB - 1) / PB)), blk(PB);
     const int v = (axis == 2 ? 4 : axis == 1 ? 0 : -8) + (a.nt ? 2 : 0) + (a.per_node ? 1 : 0);
     switch (v) {
