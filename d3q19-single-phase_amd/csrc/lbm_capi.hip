@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "blocked.h"
@@ -57,7 +58,18 @@ constexpr uint32_t RED_BLOCKS = 2048;   // partial sums per reduction
 constexpr uint32_t PLANE_CHUNKS = 512;  // row chunks for plane sums
 constexpr uint64_t COPY_NODES = 1u << 20; // nodes per upload/download chunk
 
+// Scratch of lbm_mass_sum_noalloc: the per-block partial sums of the mass
+// reduction, 16 KiB per GPU, held until the process ends.  Host code never passes `scratch` itself
+// to a kernel: it uses the device address that hipGetSymbolAddress returns
+// for the current device (scratch_ptr), kept per device.
+constexpr uint32_t NOALLOC_PARTIALS = RED_BLOCKS;
+constexpr int MAX_DEVICES = 64;
+double *g_scratch_ptr[MAX_DEVICES];
+std::mutex g_scratch_mu;
+
 } // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS];
 
 struct lbm_channel {
     lbm_config cfg;
@@ -477,7 +489,20 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (c.device < 0 || c.device >= ndev)
         return fail(LBM_ERR_ARG, "lbm_create: device %d not present (%d devices)", c.device, ndev);
+    if (c.device >= MAX_DEVICES)
+        return fail(LBM_ERR_ARG, "lbm_create: device %d beyond the library's %d-device table", c.device, MAX_DEVICES);
     HIP_TRY(hipSetDevice(c.device));
+    {
+        // device address of this GPU's copy of `scratch`, once per device
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[c.device]) {
+            void *p = nullptr;
+            HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            if (!p)
+                return fail(LBM_ERR_HIP, "lbm_create: hipGetSymbolAddress(scratch) gave a null address");
+            g_scratch_ptr[c.device] = (double *)p;
+        }
+    }
 
     lbm_channel *h = new lbm_channel();
     memset(h, 0, sizeof(*h));
@@ -1045,6 +1070,43 @@ int lbm_mass_sum(lbm_channel *h, double *mass)
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(mass, h->red_out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int lbm_mass_sum_noalloc(lbm_channel *h, double *mass_dev)
+{
+    CHECK_H(h);
+    if (!mass_dev)
+        return fail(LBM_ERR_ARG, "lbm_mass_sum_noalloc: null output");
+    // a kernel store through a host pointer would fault the GPU: accept only
+    // device memory of this handle's GPU
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, mass_dev) != hipSuccess || at.type != hipMemoryTypeDevice ||
+        at.device != h->cfg.device) {
+        (void)hipGetLastError();
+        return fail(LBM_ERR_ARG, "lbm_mass_sum_noalloc: output %p is not device memory of device %d", (void *)mass_dev,
+                    h->cfg.device);
+    }
+    double *part = g_scratch_ptr[h->cfg.device];
+    if (!part)
+        return fail(LBM_ERR_STATE, "lbm_mass_sum_noalloc: scratch of device %d not set up", h->cfg.device);
+    SumArgs a;
+    a.g = h->g;
+    a.L = h->lat[h->cur];
+    a.rho = nullptr;
+    a.solid = solid_of(h);
+    a.partials = part;
+    // the same grid as lbm_mass_sum, so the two sums are equal bit for bit
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(RED_BLOCKS, (h->g.N + 255) / 256);
+    if (grid > NOALLOC_PARTIALS) // cannot happen while RED_BLOCKS bounds the grid
+        return fail(LBM_ERR_ARG, "lbm_mass_sum_noalloc: %u partials exceed the %u-double scratch", grid,
+                    NOALLOC_PARTIALS);
+    int rc = dispatch_err(PREC_CALL(h, mass_partials, a, grid, h->stream), "mass reduction");
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_sum_columns, dim3(1), dim3(64), 0, h->stream, (const double *)part, grid, 1u, mass_dev,
+                       (double *)nullptr);
+    HIP_TRY(hipGetLastError());
     return LBM_OK;
 }
 

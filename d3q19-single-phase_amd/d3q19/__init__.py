@@ -72,6 +72,7 @@ _SIGS = {
     "lbm_avedensity": [_H, ctypes.POINTER(ctypes.c_double)],
     "lbm_run": [_H, ctypes.c_int],
     "lbm_mass_sum": [_H, ctypes.POINTER(ctypes.c_double)],
+    "lbm_mass_sum_noalloc": [_H, _D],
     "lbm_statistc2": [_H, ctypes.c_double, ctypes.c_double, _D],
     "lbm_stats_accumulate": [_H],
     "lbm_stats_read": [_H, ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_longlong)],
@@ -334,6 +335,11 @@ class Channel:
         v = ctypes.c_double()
         _call("lbm_mass_sum", self._h, ctypes.byref(v))
         return v.value
+
+    def mass_noalloc(self, mass_dev_ptr):
+        """lbm_mass_sum_noalloc: enqueue the mass sum into one double of device
+        memory at address mass_dev_ptr (no allocation, no synchronisation)."""
+        _call("lbm_mass_sum_noalloc", self._h, ctypes.c_void_p(int(mass_dev_ptr)))
 
     def statistc2(self):
         """Plane profiles of Channel-Flow/saveload.f90:1348-1502; array (lx, 15)."""

@@ -159,6 +159,18 @@ int lbm_run(lbm_channel *h, int nsteps);
  * streaming).  Synchronises. */
 int lbm_mass_sum(lbm_channel *h, double *mass);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same FP64 sum as lbm_mass_sum, bit for bit, written
+ * to mass_dev (one double of device memory on the handle's GPU) on the
+ * handle's stream.  Returns after enqueue, without synchronising, and
+ * allocates nothing: its per-block partial sums (at most 2048 doubles) live in
+ * a 16 KiB static device array per GPU, shared by every handle on that GPU.
+ * Calls that overlap on one GPU (concurrent host threads, or handles whose
+ * streams run at the same time) race on it.  A host pointer for mass_dev is
+ * rejected with LBM_ERR_ARG. */
+int lbm_mass_sum_noalloc(lbm_channel *h, double *mass_dev);
+
 /* Plane statistics of Channel-Flow/saveload.f90:1382-1470 (statistc2) for the
  * current macroscopic fields: out is (15, lx) column-major, the 15 columns
  * written at saveload.f90:1492.  ustar and ystar are the driver's values
