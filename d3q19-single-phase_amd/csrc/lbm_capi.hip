@@ -104,6 +104,7 @@ struct lbm_channel {
     int baxis;          // its sweep axis: 2 = z (slabs lx*ly), 1 = y (slabs lx*lz)
     int bforced;        // lbm_run_forcingp also pairs steps whose force varies over the nodes
     void *bring, *bhead; // its slab ring and head buffer (allocated on first use)
+    bool ave_pending;   // lbm_run_particles: avedensity due once the next bounce-back has run
 };
 
 struct lbm_trainer {
@@ -224,7 +225,14 @@ int compute_fields(lbm_channel *h, int which)
     a.f = h->fm;
     a.rho_off = h->rho_off;
     a.errbits = nullptr;
-    return dispatch_err(PREC_CALL(h, macro, a, 0, h->stream), "macrovar");
+    rc = dispatch_err(PREC_CALL(h, macro, a, 0, h->stream), "macrovar");
+    if (rc || !h->part)
+        return rc;
+    // macrovar's solid branch (collision.f90:420-459), with the current particle state
+    const Part *pt = h->part;
+    return dispatch_err(part_solid_fields(pt->pg, pt->ps, h->rho_off, h->mac[0], h->mac[1], h->mac[2], h->mac[3],
+                                          h->stream),
+                        "macrovar inside particles");
 }
 
 // macrovar of the current lattice (with the force arrays as they are now)
@@ -1337,8 +1345,12 @@ int lbm_particles_create(lbm_channel *h, const lbm_particle_config *pc, const do
         return fail(LBM_ERR_STATE, "lbm_particles_create: particles already exist on this handle");
     if (h->fc.tab)
         return fail(LBM_ERR_STATE, "lbm_particles_create: particles need a uniform force (FORCING, not FORCINGP)");
-    if (pc->npart < 1 || !(pc->rad > 0.0) || !(pc->rhopart > 0.0) || pc->reserved != 0)
-        return fail(LBM_ERR_ARG, "lbm_particles_create: need npart >= 1, rad > 0, rhopart > 0, reserved = 0");
+    if (pc->npart < 1 || !(pc->rad > 0.0) || !(pc->rhopart > 0.0) || pc->nave < 0)
+        return fail(LBM_ERR_ARG, "lbm_particles_create: need npart >= 1, rad > 0, rhopart > 0, nave >= 0");
+    if (!(pc->mingap > 0.0) || !(pc->mingap_w > 0.0) || !(pc->visc >= 0.0) ||
+        (pc->visc > 0.0 && !(pc->lub_cut > pc->lub_min && pc->lub_min > 0.0)))
+        return fail(LBM_ERR_ARG, "lbm_particles_create: need mingap, mingap_w > 0, visc >= 0 and, with visc > 0, "
+                                 "lub_cut > lub_min > 0");
     const double lx = h->g.lx, ly = h->g.ly, lz = h->g.lz;
     for (int p = 0; p < pc->npart; p++) {
         const double *X = pos + 3 * p;
@@ -1366,11 +1378,18 @@ int lbm_particles_create(lbm_channel *h, const lbm_particle_config *pc, const do
     pg.volume = 4.0 / 3.0 * pi * pc->rad * pc->rad * pc->rad;
     pg.mass = pc->rhopart * pg.volume;
     pg.inertia = 0.4 * pg.mass * pc->rad * pc->rad;
+    pg.rhopart = pc->rhopart;
     pg.mingap = pc->mingap;
     pg.mingap_w = pc->mingap_w;
-    pg.stf = pc->stf;
-    pg.stf_w = pc->stf_w;
+    pg.stf0 = pc->stf0;
+    pg.stf1 = pc->stf1;
+    pg.stf0_w = pc->stf0_w;
+    pg.stf1_w = pc->stf1_w;
+    pg.visc = pc->visc;
+    pg.lub_cut = pc->lub_cut;
+    pg.lub_min = pc->lub_min;
     pg.pressure_force = pc->pressure_force;
+    pg.nave = pc->nave;
     pg.fx = h->fc.fx;
     pg.fy = h->fc.fy;
     pg.fz = h->fc.fz;
@@ -1389,6 +1408,10 @@ int lbm_particles_create(lbm_channel *h, const lbm_particle_config *pc, const do
     A((void **)&ps.FH, 3 * np * sizeof(double));
     A((void **)&ps.TQ, 3 * np * sizeof(double));
     A((void **)&ps.FC, 3 * np * sizeof(double));
+    A((void **)&ps.FL, 3 * np * sizeof(double));
+    A((void **)&ps.FHprev, 3 * np * sizeof(double));
+    A((void **)&ps.TQprev, 3 * np * sizeof(double));
+    A((void **)&ps.VWnew, 6 * np * sizeof(double));
     A((void **)&ps.nsteps_done, sizeof(int));
     A((void **)&ps.solid, h->g.N * sizeof(int));
     A((void **)&ps.link_node, nl * sizeof(int));
@@ -1410,7 +1433,7 @@ int lbm_particles_create(lbm_channel *h, const lbm_particle_config *pc, const do
     HIP_TRY(hipMemcpyAsync(ps.X + 3 * np, pos, 3 * np * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(ps.V, vel ? vel : zero.data(), 3 * np * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(ps.W, omg ? omg : zero.data(), 3 * np * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    for (double *d : {ps.TH, ps.Aprev, ps.ALprev, ps.FH, ps.TQ, ps.FC})
+    for (double *d : {ps.TH, ps.Aprev, ps.ALprev, ps.FH, ps.TQ, ps.FC, ps.FL, ps.FHprev, ps.TQprev})
         HIP_TRY(hipMemcpyAsync(d, zero.data(), 3 * np * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemsetAsync(ps.nsteps_done, 0, sizeof(int), h->stream));
     HIP_TRY(hipMemsetAsync(ps.overflow, 0, sizeof(int), h->stream));
@@ -1418,6 +1441,7 @@ int lbm_particles_create(lbm_channel *h, const lbm_particle_config *pc, const do
     HIP_TRY(hipMemsetAsync(ps.fill_count, 0, np * sizeof(int), h->stream));
     HIP_TRY(hipMemsetAsync(ps.link_count, 0, np * sizeof(int), h->stream));
     h->part = pt;
+    h->ave_pending = false;
     rc = dispatch_err(part_classify(pg, ps, 1, h->stream), "beads_links");
     if (rc)
         return rc;
@@ -1489,19 +1513,33 @@ int lbm_beads_move(lbm_channel *h)
     return dispatch_err(part_move(pt->pg, pt->ps, h->stream), "beads_move");
 }
 
-int lbm_run_particles(lbm_channel *h, int nsteps)
+int lbm_run_particles(lbm_channel *h, int istep0, int nsteps)
 {
     CHECK_P(h);
     if (nsteps < 0)
         return fail(LBM_ERR_ARG, "lbm_run_particles: nsteps %d < 0", nsteps);
+    const int nave = h->part->pg.nave;
     for (int s = 0; s < nsteps; s++) {
         int rc = lbm_beads_collision(h);
-        if (!rc)
-            rc = launch_step(h);
+        if (rc)
+            return rc;
+        // avedensity after a step that is a multiple of nave (main.f90:163-167),
+        // taken once this bounce-back has completed the fluid nodes'
+        // populations; the collision below subtracts the mean
+        if (h->ave_pending) {
+            rc = lbm_avedensity(h, nullptr);
+            if (rc)
+                return rc;
+            h->ave_pending = false;
+        }
+        rc = launch_step(h);
         if (rc)
             return rc;
         fields_from_cur(h); // macrovar
         h->rho_off = 0.0;
+        const long long istep = (long long)istep0 + s + 1;
+        if (nave > 0 && istep % nave == 0)
+            h->ave_pending = true;
         rc = lbm_beads_move(h);
         if (!rc)
             rc = lbm_beads_links(h);
@@ -1514,14 +1552,14 @@ int lbm_run_particles(lbm_channel *h, int nsteps)
 }
 
 int lbm_particles_get(lbm_channel *h, double *pos, double *vel, double *omg, double *force, double *torque,
-                      double *contact)
+                      double *contact, double *lub)
 {
     CHECK_P(h);
     const PartState &ps = h->part->ps;
     const size_t b = 3 * (size_t)h->part->pg.np * sizeof(double);
-    const double *src[6] = {ps.X, ps.V, ps.W, ps.FH, ps.TQ, ps.FC};
-    double *dst[6] = {pos, vel, omg, force, torque, contact};
-    for (int k = 0; k < 6; k++)
+    const double *src[7] = {ps.X, ps.V, ps.W, ps.FH, ps.TQ, ps.FC, ps.FL};
+    double *dst[7] = {pos, vel, omg, force, torque, contact, lub};
+    for (int k = 0; k < 7; k++)
         if (dst[k])
             HIP_TRY(hipMemcpyAsync(dst[k], src[k], b, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));

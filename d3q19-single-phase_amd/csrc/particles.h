@@ -14,10 +14,19 @@
 //   refill          equilibrium at the local wall velocity and the mean
 //                   density of the fluid neighbours (Peng et al., Comput.
 //                   Math. Appl. 72 (2016) 349, equilibrium variant)
-//   contact         repulsive force (1/eps)((zeta - h)/zeta)^2 for gaps h
-//                   below zeta, Glowinski et al., Int. J. Multiphase Flow
-//                   25 (1999) 755
-//   integration     FP64, Adams-Bashforth 2 for velocities, trapezoid for
+//   contact         Feng & Michaelides, J. Comput. Phys. 202 (2005) 20,
+//                   eq. 28: for the gap h = d - 2R below the range zeta,
+//                   stf0 ((zeta - h)/zeta)^2, plus stf1 (-h/zeta) on overlap,
+//                   along the line of centres; walls act as mirror-image
+//                   particles (stf0_w, stf1_w); the stiffness pairs of
+//                   Channel-Flow/para.f90:357-360
+//   lubrication     normal correction below the cut-off gap hc, Nguyen &
+//                   Ladd, Phys. Rev. E 66 (2002) 046708:
+//                   -6 pi mu a^2 (1/h - 1/hc) U_n, a = R/2 between two
+//                   spheres, R against a wall (flubp, var_inc.f90:150)
+//   integration     FP64, Adams-Bashforth 2 for velocities on the
+//                   hydrodynamic force and torque averaged over the last two
+//                   steps (Ladd, J. Fluid Mech. 271 (1994) 285), trapezoid for
 //                   positions and orientation
 // Node flags: solid[n] = -1 fluid, p >= 0 inside particle p, -2-p fluid node
 // uncovered by particle p in the last move and not yet refilled.
@@ -35,16 +44,21 @@ struct PartGeom {
     int np;
     int maxlink, maxfill;
     double R, R2;
-    double mass, inertia, volume;
-    double mingap, mingap_w, stf, stf_w;
+    double mass, inertia, volume, rhopart;
+    double mingap, mingap_w, stf0, stf1, stf0_w, stf1_w; // contact
+    double visc, lub_cut, lub_min;                        // lubrication (visc 0: off)
     double fx, fy, fz;  // fluid body force (pressure-gradient force on particles)
     int pressure_force; // 1: add volume * body force
+    int nave;           // run loop: avedensity after every step that is a multiple of nave (0: never)
 };
 
 struct PartState {
     double *X, *V, *W, *TH;        // position, velocity, angular velocity, orientation (3*np)
     double *Aprev, *ALprev;        // previous accelerations (3*np)
     double *FH, *TQ, *FC;          // hydrodynamic force, torque, contact force (3*np)
+    double *FL;                    // lubrication force (3*np)
+    double *FHprev, *TQprev;       // the previous step's hydrodynamic force and torque (3*np)
+    double *VWnew;                 // new velocities, then angular velocities, until committed (6*np)
     int *nsteps_done;              // 0 before the first move (Euler start)
     int *solid;                    // N
     int *link_node;                // np*maxlink
@@ -112,6 +126,9 @@ struct PartLaunch {
 };
 
 hipError_t part_classify(const PartGeom &pg, const PartState &ps, int first, hipStream_t s);
+// macrovar's solid branch: fields at nodes inside particles
+hipError_t part_solid_fields(const PartGeom &pg, const PartState &ps, double rho_off, double *rho, double *ux,
+                             double *uy, double *uz, hipStream_t s);
 hipError_t part_sum(const PartGeom &pg, const PartState &ps, hipStream_t s);
 hipError_t part_move(const PartGeom &pg, const PartState &ps, hipStream_t s);
 

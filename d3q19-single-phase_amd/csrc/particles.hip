@@ -374,67 +374,100 @@ __global__ void __launch_bounds__(SUMW) k_p_sum(PartGeom pg, PartState ps)
     }
 }
 
-// Contact forces and FP64 integration, one block of SUMW lanes per particle:
-// the lanes split the other particles (q = l, l+64, ...), their contact sums
-// are added by lane_tree, lane 0 adds the walls and integrates.
+// Contact force magnitude of Feng & Michaelides (2005) eq. 28 for the gap h
+// and range zeta: stf0 ((zeta - h)/zeta)^2 below zeta, plus stf1 (-h/zeta)
+// on overlap (h < 0).
+__device__ inline double contact_mag(double h, double zeta, double stf0, double stf1)
+{
+    if (!(h < zeta))
+        return 0.0;
+    const double r = (zeta - h) / zeta;
+    double m = stf0 * (r * r);
+    if (h < 0.0)
+        m = m + stf1 * ((-h) / zeta);
+    return m;
+}
+
+// Coefficient of the approach velocity in the lubrication correction of
+// Nguyen & Ladd (2002): -6 pi mu a^2 (1/h - 1/hc), the gap no smaller than
+// lub_min, zero at or beyond hc or without viscosity (mu = rho0 nu, rho0 = 1).
+__device__ inline double lub_coef(double h, double a, const PartGeom &pg)
+{
+    if (!(pg.visc > 0.0) || !(h < pg.lub_cut))
+        return 0.0;
+    const double he = h > pg.lub_min ? h : pg.lub_min;
+    return -6.0 * 3.14159265358979323846 * pg.visc * (a * a) * (1.0 / he - 1.0 / pg.lub_cut);
+}
+
+// Contact and lubrication forces and FP64 integration, one block of SUMW
+// lanes per particle: the lanes split the other particles (q = l, l+64, ...),
+// their sums are added by lane_tree, lane 0 adds the walls and integrates.
+// Every block reads the positions and velocities before the move; the new
+// ones are staged (Xold, VWnew) and swapped in by k_p_commit.
 __global__ void __launch_bounds__(SUMW) k_p_move(PartGeom pg, PartState ps, double *Xold)
 {
     const int p = blockIdx.x;
     const int l = threadIdx.x;
     const double Ly = (double)pg.g.ly, Lz = (double)pg.g.lz, Lx = (double)pg.g.lx;
-    const double *Xp = ps.X + 3 * p;
-    double fc[3] = {0.0, 0.0, 0.0};
+    const double *Xp = ps.X + 3 * p, *Vp = ps.V + 3 * p;
+    double f6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}; // contact, lubrication
     for (int q = l; q < pg.np; q += SUMW) {
         if (q == p)
             continue;
-        const double *Xq = ps.X + 3 * q;
+        const double *Xq = ps.X + 3 * q, *Vq = ps.V + 3 * q;
         const double ddx = Xp[0] - Xq[0];
         const double ddy = min_image(Xp[1] - Xq[1], Ly);
         const double ddz = min_image(Xp[2] - Xq[2], Lz);
         const double dist = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+        if (!(dist > 0.0))
+            continue;
         const double h = dist - 2.0 * pg.R;
-        if (h < pg.mingap && dist > 0.0) {
-            const double r = (pg.mingap - h) / pg.mingap;
-            const double mag = pg.stf * (r * r);
-            fc[0] = fc[0] + mag * (ddx / dist);
-            fc[1] = fc[1] + mag * (ddy / dist);
-            fc[2] = fc[2] + mag * (ddz / dist);
-        }
+        const double nx = ddx / dist, ny = ddy / dist, nz = ddz / dist;
+        const double mag = contact_mag(h, pg.mingap, pg.stf0, pg.stf1);
+        const double un = (Vp[0] - Vq[0]) * nx + (Vp[1] - Vq[1]) * ny + (Vp[2] - Vq[2]) * nz;
+        const double lub = lub_coef(h, 0.5 * pg.R, pg) * un;
+        f6[0] = f6[0] + mag * nx;
+        f6[1] = f6[1] + mag * ny;
+        f6[2] = f6[2] + mag * nz;
+        f6[3] = f6[3] + lub * nx;
+        f6[4] = f6[4] + lub * ny;
+        f6[5] = f6[5] + lub * nz;
     }
-    __shared__ double s[3][SUMW];
-    lane_tree<3>(fc, s);
+    __shared__ double s[6][SUMW];
+    lane_tree<6>(f6, s);
     if (l != 0)
         return;
-    fc[0] = s[0][0];
-    fc[1] = s[1][0];
-    fc[2] = s[2][0];
-    const double hlo = Xp[0] - pg.R;
-    if (hlo < pg.mingap_w) {
-        const double r = (pg.mingap_w - hlo) / pg.mingap_w;
-        fc[0] = fc[0] + pg.stf_w * (r * r);
-    }
-    const double hhi = Lx - Xp[0] - pg.R;
-    if (hhi < pg.mingap_w) {
-        const double r = (pg.mingap_w - hhi) / pg.mingap_w;
-        fc[0] = fc[0] - pg.stf_w * (r * r);
-    }
+    double fc[3] = {s[0][0], s[1][0], s[2][0]};
+    double fl[3] = {s[3][0], s[4][0], s[5][0]};
+    // walls at x = 0 and x = lx: the mirror image lies twice the distance away
+    const double dlo = Xp[0], dhi = Lx - Xp[0];
+    fc[0] = fc[0] + contact_mag(2.0 * dlo - 2.0 * pg.R, pg.mingap_w, pg.stf0_w, pg.stf1_w);
+    fc[0] = fc[0] - contact_mag(2.0 * dhi - 2.0 * pg.R, pg.mingap_w, pg.stf0_w, pg.stf1_w);
+    fl[0] = fl[0] + lub_coef(dlo - pg.R, pg.R, pg) * Vp[0];
+    fl[0] = fl[0] + lub_coef(dhi - pg.R, pg.R, pg) * Vp[0];
     const double fb[3] = {pg.fx, pg.fy, pg.fz};
     const bool first = (*ps.nsteps_done == 0);
     double Xn[3], Vn[3], Wn[3];
     for (int j = 0; j < 3; j++) {
-        ps.FC[3 * p + j] = fc[j];
-        double ftot = ps.FH[3 * p + j] + fc[j];
+        const int k = 3 * p + j;
+        ps.FC[k] = fc[j];
+        ps.FL[k] = fl[j];
+        const double fh = first ? ps.FH[k] : 0.5 * (ps.FH[k] + ps.FHprev[k]);
+        const double tq = first ? ps.TQ[k] : 0.5 * (ps.TQ[k] + ps.TQprev[k]);
+        ps.FHprev[k] = ps.FH[k];
+        ps.TQprev[k] = ps.TQ[k];
+        double ftot = fh + fc[j] + fl[j];
         if (pg.pressure_force)
             ftot = ftot + pg.volume * fb[j];
         const double a = ftot / pg.mass;
-        const double al = ps.TQ[3 * p + j] / pg.inertia;
-        const double v0 = ps.V[3 * p + j], w0 = ps.W[3 * p + j];
-        Vn[j] = first ? v0 + a : v0 + (1.5 * a - 0.5 * ps.Aprev[3 * p + j]);
-        Wn[j] = first ? w0 + al : w0 + (1.5 * al - 0.5 * ps.ALprev[3 * p + j]);
+        const double al = tq / pg.inertia;
+        const double v0 = ps.V[k], w0 = ps.W[k];
+        Vn[j] = first ? v0 + a : v0 + (1.5 * a - 0.5 * ps.Aprev[k]);
+        Wn[j] = first ? w0 + al : w0 + (1.5 * al - 0.5 * ps.ALprev[k]);
         Xn[j] = Xp[j] + 0.5 * (v0 + Vn[j]);
-        ps.TH[3 * p + j] = ps.TH[3 * p + j] + 0.5 * (w0 + Wn[j]);
-        ps.Aprev[3 * p + j] = a;
-        ps.ALprev[3 * p + j] = al;
+        ps.TH[k] = ps.TH[k] + 0.5 * (w0 + Wn[j]);
+        ps.Aprev[k] = a;
+        ps.ALprev[k] = al;
     }
     if (Xn[1] >= Ly)
         Xn[1] = Xn[1] - Ly;
@@ -451,16 +484,15 @@ __global__ void __launch_bounds__(SUMW) k_p_move(PartGeom pg, PartState ps, doub
         finite = finite && (Xn[j] - Xn[j] == 0.0) && (Vn[j] - Vn[j] == 0.0) && (Wn[j] - Wn[j] == 0.0);
     if (!finite)
         atomicOr(ps.overflow, 2);
-    // every thread has read all positions (contacts) before any is updated:
-    // positions go to Xold first and are swapped in by k_p_commit
     for (int j = 0; j < 3; j++) {
         Xold[3 * p + j] = Xn[j];
-        ps.V[3 * p + j] = Vn[j];
-        ps.W[3 * p + j] = Wn[j];
+        ps.VWnew[3 * p + j] = Vn[j];
+        ps.VWnew[3 * pg.np + 3 * p + j] = Wn[j];
     }
 }
 
-// Swap new positions in; the previous positions are kept in Xold.
+// Swap new positions in (the previous positions are kept in Xold) and commit
+// the new velocities.
 __global__ void k_p_commit(PartGeom pg, PartState ps, double *Xold)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,9 +502,42 @@ __global__ void k_p_commit(PartGeom pg, PartState ps, double *Xold)
         const double xn = Xold[3 * p + j];
         Xold[3 * p + j] = ps.X[3 * p + j];
         ps.X[3 * p + j] = xn;
+        ps.V[3 * p + j] = ps.VWnew[3 * p + j];
+        ps.W[3 * p + j] = ps.VWnew[3 * pg.np + 3 * p + j];
     }
     if (p == 0)
         *ps.nsteps_done = *ps.nsteps_done + 1;
+}
+
+// macrovar's solid branch (Channel-Flow/collision.f90:420-459): nodes inside
+// particle p carry the rigid-body velocity V + W x r (r from the centre,
+// minimum image in y and z) and rho = rhopart, less the pending avedensity
+// offset, which the reference subtracts from every node (:508).  One block
+// per particle strides over its box.
+__global__ void __launch_bounds__(PBLOCK) k_p_solid_fields(PartGeom pg, PartState ps, double rho_off, double *rho,
+                                                            double *ux, double *uy, double *uz)
+{
+    const int p = blockIdx.x;
+    const double *X = ps.X + 3 * p, *V = ps.V + 3 * p, *W = ps.W + 3 * p;
+    const Box b = sphere_box(pg, X, 0);
+    if (b.x1 < b.x0)
+        return;
+    const int nx = b.x1 - b.x0 + 1, ny = b.y1 - b.y0 + 1, nz = b.z1 - b.z0 + 1;
+    const int total = nx * ny * nz;
+    for (int t = threadIdx.x; t < total; t += PBLOCK) {
+        const int ix = b.x0 + t % nx;
+        const int iy = wrap(b.y0 + (t / nx) % ny, pg.g.ly);
+        const int iz = wrap(b.z0 + t / (nx * ny), pg.g.lz);
+        const uint32_t n = node_of(pg.g, ix, iy, iz);
+        if (ps.solid[n] != p)
+            continue;
+        double dx, dy, dz;
+        node_offset(pg, ix, iy, iz, X, dx, dy, dz);
+        ux[n] = V[0] + (W[1] * dz - W[2] * dy);
+        uy[n] = V[1] + (W[2] * dx - W[0] * dz);
+        uz[n] = V[2] + (W[0] * dy - W[1] * dx);
+        rho[n] = pg.rhopart - rho_off;
+    }
 }
 
 // Refill: nodes uncovered by particle p get the equilibrium of the wall
@@ -551,6 +616,13 @@ hipError_t part_classify(const PartGeom &pg, const PartState &ps, int first, hip
         hipLaunchKernelGGL(k_p_uncover, dim3(pg.np), dim3(PBLOCK), 0, s, pg, ps, ps.X + 3 * (size_t)pg.np);
     hipLaunchKernelGGL(k_p_mark, dim3(pg.np), dim3(PBLOCK), 0, s, pg, ps);
     hipLaunchKernelGGL(k_p_links, dim3(pg.np), dim3(PBLOCK), 0, s, pg, ps);
+    return hipGetLastError();
+}
+
+hipError_t part_solid_fields(const PartGeom &pg, const PartState &ps, double rho_off, double *rho, double *ux,
+                             double *uy, double *uz, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_p_solid_fields, dim3(pg.np), dim3(PBLOCK), 0, s, pg, ps, rho_off, rho, ux, uy, uz);
     return hipGetLastError();
 }
 

@@ -49,8 +49,9 @@ class LbmConfig(ctypes.Structure):
 class ParticleConfig(ctypes.Structure):
     """Mirror of struct lbm_particle_config in include/lbm.h."""
     _fields_ = [("npart", ctypes.c_int), ("maxlink", ctypes.c_int)] + [
-        (n, ctypes.c_double) for n in ("rad", "rhopart", "mingap", "mingap_w", "stf", "stf_w")] + [
-        ("pressure_force", ctypes.c_int), ("reserved", ctypes.c_int)]
+        (n, ctypes.c_double) for n in ("rad", "rhopart", "mingap", "mingap_w", "stf0", "stf1", "stf0_w", "stf1_w",
+                                       "visc", "lub_cut", "lub_min")] + [
+        ("pressure_force", ctypes.c_int), ("nave", ctypes.c_int)]
 
 
 # Every function the C ABI exports, with its argument types.
@@ -89,8 +90,8 @@ _SIGS = {
     "lbm_beads_filling": [_H],
     "lbm_beads_collision": [_H],
     "lbm_beads_move": [_H],
-    "lbm_run_particles": [_H, ctypes.c_int],
-    "lbm_particles_get": [_H, _D, _D, _D, _D, _D, _D],
+    "lbm_run_particles": [_H, ctypes.c_int, ctypes.c_int],
+    "lbm_particles_get": [_H, _D, _D, _D, _D, _D, _D, _D],
     "lbm_particles_links": [_H, ctypes.POINTER(ctypes.c_int), _D, _D, _D, _D],
     "lbm_particles_solid": [_H, _D],
     "lbm_closure_set": [_H, _D, ctypes.c_double],
@@ -403,17 +404,25 @@ class Channel:
 
     # -- rigid particles (beads_* of Channel-Flow/var_inc.f90:166-168) ----------
     def particles_create(self, pos, vel=None, omg=None, rad=5.0, rhopart=1.0, mingap=3.0, mingap_w=3.0,
-                         stf=0.025, stf_w=0.025, pressure_force=True, maxlink=0):
-        """Spheres at pos (npart, 3) in node coordinates (node ix at ix - 0.5, 1-based)."""
+                         stf0=0.025, stf1=0.002, stf0_w=0.025, stf1_w=0.002, visc=0.0, lub_cut=2.0 / 3.0,
+                         lub_min=0.01, pressure_force=True, nave=100, maxlink=0):
+        """Spheres at pos (npart, 3) in node coordinates (node ix at ix - 0.5, 1-based).
+        Contact: Feng & Michaelides (2005) eq. 28 with the stiffness pairs
+        stf0/stf1 (particles) and stf0_w/stf1_w (walls) of Channel-Flow/
+        para.f90:357-360.  Lubrication (Nguyen & Ladd 2002) when visc > 0.
+        run_particles takes avedensity after every nave-th step."""
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         pc = ParticleConfig()
         pc.npart, pc.maxlink = pos.shape[0], int(maxlink)
-        pc.rad, pc.rhopart, pc.mingap, pc.mingap_w, pc.stf, pc.stf_w = rad, rhopart, mingap, mingap_w, stf, stf_w
-        pc.pressure_force, pc.reserved = int(bool(pressure_force)), 0
+        pc.rad, pc.rhopart, pc.mingap, pc.mingap_w = rad, rhopart, mingap, mingap_w
+        pc.stf0, pc.stf1, pc.stf0_w, pc.stf1_w = stf0, stf1, stf0_w, stf1_w
+        pc.visc, pc.lub_cut, pc.lub_min = visc, lub_cut, lub_min
+        pc.pressure_force, pc.nave = int(bool(pressure_force)), int(nave)
         v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
         w = None if omg is None else np.ascontiguousarray(omg, dtype=np.float64).reshape(-1, 3)
         _call("lbm_particles_create", self._h, ctypes.byref(pc), _ptr(pos), _ptr(v), _ptr(w))
         self.npart = pos.shape[0]
+        self.istep = 0
 
     def beads_links(self):
         _call("lbm_beads_links", self._h)
@@ -427,13 +436,18 @@ class Channel:
     def beads_move(self):
         _call("lbm_beads_move", self._h)
 
-    def run_particles(self, nsteps):
-        _call("lbm_run_particles", self._h, int(nsteps))
+    def run_particles(self, nsteps, istep0=None):
+        """Steps istep0+1 .. istep0+nsteps of the particle-laden loop (istep0
+        defaults to the steps this object has run since particles_create)."""
+        if istep0 is None:
+            istep0 = self.istep
+        _call("lbm_run_particles", self._h, int(istep0), int(nsteps))
+        self.istep = int(istep0) + int(nsteps)
 
     def particles_get(self):
-        out = {k: np.empty((self.npart, 3)) for k in ("pos", "vel", "omg", "force", "torque", "contact")}
-        _call("lbm_particles_get", self._h, *[_ptr(out[k]) for k in ("pos", "vel", "omg", "force", "torque",
-                                                                   "contact")])
+        keys = ("pos", "vel", "omg", "force", "torque", "contact", "lub")
+        out = {k: np.empty((self.npart, 3)) for k in keys}
+        _call("lbm_particles_get", self._h, *[_ptr(out[k]) for k in keys])
         return out
 
     def particles_links(self):

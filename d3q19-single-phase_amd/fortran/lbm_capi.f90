@@ -26,8 +26,10 @@
       ! struct lbm_particle_config
       type, bind(C) :: lbm_particle_config
         integer(c_int) :: npart, maxlink
-        real(c_double) :: rad, rhopart, mingap, mingap_w, stf, stf_w
-        integer(c_int) :: pressure_force, reserved
+        real(c_double) :: rad, rhopart, mingap, mingap_w
+        real(c_double) :: stf0, stf1, stf0_w, stf1_w
+        real(c_double) :: visc, lub_cut, lub_min
+        integer(c_int) :: pressure_force, nave
       end type lbm_particle_config
 
       integer(c_int), parameter :: LBM_MLP_TOTAL = 1452
@@ -187,17 +189,17 @@
           type(c_ptr), value :: h
         end function
 
-        integer(c_int) function lbm_run_particles(h, nsteps) bind(C, name="lbm_run_particles")
+        integer(c_int) function lbm_run_particles(h, istep0, nsteps) bind(C, name="lbm_run_particles")
           import :: c_int, c_ptr
           type(c_ptr), value :: h
-          integer(c_int), value :: nsteps
+          integer(c_int), value :: istep0, nsteps
         end function
 
-        ! any of the six outputs may be c_null_ptr
-        integer(c_int) function lbm_particles_get(h, pos, vel, omg, force, torque, contact) &
+        ! any of the seven outputs may be c_null_ptr
+        integer(c_int) function lbm_particles_get(h, pos, vel, omg, force, torque, contact, lub) &
             bind(C, name="lbm_particles_get")
           import :: c_int, c_ptr
-          type(c_ptr), value :: h, pos, vel, omg, force, torque, contact
+          type(c_ptr), value :: h, pos, vel, omg, force, torque, contact, lub
         end function
 
         ! learned closure (FP32/FP16 storage only)

@@ -234,12 +234,20 @@ typedef struct lbm_particle_config {
     int maxlink;           /* per-particle cut-link capacity (0 = from the radius) */
     double rad;            /* radius (var_inc.f90:67 rad) */
     double rhopart;        /* particle / fluid density ratio (var_inc.f90:65) */
-    double mingap;         /* particle-particle contact range (var_inc.f90:67) */
+    double mingap;         /* particle-particle contact range zeta (var_inc.f90:67) */
     double mingap_w;       /* particle-wall contact range (var_inc.f90:67) */
-    double stf;            /* particle-particle contact stiffness 1/eps */
-    double stf_w;          /* particle-wall contact stiffness */
+    double stf0, stf1;     /* particle-particle contact force scales of Feng & Michaelides,
+                              J. Comput. Phys. 202 (2005) 20, eq. 28: approach and overlap
+                              terms (para.f90:357-358: 0.025, 0.002) */
+    double stf0_w, stf1_w; /* the same against the walls, mirror-image particles (para.f90:359-360) */
+    double visc;           /* kinematic viscosity for the lubrication correction (flubp,
+                              var_inc.f90:150; Nguyen & Ladd, Phys. Rev. E 66 (2002) 046708);
+                              0 = none */
+    double lub_cut;        /* lubrication cut-off gap hc (e.g. 2/3) */
+    double lub_min;        /* smallest gap the lubrication force uses (e.g. 0.01) */
     int pressure_force;    /* 1: each particle also feels volume x body force */
-    int reserved;          /* must be 0 */
+    int nave;              /* lbm_run_particles: avedensity after every step that is a multiple
+                              of nave (main.f90:163-167: 100); 0 = never */
 } lbm_particle_config;
 
 /* Creates the particle set (positions, velocities, angular velocities, each
@@ -261,18 +269,26 @@ int lbm_beads_filling(lbm_channel *h);
  * on every particle (FP64, summed link by link in a fixed order). */
 int lbm_beads_collision(lbm_channel *h);
 
-/* beads_move: contact forces and FP64 integration of the rigid-body motion
- * (Adams-Bashforth 2 velocities, trapezoidal positions). */
+/* beads_move: contact and lubrication forces and FP64 integration of the
+ * rigid-body motion (Adams-Bashforth 2 velocities on the hydrodynamic force
+ * and torque averaged over the last two steps, trapezoidal positions). */
 int lbm_beads_move(lbm_channel *h);
 
-/* nsteps of: beads_collision, collision_MRT, macrovar, beads_move,
- * beads_links, beads_filling. */
-int lbm_run_particles(lbm_channel *h, int nsteps);
+/* Steps istep0+1 .. istep0+nsteps of the particle-laden main loop:
+ * beads_collision, collision_MRT, macrovar, beads_move, beads_links,
+ * beads_filling.  After a step that is a multiple of the configuration's
+ * nave, avedensity (main.f90:163-167) is taken once the next step's
+ * bounce-back has completed the fluid populations, and that step's
+ * collision subtracts the mean; a mean still due at the end of a call is
+ * taken in the next call. */
+int lbm_run_particles(lbm_channel *h, int istep0, int nsteps);
 
 /* Particle state (each 3*npart, any pointer may be NULL).  force/torque are
- * the last momentum-exchange sums; contact the last contact forces. */
+ * the last momentum-exchange sums, contact the last contact forces, lub the
+ * last lubrication forces (the reference's fHIp and flubp,
+ * var_inc.f90:149-150). */
 int lbm_particles_get(lbm_channel *h, double *pos, double *vel, double *omg, double *force, double *torque,
-                      double *contact);
+                      double *contact, double *lub);
 
 /* Cut links: counts[npart]; node/dir/q hold npart*maxlink entries
  * (particle-major, counts[p] valid each).  *maxlink receives the capacity. */

@@ -28,7 +28,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liblbm_oracle.so")
+# D3Q19_ORACLE_LIB names another build of the same sources; only the test that
+# plants mistakes in the oracle (tests/test_oracle_mutations.py) sets it.
+_LIB_PATH = os.environ.get("D3Q19_ORACLE_LIB") or os.path.join(_HERE, "liblbm_oracle.so")
 
 _DBL = ctypes.c_double
 _INT = ctypes.c_int
@@ -211,13 +213,14 @@ class PGeom(ctypes.Structure):
     """Mirror of orc_pgeom (particles_oracle.h)."""
     _fields_ = [(n, _INT) for n in ("lx", "ly", "lz", "np", "maxlink", "maxfill", "pressure_force",
                                      "reserved")] + [
-        (n, _DBL) for n in ("R", "R2", "mass", "inertia", "volume", "mingap", "mingap_w", "stf", "stf_w",
-                            "fx", "fy", "fz")]
+        (n, _DBL) for n in ("R", "R2", "mass", "inertia", "volume", "rhopart", "mingap", "mingap_w",
+                            "stf0", "stf1", "stf0_w", "stf1_w", "visc", "lub_cut", "lub_min", "fx", "fy", "fz")]
 
 
 class PState(ctypes.Structure):
     """Mirror of orc_pstate (particles_oracle.h)."""
     _fields_ = [(n, ctypes.c_void_p) for n in ("X", "Xold", "V", "W", "TH", "Aprev", "ALprev", "FH", "TQ", "FC",
+                                               "FL", "FHprev", "TQprev",
                                                "solid", "link_node", "link_dir", "link_q", "link_count",
                                                "fill_node", "fill_count")] + [("nsteps_done", _INT)]
 
@@ -230,14 +233,20 @@ def _plib():
         L.orc_p_classify.restype = _INT
         L.orc_p_ibb.argtypes = [G, S, ctypes.c_void_p]
         L.orc_p_step.argtypes = [ctypes.POINTER(OrcParams), G, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                 _DBL, _DBL, _DBL]
+                                 _DBL, _DBL, _DBL, _DBL]
         L.orc_p_move.argtypes = [G, S]
         L.orc_p_refill.argtypes = [G, S, ctypes.c_void_p]
+        L.orc_p_rho_mean.argtypes = [G, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_p_rho_mean.restype = _DBL
+        L.orc_p_macrovar.argtypes = [G, S, ctypes.c_void_p, _DBL, _DBL, _DBL, _DBL] + [ctypes.c_void_p] * 4
+        L.orc_device_sum.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p]
+        L.orc_device_sum.restype = _DBL
         L._p_ready = True
     return L
 
 
-def particle_geometry(lx, ly, lz, npart, rad, rhopart=1.0, mingap=3.0, mingap_w=3.0, stf=0.025, stf_w=0.025,
+def particle_geometry(lx, ly, lz, npart, rad, rhopart=1.0, mingap=3.0, mingap_w=3.0, stf0=0.025, stf1=0.002,
+                      stf0_w=0.025, stf1_w=0.002, visc=0.0, lub_cut=2.0 / 3.0, lub_min=0.01,
                       pressure_force=1, force=(0.0, 0.0, 0.0), maxlink=0):
     """Particle constants formed exactly as lbm_particles_create forms them."""
     import math
@@ -252,7 +261,10 @@ def particle_geometry(lx, ly, lz, npart, rad, rhopart=1.0, mingap=3.0, mingap_w=
     g.volume = 4.0 / 3.0 * math.pi * rad * rad * rad
     g.mass = rhopart * g.volume
     g.inertia = 0.4 * g.mass * rad * rad
-    g.mingap, g.mingap_w, g.stf, g.stf_w = mingap, mingap_w, stf, stf_w
+    g.rhopart = rhopart
+    g.mingap, g.mingap_w = mingap, mingap_w
+    g.stf0, g.stf1, g.stf0_w, g.stf1_w = stf0, stf1, stf0_w, stf1_w
+    g.visc, g.lub_cut, g.lub_min = visc, lub_cut, lub_min
     g.fx, g.fy, g.fz = force
     return g
 
@@ -279,6 +291,7 @@ class OracleParticleChannel:
         self.V = z3() if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1).copy()
         self.W = z3() if omg is None else np.ascontiguousarray(omg, dtype=np.float64).reshape(-1).copy()
         self.TH, self.Aprev, self.ALprev, self.FH, self.TQ, self.FC = (z3() for _ in range(6))
+        self.FL, self.FHprev, self.TQprev = (z3() for _ in range(3))
         self.solid = np.full(self.N, -1, dtype=np.int32)
         self.link_node = np.zeros(np_ * geom.maxlink, dtype=np.int32)
         self.link_dir = np.zeros(np_ * geom.maxlink, dtype=np.int32)
@@ -287,11 +300,13 @@ class OracleParticleChannel:
         self.fill_node = np.zeros(np_ * geom.maxfill, dtype=np.int32)
         self.fill_count = np.zeros(np_, dtype=np.int32)
         self.s = PState()
-        for name in ("X", "Xold", "V", "W", "TH", "Aprev", "ALprev", "FH", "TQ", "FC", "solid", "link_node",
-                     "link_dir", "link_q", "link_count", "fill_node", "fill_count"):
+        for name in ("X", "Xold", "V", "W", "TH", "Aprev", "ALprev", "FH", "TQ", "FC", "FL", "FHprev", "TQprev",
+                     "solid", "link_node", "link_dir", "link_q", "link_count", "fill_node", "fill_count"):
             setattr(self.s, name, getattr(self, name).ctypes.data_as(ctypes.c_void_p))
         self.s.nsteps_done = 0
         self.force = (geom.fx, geom.fy, geom.fz)
+        self.rho_off = 0.0
+        self.ave_pending = False
         self.beads_links(first=True)
 
     def beads_links(self, first=False):
@@ -303,8 +318,27 @@ class OracleParticleChannel:
 
     def collision_MRT(self):
         _plib().orc_p_step(ctypes.byref(self.p), ctypes.byref(self.g), self.solid.ctypes.data_as(ctypes.c_void_p),
-                           _ptr(self.L[self.cur]), _ptr(self.L[1 - self.cur]), *self.force)
+                           _ptr(self.L[self.cur]), _ptr(self.L[1 - self.cur]), *self.force, self.rho_off)
         self.cur = 1 - self.cur
+        self.rho_off = 0.0
+
+    def rho_mean(self):
+        """avedensity's mean over the fluid nodes of the current lattice (device order)."""
+        return _plib().orc_p_rho_mean(ctypes.byref(self.g), self.solid.ctypes.data_as(ctypes.c_void_p),
+                                      _ptr(self.L[self.cur]))
+
+    def avedensity(self):
+        """Channel-Flow/collision.f90:487-513: the next collision takes rho - mean."""
+        self.rho_off = self.rho_mean()
+        return self.rho_off
+
+    def macrovar(self):
+        """Fields (rho, ux, uy, uz), each (lz, ly, lx), of the current state,
+        with the rigid-body velocity and rhopart inside particles."""
+        out = [np.empty(self.N) for _ in range(4)]
+        _plib().orc_p_macrovar(ctypes.byref(self.g), ctypes.byref(self.s), _ptr(self.L[self.cur]), *self.force,
+                               self.rho_off, *(_ptr(a) for a in out))
+        return tuple(a.reshape(self.g.lz, self.g.ly, self.g.lx) for a in out)
 
     def beads_move(self):
         _plib().orc_p_move(ctypes.byref(self.g), ctypes.byref(self.s))
@@ -312,11 +346,21 @@ class OracleParticleChannel:
     def beads_filling(self):
         _plib().orc_p_refill(ctypes.byref(self.g), ctypes.byref(self.s), _ptr(self.L[self.cur]))
 
-    def step(self, n=1):
-        """n of: beads_collision, collision_MRT, beads_move, beads_links, beads_filling."""
-        for _ in range(n):
+    def step(self, n=1, istep0=None, nave=100):
+        """n steps of the particle-laden main loop: beads_collision, collision_MRT,
+        macrovar, beads_move, beads_links, beads_filling.  With istep0 the
+        steps are istep0+1 .. istep0+n and, as Channel-Flow/main.f90:163-167
+        asks after every step that is a multiple of nave, avedensity's mean
+        is taken once the next step's bounce-back has completed the fluid
+        nodes' populations, and the next collision subtracts it."""
+        for k in range(n):
             self.beads_collision()
+            if self.ave_pending:
+                self.avedensity()
+                self.ave_pending = False
             self.collision_MRT()
+            if istep0 is not None and nave > 0 and (istep0 + k + 1) % nave == 0:
+                self.ave_pending = True
             self.beads_move()
             self.beads_links()
             self.beads_filling()

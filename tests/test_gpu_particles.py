@@ -72,13 +72,74 @@ def test_coupled_particle_run_bitwise():
         o.step(nsteps)
         st = ch.particles_get()
         for key, arr in (("pos", o.X), ("vel", o.V), ("omg", o.W), ("force", o.FH), ("torque", o.TQ),
-                         ("contact", o.FC)):
+                         ("contact", o.FC), ("lub", o.FL)):
             assert np.array_equal(bits(st[key].ravel()), bits(arr)), key
         assert np.array_equal(ch.particles_solid(), o.solid)
         post = ch.download_post().ravel()
         assert np.array_equal(bits(post), bits(o.post))
         # the particles moved across nodes, so refill was exercised
         assert np.max(np.abs(st["pos"] - pos)) > 1.0
+
+
+def test_turbulent_constants_coupled_run_bitwise_300_steps():
+    """The 32x64x32 channel at the turbulent constants (tau = 0.5108, rate
+    set 1, Re_tau = 180 body force) with three freely moving neutrally
+    buoyant spheres of radius 5 released at rest into the laminar start flow
+    (an impulsive start), lubrication on, avedensity after steps 100, 200 and
+    300 (the last one due in a later call): 300 coupled FP64 steps in two
+    calls, bitwise against the oracle, fields inside particles included."""
+    lx, ly, lz, rad, nsteps = 32, 64, 32, 5.0, 300
+    pos = workloads.sphere_positions(lx, ly, lz, 3, rad, seed=3)
+    ref = O.OracleChannel(lx, ly, lz)
+    ref.ux[:], ref.uy[:], ref.uz[:] = workloads.laminar_parabola_noise(lx, ly, lz, seed=31)
+    ref.initpop()
+    with d3q19.Channel(lx, ly, lz) as ch:
+        assert abs(1.0 / ch.p["s9"] - 0.5108) < 1e-12
+        ch.upload_f(ref.f)
+        ch.FORCING()
+        ch.macrovar()
+        L0 = ch.download_post()
+        visc = ch.p["visc"]
+        ch.particles_create(pos, rad=rad, visc=visc)
+        g = O.particle_geometry(lx, ly, lz, 3, rad, visc=visc, force=(0.0, ch.p["force_in_y"], 0.0))
+        o = O.OracleParticleChannel(O.para(lx, ly, lz), g, L0, pos)
+        ch.run_particles(100)   # the mean due after step 100 is taken by the next call
+        ch.run_particles(200)
+        o.step(nsteps, istep0=0)
+        assert o.ave_pending
+        st = ch.particles_get()
+        for key, arr in (("pos", o.X), ("vel", o.V), ("omg", o.W), ("force", o.FH), ("torque", o.TQ),
+                         ("contact", o.FC), ("lub", o.FL)):
+            assert np.array_equal(bits(st[key].ravel()), bits(arr)), key
+        assert np.array_equal(ch.particles_solid(), o.solid)
+        assert np.array_equal(bits(ch.download_post().ravel()), bits(o.post))
+        assert np.all(np.isfinite(o.post)) and np.max(np.abs(st["vel"])) > 1e-3
+        for name, a, b in zip(("rho", "ux", "uy", "uz"), ch.macrovar(download=True), o.macrovar()):
+            assert np.array_equal(bits(a), bits(b)), name
+
+
+def test_avedensity_mean_over_fluid_nodes_with_particles():
+    """avedensity with particles: the mean over fluid nodes only, in the
+    device's summation order, equals the oracle's restatement bit for bit."""
+    lx, ly, lz, rad = 32, 64, 32, 5.0
+    pos = workloads.sphere_positions(lx, ly, lz, 3, rad, seed=3)
+    ref = O.OracleChannel(lx, ly, lz)
+    ref.ux[:], ref.uy[:], ref.uz[:] = workloads.laminar_parabola_noise(lx, ly, lz, seed=33, uc=0.03)
+    ref.initpop()
+    with d3q19.Channel(lx, ly, lz) as ch:
+        ch.upload_f(ref.f)
+        ch.FORCING()
+        ch.macrovar()
+        L0 = ch.download_post()
+        ch.particles_create(pos, rad=rad)
+        g = O.particle_geometry(lx, ly, lz, 3, rad, force=(0.0, ch.p["force_in_y"], 0.0))
+        o = O.OracleParticleChannel(O.para(lx, ly, lz), g, L0, pos)
+        ch.run_particles(7)
+        o.step(7, istep0=0)
+        mean = ch.avedensity()
+        want = o.rho_mean()
+        assert np.array_equal(bits(np.array([mean])), bits(np.array([want])))
+        assert mean != 0.0
 
 
 def test_particle_run_fp32_is_close_to_fp64():
