@@ -19,8 +19,9 @@ kernel's cost).
 FP16 storage is the variant.
 
 `value` runs lbm_run's two-step blocked sweep (pairs of steps through a
-ring of z planes held in the Infinity Cache, bitwise equal to single
-steps); "single_step" is the same workload with the sweep off.
+ring of slabs held in the Infinity Cache -- y slabs in FP64, z slabs in
+FP32 -- bitwise equal to single steps); "single_step" is the same workload
+with the sweep off.
 
 "forcingp": the same step under the perturbation forcing FORCINGP
 (Channel-Flow/collision.f90:529-602), its force evaluated per node inside
@@ -33,8 +34,14 @@ of the nodes) bounds the time-to-statistics speed-up of the closure run.
 
 Roofline: the step kernel is HBM-bound; algorithmic traffic is one read and
 one write of the 19 populations per node (304 B FP64, 152 B FP32, 76 B FP16).
-`achieved` = algorithmic bytes per launch / mean kernel duration from HIP
-events recorded on the library's stream around every launch.
+`achieved` = algorithmic bytes per step / mean device time per step from HIP
+events recorded on the library's stream around every single step or blocked
+pair (halved); for the blocked sweep this is an effective bandwidth, since
+the Infinity Cache serves the intermediate lattice.  `traffic` is the
+rocprofv3 FETCH_SIZE + WRITE_SIZE figure between L2 and the memory fabric.
+
+"particles": the coupled step with 2000 spheres of radius 5 at the same
+turbulent constants (--particles 0 leaves it out).
 
 cpu_baseline: the CPU oracle's out-of-place schedule (same arithmetic as the
 reference, OpenMP over z planes) on the host cores, on a bounded sample.
@@ -127,7 +134,16 @@ def measure(prec, workload, steps, warmup, single_steps=False):
         "event_total_ms": total_ms,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": algo},
+                     "algorithmic_bytes": algo,
+                     # achieved: one lattice read and write per step (the single
+                     # step's HBM traffic) over the device time per step; a
+                     # blocked pair keeps its intermediate lattice in the
+                     # Infinity Cache, so this is an effective bandwidth
+                     "basis": "algorithmic bytes per step (read + write of 19 populations) / device time per step",
+                     # traffic: rocprofv3 FETCH_SIZE + WRITE_SIZE, measured
+                     # between L2 and the memory fabric: an upper bound on HBM
+                     # bytes when the Infinity Cache serves the ring
+                     "traffic_point": "L2-to-fabric (upper bound on HBM bytes)"},
         # change of the mean density fluctuation (sum of shifted populations / N)
         "mass_drift_per_node": (m1 - m0) / n,
         # lbm_run's pairing of steps: sweep axis and slabs per chunk (0: single steps)
@@ -175,27 +191,20 @@ def measure_closure(steps, warmup):
 def measure_particles(prec, workload, npart, rad, steps, warmup):
     """Coupled particle step (beads_collision, collision_MRT with solid nodes
     skipped, macrovar, beads_move, beads_links, beads_filling) with npart
-    seeded spheres of radius rad on the workload's grid.
-
-    The flow uses the laminar constants of para.f90:74-88 (tau ~ 3.4) with a
-    parabolic profile (U_c = 0.05) and the spheres start at the local fluid
-    velocity: at the turbulent branch's tau = 0.5108 a coupled run diverges
-    within ~150 steps (both rate sets 1 and 3, CPU oracle), which a timing
-    run must not depend on.  The kernels' work is set by the geometry (links,
-    refills per step), not by the regime."""
+    seeded neutrally buoyant spheres of radius rad on the workload's grid, at
+    the turbulent constants of the main workload (tau = 0.5108, rate set 1,
+    log-law start with its divergence-free perturbation), lubrication on.
+    The spheres start at the fluid velocity of the node nearest their
+    centre."""
     lx, ly, lz, _ = WORKLOADS[workload]
     n = lx * ly * lz
     pos = workloads.sphere_positions(lx, ly, lz, npart, rad, seed=3)
-    vel = np.zeros_like(pos)
-    vel[:, 1] = 0.05 * (1.0 - ((pos[:, 0] - lx / 2) / (lx / 2)) ** 2)
-    with d3q19.Channel(lx, ly, lz, prec, laminar=True) as ch:
-        ux, uy, uz = workloads.laminar_parabola_noise(lx, ly, lz, seed=1, uc=0.05, amp=0.0)
-        ch.set_macros(ux=ux, uy=uy, uz=uz)
+    with d3q19.Channel(lx, ly, lz, prec, rstar=RSTAR.get(workload)) as ch:
+        ux, uy, uz = init_channel(ch, lx, ly, lz)
+        ix, iy, iz = (np.floor(pos[:, k]).astype(int) for k in range(3))
+        vel = np.stack([a[iz, iy, ix] for a in (ux, uy, uz)], axis=1)
         del ux, uy, uz
-        ch.FORCING()
-        ch.initpop()
-        ch.macrovar()
-        ch.particles_create(pos, vel=vel, rad=rad)
+        ch.particles_create(pos, vel=vel, rad=rad, visc=ch.p["visc"])
         solid = int(np.count_nonzero(ch.particles_solid() >= 0))
         ch.run_particles(warmup)
         ch.sync()
@@ -203,8 +212,11 @@ def measure_particles(prec, workload, npart, rad, steps, warmup):
         ch.run_particles(steps)
         ch.sync()
         wall = time.perf_counter() - t0
-    return {"npart": npart, "radius": rad, "solid_fraction": round(solid / n, 5), "value": round(n * steps / wall / 1e6, 1),
-            "unit": "MLUPS", "ms_per_step": round(wall / steps * 1e3, 4)}
+        st = ch.particles_get()
+    return {"npart": npart, "radius": rad, "solid_fraction": round(solid / n, 5),
+            "value": round(n * steps / wall / 1e6, 1), "unit": "MLUPS", "ms_per_step": round(wall / steps * 1e3, 4),
+            "regime": "turbulent constants (tau 0.5108), spheres released at the local fluid velocity",
+            "max_particle_speed": round(float(np.abs(st["vel"]).max()), 5)}
 
 
 COPY_KERNEL_TAG = "MulFunctor"  # torch.mul(src, 2.0, out=dst): reads and writes 8 B per element
@@ -277,6 +289,19 @@ def traffic_summary(prec, workload, fetch_csv, write_csv):
             "traffic_over_algorithmic": (fetch + write) / algo}
 
 
+# The oracle's step against the compiled reference on one core of the build
+# container (Intel Xeon 2.7 GHz): 5.35 against 5.45 MLUPS on 192x384x96.
+PORT_VS_REFERENCE_1CORE = round(5.35 / 5.45, 3)
+
+
+def host_cores():
+    """Cores this process may use: its affinity set, capped by OMP_NUM_THREADS
+    when that is set (the GPU box gives one GPU's job 16)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(cap))) if cap.isdigit() and int(cap) > 0 else n
+
+
 def cpu_baseline(sample_steps, threads):
     """Oracle out-of-place step (collision + macrovar) on the full default grid."""
     import oracle
@@ -295,7 +320,9 @@ def cpu_baseline(sample_steps, threads):
     n = lx * ly * lz
     return {"value": round(n * sample_steps / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"{sample_steps} steps (collision_MRT + macrovar) of the full {lx}x{ly}x{lz} FP64 grid, "
-                      f"oracle out-of-place schedule, OpenMP {threads} threads, {dt:.1f} s"}
+                      f"oracle out-of-place schedule, OpenMP {threads} threads (every core this job may use), "
+                      f"{dt:.1f} s",
+            "port_vs_reference_1core": PORT_VS_REFERENCE_1CORE}
 
 
 def main():
@@ -310,7 +337,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-closure", action="store_true", help="skip the coarse closure-run throughput")
     ap.add_argument("--no-forcingp", action="store_true", help="skip the perturbation-forcing step throughput")
-    ap.add_argument("--particles", type=int, default=0, metavar="NPART",
+    ap.add_argument("--particles", type=int, default=2000, metavar="NPART",
                     help="also time the coupled particle step with NPART spheres of radius 5 (FP64 and FP32)")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--traffic-probe", metavar="PREC", choices=sorted(BYTES_PER_NODE),
@@ -383,16 +410,21 @@ def main():
         log(f"bench: closure coarse {c['mlups']:.1f} MLUPS")
 
     particles = None
-    if args.particles:
+    if args.particles and args.workload == "c2":
         particles = {}
         for prec in ("fp64", "fp32"):
             log(f"bench: {args.particles} particles, {prec}")
-            particles[prec] = measure_particles(prec, args.workload, args.particles, 5.0, args.steps, args.warmup)
-            log(f"bench: particles {prec} {particles[prec]['value']} MLUPS")
+            try:
+                particles[prec] = measure_particles(prec, args.workload, args.particles, 5.0, args.steps,
+                                                    args.warmup)
+                log(f"bench: particles {prec} {particles[prec]['value']} MLUPS")
+            except d3q19.LbmError as e:  # reported in the line, not fatal to the headline figure
+                particles[prec] = {"error": str(e)}
+                log(f"bench: particles {prec} failed: {e}")
 
     cpu = None
     if not args.no_cpu and rank == 0:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cores()
         log(f"bench: cpu baseline, {threads} threads")
         cpu = cpu_baseline(args.cpu_steps, threads)
 

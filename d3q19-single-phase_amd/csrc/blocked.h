@@ -20,6 +20,8 @@
 // launch over a ring of 2K + 2 slots measured slower.)
 #pragma once
 
+#include <string.h>
+
 #include <stdint.h>
 
 #include <stdlib.h>
@@ -127,11 +129,13 @@ constexpr long BLOCK_PREF_SLABS = 8;
 // least pref_slabs, else the other one on that condition, else whichever
 // holds at least min_slabs, else none.  axis_env / planes_env (may be null)
 // are the values of LBM_BLOCK_AXIS ("z" or "y") and LBM_BLOCK_PLANES (K, 0
-// turns the sweep off), which override the choice; blocked = false (FP16
-// storage, whose step is bound by arithmetic) makes the default K zero.
+// turns the sweep off), which override the choice; any other value of either
+// is reported in `bad`.  blocked = false (FP16 storage, whose step is bound by
+// arithmetic) makes the default K zero.
 struct BlockChoice {
     int axis;
     uint32_t K;
+    const char *bad; // null, or the variable whose value is invalid (the caller fails)
 };
 
 inline BlockChoice block_choice(const Geom &g, uint32_t elem, bool blocked, double ring_bytes, long min_slabs,
@@ -147,20 +151,29 @@ inline BlockChoice block_choice(const Geom &g, uint32_t elem, bool blocked, doub
     int axis = big;
     if (chunk(big) < pref_slabs)
         axis = chunk(small) >= pref_slabs ? small : (chunk(big) >= min_slabs ? big : small);
-    if (axis_env)
-        axis = (axis_env[0] == 'y' && axis_env[1] == 0) ? 1 : 2;
+    if (axis_env) {
+        if (strcmp(axis_env, "y") == 0)
+            axis = 1;
+        else if (strcmp(axis_env, "z") == 0)
+            axis = 2;
+        else
+            return BlockChoice{axis, 0, "LBM_BLOCK_AXIS (y or z)"};
+    }
     long kb = blocked ? chunk(axis) : 0;
     if (kb < min_slabs)
         kb = 0;
     if (planes_env) {
         char *end = nullptr;
-        kb = strtol(planes_env, &end, 10);
+        const long v = strtol(planes_env, &end, 10);
+        if (end == planes_env || *end != 0 || v < 0)
+            return BlockChoice{axis, 0, "LBM_BLOCK_PLANES (an integer >= 0)"};
+        kb = v;
     }
     const BlockGeom bg = block_geom(g, axis);
     kb = std::min<long>(std::min<long>(kb, 16), (long)bg.ns);
     if (kb < 1 || (uint64_t)slab_padded(g.lx * bg.nr) * (uint64_t)kb >= (1ull << 31))
         kb = 0;
-    return BlockChoice{axis, (uint32_t)kb};
+    return BlockChoice{axis, (uint32_t)kb, nullptr};
 }
 
 // A pair under one uniform force (r.fx, r.fy, r.fz).

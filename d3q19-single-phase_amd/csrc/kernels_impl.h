@@ -655,19 +655,25 @@ __global__ void __launch_bounds__(BLOCK) k_macro(MacroArgs a)
         }
     }
     if constexpr (WHAT == 1) {
-        // max over the block, then one atomic per block on the bit pattern
-        // (non-negative doubles order like their unsigned bit patterns)
-        for (int o = 32; o > 0; o >>= 1)
-            err = fmax(err, __shfl_xor(err, o));
-        __shared__ double wmax[BLOCK / 64];
+        // max over the block, then one atomic per block, all on the bit
+        // patterns: non-negative doubles order like their unsigned bit
+        // patterns, and a NaN with the sign cleared (fabs) orders above
+        // +inf, so a NaN anywhere becomes the result instead of being
+        // dropped as fmax would drop it
+        unsigned long long u = (unsigned long long)__double_as_longlong(err);
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long v = (unsigned long long)__shfl_xor((long long)u, o);
+            u = v > u ? v : u;
+        }
+        __shared__ unsigned long long wmax[BLOCK / 64];
         if ((threadIdx.x & 63) == 0)
-            wmax[threadIdx.x >> 6] = err;
+            wmax[threadIdx.x >> 6] = u;
         __syncthreads();
         if (threadIdx.x == 0) {
-            double m = wmax[0];
+            unsigned long long m = wmax[0];
             for (int w = 1; w < BLOCK / 64; w++)
-                m = fmax(m, wmax[w]);
-            atomicMax(a.errbits, (unsigned long long)__double_as_longlong(m));
+                m = wmax[w] > m ? wmax[w] : m;
+            atomicMax(a.errbits, m);
         }
     }
 }

@@ -546,6 +546,10 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         const BlockChoice bc = block_choice(h->g, h->elem, c.precision != LBM_FP16, BLOCK_RING_BYTES,
                                             BLOCK_MIN_SLABS, BLOCK_PREF_SLABS, getenv("LBM_BLOCK_AXIS"),
                                             getenv("LBM_BLOCK_PLANES"));
+        if (bc.bad) {
+            delete h;
+            return fail(LBM_ERR_ARG, "lbm_create: invalid value of %s", bc.bad);
+        }
         h->baxis = bc.axis;
         h->bK = bc.K;
         // LBM_BLOCK_FORCED=1: pairs under a force evaluated per node too.
@@ -1230,8 +1234,19 @@ int lbm_time_steps(lbm_channel *h, int nsteps, double *avg_ms, double *total_ms)
     CHECK_H(h);
     if (nsteps <= 0)
         return fail(LBM_ERR_ARG, "lbm_time_steps: nsteps must be positive");
-    // one timed unit per single step, or per blocked pair of steps
-    std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
+    // one timed unit per single step, or per blocked pair of steps; the
+    // events are destroyed on every return path
+    struct Events {
+        std::vector<hipEvent_t> e;
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x)
+                    (void)hipEventDestroy(x);
+        }
+    } evs;
+    evs.e.assign(2 * (size_t)nsteps, nullptr);
+    std::vector<hipEvent_t> &ev = evs.e;
     for (auto &e : ev)
         HIP_TRY(hipEventCreate(&e));
     int rc = LBM_OK;
@@ -1264,8 +1279,6 @@ int lbm_time_steps(lbm_channel *h, int nsteps, double *avg_ms, double *total_ms)
         if (total_ms)
             *total_ms = tot;
     }
-    for (auto &e : ev)
-        (void)hipEventDestroy(e);
     return rc;
 }
 

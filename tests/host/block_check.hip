@@ -246,6 +246,18 @@ int main()
                   check_choice(12, 6, 10, 8, true, nullptr, nullptr, 1, 6);
     if (!choice)
         return 1;
+    // invalid environment values are reported, not taken as a default
+    const Geom g0 = geom(192, 768, 384);
+    for (const char *bad_axis : {"Y", "x", "", "yz"})
+        if (!block_choice(g0, 8, true, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, BLOCK_PREF_SLABS, bad_axis, nullptr).bad) {
+            fprintf(stderr, "LBM_BLOCK_AXIS=\"%s\" accepted\n", bad_axis);
+            return 1;
+        }
+    for (const char *bad_k : {"", "abc", "4x", "-1", " "})
+        if (!block_choice(g0, 8, true, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, BLOCK_PREF_SLABS, nullptr, bad_k).bad) {
+            fprintf(stderr, "LBM_BLOCK_PLANES=\"%s\" accepted\n", bad_k);
+            return 1;
+        }
     printf("ok sweep choice\n");
 
     const uint32_t cases[][4] = {{4, 3, 1, 2},  {4, 3, 2, 2},  {5, 4, 3, 2},  {6, 5, 7, 2},   {5, 3, 9, 4},

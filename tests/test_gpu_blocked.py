@@ -100,26 +100,29 @@ def test_default_sweep_on_the_turbulent_channel(monkeypatch, prec, want):
             assert ch.block_info()[1] == 0
 
 
-def test_default_sweep_bitwise_at_full_size(monkeypatch):
-    """The FP64 default (y slabs, eleven to a chunk) on the full 192x768x384
-    channel from the bench's start (log-law + divergence-free perturbation):
-    two blocked pairs equal four single steps bit for bit, populations and
-    the FP64 mass."""
+@pytest.mark.parametrize("prec,want", [("fp64", ("y", 11)), ("fp32", ("z", 11))])
+def test_default_sweep_bitwise_at_full_size(monkeypatch, prec, want):
+    """The defaults behind the bench numbers (FP64: y slabs, FP32: z slabs,
+    eleven to a chunk) on the full 192x768x384 channel from the bench's start
+    (log-law + divergence-free perturbation): two blocked pairs equal four
+    single steps bit for bit, populations and the FP64 mass.  (The FP32 unit
+    allows FMA contraction; the sweep and the single step inline the same
+    node arithmetic and contract it the same way.)"""
     from d3q19 import workloads
     lx, ly, lz = 192, 768, 384
     monkeypatch.delenv("LBM_BLOCK_AXIS", raising=False)
-    print("full-size blocked vs single steps, fp64", flush=True)
+    print(f"full-size blocked vs single steps, {prec}", flush=True)
     chans = []
     for planes in (None, "0"):
         if planes is None:
             monkeypatch.delenv("LBM_BLOCK_PLANES", raising=False)
         else:
             monkeypatch.setenv("LBM_BLOCK_PLANES", planes)
-        chans.append(d3q19.Channel(lx, ly, lz, "fp64"))
+        chans.append(d3q19.Channel(lx, ly, lz, prec))
     monkeypatch.delenv("LBM_BLOCK_PLANES", raising=False)
     blk, one = chans
     with blk, one:
-        assert blk.block_info() == ("y", 11) and one.block_info()[1] == 0
+        assert blk.block_info() == want and one.block_info()[1] == 0
         ux, uy, uz = workloads.loglaw_divfree(lx, ly, lz, blk.p["ustar"], blk.p["ystar"], seed=2)
         for c in (blk, one):
             c.set_macros(ux=ux, uy=uy, uz=uz)

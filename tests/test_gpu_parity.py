@@ -295,3 +295,20 @@ def test_reduced_precision_tracks_fp64(prec, tol):
         pa = uy_a.mean(axis=(0, 1))
         pb = uy_b.mean(axis=(0, 1))
         assert np.max(np.abs(pb - pa)) / np.max(np.abs(pa)) < tol
+
+
+def test_rhoupdat_reports_a_nan():
+    """A NaN anywhere in the lattice makes max|drho| NaN (the block and wave
+    maxima order the bit patterns, where a NaN ranks above +inf), so the
+    pre-relaxation loop of Channel-Flow/main.f90:70-90 can never take it for
+    convergence."""
+    lx, ly, lz = 24, 20, 18
+    f = (workloads.uniform(71, 19 * lx * ly * lz) - 0.5).reshape(lz, ly, lx, 19) * 1e-4
+    with d3q19.Channel(lx, ly, lz) as ch:
+        ch.upload_f(f)
+        ch.FORCING()
+        assert np.isfinite(ch.rhoupdat())
+        assert ch.rhoupdat() == 0.0  # unchanged lattice: no change of rho
+        f[11, 7, 5, 3] = np.nan
+        ch.upload_f(f)
+        assert np.isnan(ch.rhoupdat())
