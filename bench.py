@@ -40,6 +40,10 @@ pair (halved); for the blocked sweep this is an effective bandwidth, since
 the Infinity Cache serves the intermediate lattice.  `traffic` is the
 rocprofv3 FETCH_SIZE + WRITE_SIZE figure between L2 and the memory fabric.
 
+"per_call": the same FP64 steps driven as the reference's main loop calls
+them, collision_MRT then macrovar per step through the C ABI (the library
+defers and pairs them into blocked sweeps), wall clock around the loop.
+
 "particles": the coupled step with 2000 spheres of radius 5 at the same
 turbulent constants (--particles 0 leaves it out).
 
@@ -149,6 +153,30 @@ def measure(prec, workload, steps, warmup, single_steps=False):
         # lbm_run's pairing of steps: sweep axis and slabs per chunk (0: single steps)
         "blocked_sweep": {"axis": axis, "slabs_per_chunk": slabs},
     }
+
+
+def measure_per_call(prec, workload, steps, warmup):
+    """The main loop as a driver calling the reference's subroutines one at a
+    time (collision_MRT then macrovar per step, Channel-Flow/main.f90:157-161)
+    through the C ABI; the library pairs those calls into blocked sweeps.
+    Wall clock around the host loop."""
+    lx, ly, lz, _ = WORKLOADS[workload]
+    n = lx * ly * lz
+    with d3q19.Channel(lx, ly, lz, prec, rstar=RSTAR.get(workload)) as ch:
+        init_channel(ch, lx, ly, lz)
+        for _ in range(warmup):
+            ch.collision_MRT()
+            ch.macrovar()
+        ch.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ch.collision_MRT()
+            ch.macrovar()
+        ch.sync()
+        wall = time.perf_counter() - t0
+    achieved = n * BYTES_PER_NODE[prec] / (wall / steps) / 1e9
+    return {"mlups": n * steps / wall / 1e6, "ms_per_step": wall / steps * 1e3,
+            "wall_frac": achieved / HBM_PEAK_GBS}
 
 
 def measure_forcingp(prec, workload, steps, warmup):
@@ -374,6 +402,11 @@ def main():
         f"{main_m['roofline']['achieved']:.0f} GB/s")
     single = measure(main_prec, args.workload, args.steps, args.warmup, single_steps=True)
     log(f"bench: {main_prec} single steps {single['mlups']:.1f} MLUPS, {single['roofline']['achieved']:.0f} GB/s")
+    pc = measure_per_call(main_prec, args.workload, args.steps, args.warmup)
+    per_call = {"value": round(pc["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(pc["ms_per_step"], 4),
+                "hbm_frac_wall": round(pc["wall_frac"], 4),
+                "calls": "collision_MRT + macrovar per step through the C ABI (paired by the library)"}
+    log(f"bench: {main_prec} per-call loop {pc['mlups']:.1f} MLUPS ({pc['wall_frac']:.3f} of 8 TB/s)")
     variants = {}
     for prec in [v for v in args.variants.split(",") if v and v != main_prec]:
         log(f"bench: variant {prec}")
@@ -450,6 +483,7 @@ def main():
         "cpu_baseline": cpu,
         "single_step": {"value": round(single["mlups"], 1), "unit": "MLUPS", "kernel_ms": round(single["kernel_ms"], 4),
                         "roofline_frac": single["roofline"]["frac"]},
+        "per_call": per_call,
         "variants": variants,
         "closure_coarse": closure,
         "forcingp": forcingp,

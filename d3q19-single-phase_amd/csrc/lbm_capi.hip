@@ -105,6 +105,13 @@ struct lbm_channel {
     int bforced;        // lbm_run_forcingp also pairs steps whose force varies over the nodes
     void *bring, *bhead; // its slab ring and head buffer (allocated on first use)
     bool ave_pending;   // lbm_run_particles: avedensity due once the next bounce-back has run
+    // Per-call pairing: collision_MRT calls whose launch is deferred (0-2) so
+    // that collision, macrovar, collision, macrovar runs as one blocked pair;
+    // pend_mac: a fields-only macrovar followed the first; defer: enabled
+    // (LBM_DEFER=0 at lbm_create turns it off)
+    int pend;
+    bool pend_mac;
+    int defer;
 };
 
 struct lbm_trainer {
@@ -558,6 +565,13 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         // costs the latency-bound sweep what the HBM-bound step hides).
         const char *kf = getenv("LBM_BLOCK_FORCED");
         h->bforced = (kf && strcmp(kf, "1") == 0) ? 1 : 0;
+        // LBM_DEFER=0: collision_MRT launches at once (no per-call pairing)
+        const char *kd = getenv("LBM_DEFER");
+        if (kd && strcmp(kd, "0") != 0 && strcmp(kd, "1") != 0) {
+            delete h;
+            return fail(LBM_ERR_ARG, "lbm_create: invalid value of LBM_DEFER (0 or 1)");
+        }
+        h->defer = (kd && strcmp(kd, "0") == 0) ? 0 : 1;
     }
     fields_from_cur(h);
     h->fresh = true;
@@ -646,12 +660,45 @@ int lbm_destroy(lbm_channel *h)
     return LBM_OK;
 }
 
-#define CHECK_H(h)                                                       \
+// Launches the collisions lbm_collision_mrt deferred, as single steps, and
+// leaves the state the calls one by one would have left: the first step, the
+// fields of a macrovar that followed it, the second step (whose macrovar has
+// not come, or the pair would have run).
+static int flush_pending(lbm_channel *h)
+{
+    const int n = h->pend;
+    if (n == 0)
+        return LBM_OK;
+    const bool mac = h->pend_mac;
+    h->pend = 0;
+    h->pend_mac = false;
+    int rc = launch_step(h);
+    if (rc)
+        return rc;
+    if (n == 1 && !mac)
+        return LBM_OK; // fields FROM_PREV, as launch_step left them
+    fields_from_cur(h);
+    h->rho_off = 0.0;
+    return n == 2 ? launch_step(h) : LBM_OK;
+}
+
+// Entry with no deferred work flushed: lbm_collision_mrt and lbm_macrovar.
+#define CHECK_H_KEEP(h)                                                  \
     do {                                                                 \
         if (!(h))                                                        \
             return fail(LBM_ERR_ARG, "%s: null handle", __func__);       \
         g_err[0] = 0;                                                    \
         HIP_TRY(hipSetDevice((h)->cfg.device));                          \
+    } while (0)
+
+// Every other entry point first launches deferred collisions, so it sees the
+// state the calls one by one leave.
+#define CHECK_H(h)                                                       \
+    do {                                                                 \
+        CHECK_H_KEEP(h);                                                 \
+        const int frc_ = flush_pending(h);                               \
+        if (frc_)                                                        \
+            return frc_;                                                 \
     } while (0)
 
 static int ensure_stage(lbm_channel *h)
@@ -831,15 +878,46 @@ int lbm_initpop(lbm_channel *h)
     return LBM_OK;
 }
 
+static bool block_ready(const lbm_channel *h);
+static int run_pair(lbm_channel *h);
+
 int lbm_collision_mrt(lbm_channel *h)
 {
-    CHECK_H(h);
+    CHECK_H_KEEP(h);
+    // Per-call pairing: a collision that lbm_run could pair is deferred; if
+    // macrovar and another collision follow, the two run as one blocked pair
+    // at the next macrovar (bitwise the single steps).  Any other call
+    // launches what is deferred first.
+    if (h->defer && h->pend == 0 && block_ready(h)) {
+        h->pend = 1;
+        h->pend_mac = false;
+        return LBM_OK;
+    }
+    if (h->pend == 1 && h->pend_mac) {
+        h->pend = 2;
+        return LBM_OK;
+    }
+    const int rc = flush_pending(h);
+    if (rc)
+        return rc;
     return launch_step(h);
 }
 
 int lbm_macrovar(lbm_channel *h, double *rho, double *ux, double *uy, double *uz)
 {
-    CHECK_H(h);
+    CHECK_H_KEEP(h);
+    if (h->pend && !rho && !ux && !uy && !uz) {
+        if (h->pend == 1) { // fields of the deferred step (again, if twice)
+            h->pend_mac = true;
+            return LBM_OK;
+        }
+        h->pend = 0; // collision, macrovar, collision, macrovar: one pair
+        h->pend_mac = false;
+        return run_pair(h);
+    }
+    const int frc = flush_pending(h);
+    if (frc)
+        return frc;
     fields_from_cur(h);
     h->rho_off = 0.0;
     h->fresh = false;
@@ -1755,6 +1833,12 @@ int lbm_trainer_add_filtered(lbm_trainer *t, lbm_channel *fine, double nu_coarse
     CHECK_T(t);
     if (!fine)
         return fail(LBM_ERR_ARG, "lbm_trainer_add_filtered: null channel");
+    {
+        HIP_TRY(hipSetDevice(fine->cfg.device));
+        const int frc = flush_pending(fine);
+        if (frc)
+            return frc;
+    }
     if (fine->cfg.device != t->device)
         return fail(LBM_ERR_ARG, "lbm_trainer_add_filtered: channel and trainer on different devices");
     if ((fine->g.lx | fine->g.ly | fine->g.lz) & 1)

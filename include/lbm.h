@@ -126,7 +126,15 @@ int lbm_initpop(lbm_channel *h);
 /* collision_MRT, Channel-Flow/collision.f90:24-273, including
  * collisionExchnge (:281-372), which on one GPU is the periodic y/z wrap.
  * Reads the macroscopic fields as the reference does: the values left by the
- * last macrovar / rhoupdat / avedensity / lbm_set_macros. */
+ * last macrovar / rhoupdat / avedensity / lbm_set_macros.
+ * A driver calling collision_MRT and macrovar once per step (main.f90:157-161)
+ * gets lbm_run's blocked pairs: when the step could be paired (fields from
+ * macrovar, uniform force, no particles or closure) its launch is deferred,
+ * and collision, macrovar, collision, macrovar runs as one pair at the second
+ * macrovar.  Every other entry point first launches what is deferred, so
+ * results, fields and downloads are those of the calls one by one, bit for
+ * bit; a launch error of a deferred step is reported by the call that
+ * launches it.  LBM_DEFER=0 at lbm_create turns the deferral off. */
 int lbm_collision_mrt(lbm_channel *h);
 
 /* macrovar, Channel-Flow/collision.f90:378-463: rho, u of the current lattice
