@@ -92,12 +92,14 @@ def traffic_from_profiles(prec, workload):
 
 
 def init_channel(ch, lx, ly, lz):
-    ux, uy, uz = workloads.loglaw_divfree(lx, ly, lz, ch.p["ustar"], ch.p["ystar"], seed=2)
-    ch.set_macros(ux=ux, uy=uy, uz=uz)
+    """Log-law mean profile plus the seeded divergence-free perturbation (10%),
+    summed on the device from plane x profile terms; returns the terms."""
+    terms = workloads.loglaw_divfree_terms(lx, ly, lz, ch.p["ustar"], ch.p["ystar"], seed=2)
+    ch.set_velocity_separable(terms)
     ch.FORCING()
     ch.initpop()
     ch.macrovar()
-    return ux, uy, uz
+    return terms
 
 
 def measure(prec, workload, steps, warmup, single_steps=False):
@@ -228,10 +230,9 @@ def measure_particles(prec, workload, npart, rad, steps, warmup):
     n = lx * ly * lz
     pos = workloads.sphere_positions(lx, ly, lz, npart, rad, seed=3)
     with d3q19.Channel(lx, ly, lz, prec, rstar=RSTAR.get(workload)) as ch:
-        ux, uy, uz = init_channel(ch, lx, ly, lz)
+        terms = init_channel(ch, lx, ly, lz)
         ix, iy, iz = (np.floor(pos[:, k]).astype(int) for k in range(3))
-        vel = np.stack([a[iz, iy, ix] for a in (ux, uy, uz)], axis=1)
-        del ux, uy, uz
+        vel = np.stack(workloads.separable_at(terms, ix, iy, iz), axis=1)
         ch.particles_create(pos, vel=vel, rad=rad, visc=ch.p["visc"])
         solid = int(np.count_nonzero(ch.particles_solid() >= 0))
         ch.run_particles(warmup)

@@ -857,6 +857,79 @@ int lbm_set_macros(lbm_channel *h, const double *rho, const double *ux, const do
     return LBM_OK;
 }
 
+namespace {
+
+constexpr int SEP_MAX_TERMS = 64;
+struct SepTerms {
+    int n;
+    int comp[SEP_MAX_TERMS];
+};
+
+// rho = 0 and u_c = sum over the terms k of component c of
+// planes[k][row] * profiles[k][ix] (row = iy + ly*iz), in increasing k from 0.
+__global__ void k_separable(Geom g, SepTerms t, const double *__restrict__ planes,
+                            const double *__restrict__ profiles, double *rho, double *ux, double *uy, double *uz)
+{
+    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.N)
+        return;
+    const uint64_t row = n / g.lx, ix = n - row * g.lx;
+    const uint64_t rows = g.N / g.lx;
+    double u[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < t.n; k++)
+        u[t.comp[k]] = u[t.comp[k]] + planes[(uint64_t)k * rows + row] * profiles[(uint64_t)k * g.lx + ix];
+    rho[n] = 0.0;
+    ux[n] = u[0];
+    uy[n] = u[1];
+    uz[n] = u[2];
+}
+
+} // namespace
+
+int lbm_set_velocity_separable(lbm_channel *h, int nterms, const int *comp, const double *planes,
+                               const double *profiles)
+{
+    CHECK_H(h);
+    if (nterms < 1 || nterms > SEP_MAX_TERMS || !comp || !planes || !profiles)
+        return fail(LBM_ERR_ARG, "lbm_set_velocity_separable: need 1 <= nterms <= %d and non-null arrays",
+                    SEP_MAX_TERMS);
+    SepTerms t;
+    t.n = nterms;
+    for (int k = 0; k < nterms; k++) {
+        if (comp[k] < 0 || comp[k] > 2)
+            return fail(LBM_ERR_ARG, "lbm_set_velocity_separable: term %d has component %d (0, 1 or 2)", k, comp[k]);
+        t.comp[k] = comp[k];
+    }
+    int rc = materialize(h);
+    if (rc)
+        return rc;
+    const size_t rows = (size_t)h->g.ly * h->g.lz;
+    const size_t pb = (size_t)nterms * rows * sizeof(double), qb = (size_t)nterms * h->g.lx * sizeof(double);
+    double *dp = nullptr, *dq = nullptr;
+    hipError_t e = hipMalloc((void **)&dp, pb);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&dq, qb);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dp, planes, pb, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dq, profiles, qb, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_separable, dim3((unsigned)((h->g.N + 255) / 256)), dim3(256), 0, h->stream, h->g, t, dp,
+                           dq, h->mac[0], h->mac[1], h->mac[2], h->mac[3]);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(h->stream);
+    if (dp)
+        (void)hipFree(dp);
+    if (dq)
+        (void)hipFree(dq);
+    if (e != hipSuccess)
+        return fail(LBM_ERR_HIP, "lbm_set_velocity_separable: %s", hipGetErrorString(e));
+    h->fresh = false;
+    return LBM_OK;
+}
+
 int lbm_initpop(lbm_channel *h)
 {
     CHECK_H(h);
@@ -1386,36 +1459,59 @@ int lbm_block_info(lbm_channel *h, int *axis, int *slabs)
     return LBM_OK;
 }
 
+// Nodes [n0, n0 + count) of the stored post-collision lattice as FP64,
+// population-major (out[i*count + k] = P_i(n0 + k)).
+static int download_post_range(lbm_channel *h, uint64_t n0, uint64_t count, double *out)
+{
+    const uint64_t N = h->g.N;
+    const unsigned char *L = (const unsigned char *)h->lat[h->cur];
+    if (h->prec == LBM_FP64) {
+        for (int i = 0; i < 19; i++)
+            HIP_TRY(hipMemcpyAsync(out + (size_t)i * count, L + ((size_t)i * N + n0) * sizeof(double),
+                                   count * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return LBM_OK;
+    }
+    // FP32/FP16: widen on the host, population by population
+    std::vector<unsigned char> buf(count * h->elem);
+    for (int i = 0; i < 19; i++) {
+        HIP_TRY(hipMemcpyAsync(buf.data(), L + ((size_t)i * N + n0) * h->elem, count * h->elem,
+                               hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        double *o = out + (size_t)i * count;
+        if (h->prec == LBM_FP32) {
+            const float *v = (const float *)buf.data();
+            for (uint64_t n = 0; n < count; n++)
+                o[n] = v[n];
+        } else {
+            const _Float16 *v = (const _Float16 *)buf.data();
+            const double sc = i == 0 ? 0.25 : (i <= 6 ? 1.0 / 16.0 : 1.0 / 32.0);
+            for (uint64_t n = 0; n < count; n++)
+                o[n] = (double)(float)v[n] * sc;
+        }
+    }
+    return LBM_OK;
+}
+
 int lbm_download_post(lbm_channel *h, double *post)
 {
     CHECK_H(h);
     if (!post)
         return fail(LBM_ERR_ARG, "lbm_download_post: null output");
-    const uint64_t N = h->g.N;
-    if (h->prec == LBM_FP64) {
-        HIP_TRY(hipMemcpyAsync(post, h->lat[h->cur], N * 19 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
+    return download_post_range(h, 0, h->g.N, post);
+}
+
+int lbm_download_post_range(lbm_channel *h, long long first, long long count, double *post)
+{
+    CHECK_H(h);
+    if (!post)
+        return fail(LBM_ERR_ARG, "lbm_download_post_range: null output");
+    if (first < 0 || count < 0 || (unsigned long long)first + (unsigned long long)count > h->g.N)
+        return fail(LBM_ERR_ARG, "lbm_download_post_range: nodes [%lld, %lld + %lld) outside the %llu-node lattice",
+                    first, first, count, (unsigned long long)h->g.N);
+    if (count == 0)
         return LBM_OK;
-    }
-    // FP32/FP16: widen on the host, population by population
-    std::vector<unsigned char> buf(N * h->elem);
-    for (int i = 0; i < 19; i++) {
-        HIP_TRY(hipMemcpyAsync(buf.data(), (const unsigned char *)h->lat[h->cur] + (size_t)i * N * h->elem,
-                               N * h->elem, hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        double *o = post + (size_t)i * N;
-        if (h->prec == LBM_FP32) {
-            const float *v = (const float *)buf.data();
-            for (uint64_t n = 0; n < N; n++)
-                o[n] = v[n];
-        } else {
-            const _Float16 *v = (const _Float16 *)buf.data();
-            const double sc = i == 0 ? 0.25 : (i <= 6 ? 1.0 / 16.0 : 1.0 / 32.0);
-            for (uint64_t n = 0; n < N; n++)
-                o[n] = (double)(float)v[n] * sc;
-        }
-    }
-    return LBM_OK;
+    return download_post_range(h, (uint64_t)first, (uint64_t)count, post);
 }
 
 static int part_alloc(Part *pt, void **p, size_t bytes)

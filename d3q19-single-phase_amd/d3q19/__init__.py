@@ -67,6 +67,7 @@ _SIGS = {
     "lbm_run_forcingp": [_H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double],
     "lbm_set_macros": [_H, _D, _D, _D, _D],
     "lbm_initpop": [_H],
+    "lbm_set_velocity_separable": [_H, ctypes.c_int, _D, _D, _D],
     "lbm_collision_mrt": [_H],
     "lbm_macrovar": [_H, _D, _D, _D, _D],
     "lbm_rhoupdat": [_H, ctypes.POINTER(ctypes.c_double)],
@@ -85,6 +86,7 @@ _SIGS = {
     "lbm_device_bytes": [_H, ctypes.POINTER(ctypes.c_size_t)],
     "lbm_block_info": [_H, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "lbm_download_post": [_H, _D],
+    "lbm_download_post_range": [_H, ctypes.c_longlong, ctypes.c_longlong, _D],
     "lbm_particles_create": [_H, _D, _D, _D, _D],
     "lbm_beads_links": [_H],
     "lbm_beads_filling": [_H],
@@ -275,6 +277,17 @@ class Channel:
                 raise ValueError(f"field of {a.size} values for {self.N} nodes")
         _call("lbm_set_macros", self._h, *[_ptr(a) for a in arrs])
 
+    def set_velocity_separable(self, terms):
+        """rho = 0 and u_c = sum of plane (lz, ly) x profile (lx) terms, summed on
+        the device in the order given; terms = [(component, plane, profile), ...]."""
+        comp = np.ascontiguousarray([int(c) for c, _, _ in terms], dtype=np.int32)
+        planes = np.ascontiguousarray(np.stack([np.asarray(p, dtype=np.float64).reshape(self.lz, self.ly)
+                                                for _, p, _ in terms]))
+        profs = np.ascontiguousarray(np.stack([np.asarray(q, dtype=np.float64).reshape(self.lx)
+                                               for _, _, q in terms]))
+        _call("lbm_set_velocity_separable", self._h, len(terms), comp.ctypes.data_as(ctypes.c_void_p),
+              _ptr(planes), _ptr(profs))
+
     def initpop(self):
         """Channel-Flow/initial.f90:19-46: rho = 0, f = equilibrium of the current u."""
         _call("lbm_initpop", self._h)
@@ -400,6 +413,12 @@ class Channel:
         """Stored post-collision lattice, FP64, shape (19, N) (node = ix + lx*(iy + ly*iz))."""
         out = np.empty((19, self.N))
         _call("lbm_download_post", self._h, _ptr(out))
+        return out
+
+    def download_post_range(self, first, count):
+        """Nodes [first, first + count) of the stored post-collision lattice, FP64, shape (19, count)."""
+        out = np.empty((19, int(count)))
+        _call("lbm_download_post_range", self._h, int(first), int(count), _ptr(out))
         return out
 
     # -- rigid particles (beads_* of Channel-Flow/var_inc.f90:166-168) ----------

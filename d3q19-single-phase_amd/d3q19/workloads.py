@@ -48,19 +48,9 @@ def laminar_parabola_noise(lx, ly, lz, seed=1, uc=0.05, amp=0.05):
     return ux, uy, uz
 
 
-def loglaw_divfree(lx, ly, lz, ustar, ystar, seed=2, amp=0.10, nmodes=8, kmax=(3, 3), near_wall=False):
-    """Log-law mean profile (Channel-Flow/initial.f90:104-115) plus a
-    divergence-free random-phase perturbation u' = curl(psi).
-
-    psi_c(x, y, z) = sum_m a_{c,m} g(x) cos(ky_m y + kz_m z + phi_{c,m}) with the
-    wall envelope g(x) = sin^2(pi x / lx) (x = i - 0.5), or sin^2(2 pi x / lx)
-    with near_wall (peaks at a quarter of the width from each wall), integer
-    wavenumbers 1..kmax[0] in y and 1..kmax[1] in z on the periodic box;
-    derivatives are taken analytically, so div u' = 0 exactly in the
-    continuum.  u' is scaled so that max|u'| equals amp times the centreline
-    mean velocity.
-    """
-    prof = np.asarray(loglaw_profile(lx, ustar, ystar))
+def _divfree_planes(lx, ly, lz, seed, nmodes, kmax, near_wall):
+    """Wall envelope g, its derivative dg (length lx) and the (lz, ly) plane
+    fields A..E of the random-phase stream function (see loglaw_divfree)."""
     x = np.arange(1, lx + 1, dtype=np.float64) - 0.5
     y = np.arange(1, ly + 1, dtype=np.float64) - 0.5
     z = np.arange(1, lz + 1, dtype=np.float64) - 0.5
@@ -92,6 +82,23 @@ def loglaw_divfree(lx, ly, lz, ustar, ystar, seed=2, amp=0.10, nmodes=8, kmax=(3
         C += a[2] * c[2]                                  # d psi_z/dx = g' C
         D += a[1] * c[1]                                  # d psi_y/dx = g' D
         E += a[0] * (-ky) * s[0]                          # d psi_x/dy
+    return g, dg, A, B, C, D, E
+
+
+def loglaw_divfree(lx, ly, lz, ustar, ystar, seed=2, amp=0.10, nmodes=8, kmax=(3, 3), near_wall=False):
+    """Log-law mean profile (Channel-Flow/initial.f90:104-115) plus a
+    divergence-free random-phase perturbation u' = curl(psi).
+
+    psi_c(x, y, z) = sum_m a_{c,m} g(x) cos(ky_m y + kz_m z + phi_{c,m}) with the
+    wall envelope g(x) = sin^2(pi x / lx) (x = i - 0.5), or sin^2(2 pi x / lx)
+    with near_wall (peaks at a quarter of the width from each wall), integer
+    wavenumbers 1..kmax[0] in y and 1..kmax[1] in z on the periodic box;
+    derivatives are taken analytically, so div u' = 0 exactly in the
+    continuum.  u' is scaled so that max|u'| equals amp times the centreline
+    mean velocity.
+    """
+    prof = np.asarray(loglaw_profile(lx, ustar, ystar))
+    g, dg, A, B, C, D, E = _divfree_planes(lx, ly, lz, seed, nmodes, kmax, near_wall)
     ux = A[:, :, None] * g[None, None, :]
     uy = B[:, :, None] * g[None, None, :] - C[:, :, None] * dg[None, None, :]
     uz = D[:, :, None] * dg[None, None, :] - E[:, :, None] * g[None, None, :]
@@ -102,6 +109,53 @@ def loglaw_divfree(lx, ly, lz, ustar, ystar, seed=2, amp=0.10, nmodes=8, kmax=(3
     uz *= scale
     uy += prof[None, None, :]
     return ux, uy, uz
+
+
+def loglaw_divfree_terms(lx, ly, lz, ustar, ystar, seed=2, amp=0.10, nmodes=8, kmax=(3, 3), near_wall=False):
+    """The field of loglaw_divfree as separable terms [(component, plane
+    (lz, ly), profile (lx)), ...] for Channel.set_velocity_separable, which
+    sums them on the device: no 3 x N host arrays.  The perturbation's
+    maximum is found plane by plane, wall-normal index by wall-normal index,
+    with the same products and differences, so the scale is loglaw_divfree's;
+    the device sums the scaled terms, so the fields agree with
+    loglaw_divfree's to rounding, not bit for bit (separable_fields gives
+    the device's values)."""
+    prof = np.asarray(loglaw_profile(lx, ustar, ystar))
+    g, dg, A, B, C, D, E = _divfree_planes(lx, ly, lz, seed, nmodes, kmax, near_wall)
+    umax = float(np.abs(A).max() * np.abs(g).max())
+    b1 = np.empty_like(A)
+    b2 = np.empty_like(A)
+    for i in range(lx):
+        for P, p, Q, q in ((B, g[i], C, dg[i]), (D, dg[i], E, g[i])):
+            np.multiply(P, p, out=b1)
+            np.multiply(Q, q, out=b2)
+            np.subtract(b1, b2, out=b1)
+            np.abs(b1, out=b1)
+            umax = max(umax, float(b1.max()))
+    scale = amp * prof.max() / umax
+    one = np.ones((lz, ly))
+    return [(0, A * scale, g), (1, one, prof), (1, B * scale, g), (1, -(C * scale), dg),
+            (2, D * scale, dg), (2, -(E * scale), g)]
+
+
+def separable_fields(terms, lx, ly, lz):
+    """Host evaluation of separable terms exactly as the device sums them
+    (each component from 0.0, term by term in the order given, product then
+    add); small grids only."""
+    u = [np.zeros((lz, ly, lx)) for _ in range(3)]
+    for c, P, Q in terms:
+        u[c] = u[c] + np.asarray(P).reshape(lz, ly)[:, :, None] * np.asarray(Q).reshape(lx)[None, None, :]
+    return tuple(u)
+
+
+def separable_at(terms, ix, iy, iz):
+    """The separable field's (ux, uy, uz) at nodes (ix, iy, iz) (0-based index
+    arrays), summed as the device sums it."""
+    ix, iy, iz = (np.asarray(a) for a in (ix, iy, iz))
+    u = [np.zeros(ix.shape) for _ in range(3)]
+    for c, P, Q in terms:
+        u[c] = u[c] + np.asarray(P)[iz, iy] * np.asarray(Q)[ix]
+    return tuple(u)
 
 
 def sphere_positions(lx, ly, lz, n, rad, gap=3.0, gap_w=3.0, seed=3, max_tries=2_000_000):

@@ -119,6 +119,18 @@ int lbm_run_forcingp(lbm_channel *h, int istep0, int nsteps, int npforcing, doub
 int lbm_set_macros(lbm_channel *h, const double *rho, const double *ux, const double *uy,
                    const double *uz);
 
+/* Sets the fields the next collision reads to rho = 0 (initial.f90:32) and a
+ * velocity that is a sum of separable terms,
+ *   u_c(ix, iy, iz) = sum over k with comp[k] == c of planes[k][iy + ly*iz] * profiles[k][ix],
+ * each component summed in increasing k from 0 (comp[k] = 0, 1, 2 for ux,
+ * uy, uz; planes nterms x ly*lz, profiles nterms x lx, nterms <= 64).  For
+ * initial fields of large lattices (initvel, initial.f90:75-147: a mean
+ * profile in x plus perturbations) without 3 x N host arrays, which take
+ * 25.8 GB for the 2048x1024x512 channel.  No reference counterpart as one
+ * call. */
+int lbm_set_velocity_separable(lbm_channel *h, int nterms, const int *comp, const double *planes,
+                               const double *profiles);
+
 /* initpop, Channel-Flow/initial.f90:19-46: rho = 0 and f = equilibrium of
  * (rho = 0, u) from the current velocity fields, on the device. */
 int lbm_initpop(lbm_channel *h);
@@ -226,6 +238,12 @@ int lbm_block_info(lbm_channel *h, int *axis, int *slabs);
  * population-major, node index ix + lx*(iy + ly*iz)), for checkpoint-free
  * inspection and tests.  Synchronises. */
 int lbm_download_post(lbm_channel *h, double *post);
+
+/* The nodes [first, first + count) of the same stored lattice, FP64,
+ * population-major: post[i*count + k] is population i of node first + k.
+ * For a window of a lattice too large to download whole (the 2048x1024x512
+ * channel's FP32 lattice is 81.6 GB).  Synchronises. */
+int lbm_download_post_range(lbm_channel *h, long long first, long long count, double *post);
 
 /* ---------------------------------------------------------------------------
  * Rigid spherical particles.  The reference keeps only the data structures of
