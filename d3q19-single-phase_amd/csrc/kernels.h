@@ -267,6 +267,39 @@ __host__ __device__ inline void block_src(const PlaneRef (&pl)[3], const BlockLa
     }
 }
 
+// Run monitor of Channel-Flow/saveload.f90:1507-1676 (diag): per block, the
+// fluid-node sums of ux, uy, uz, ux^2, uy^2, uz^2 and the count, the largest
+// |u| with the smallest node index that reaches it, and the largest and
+// smallest rho; partials[q*grid + block].  SRC 0: field arrays; SRC 1:
+// macrovar(F) of lattice L with rho - rho_off.
+constexpr int DIAG_Q = 11;
+struct DiagArgs {
+    Geom g;
+    const void *L;
+    const double *rho, *ux, *uy, *uz;
+    const int *solid; // nullable: when set, only fluid nodes count
+    ForceSpec f;
+    double rho_off;
+    double *partials;
+};
+
+// dissipation_calc (Channel-Flow/saveload.f90:4644-4882): per wall-normal
+// plane the fluid-node sums of S_ij S_ij and S_xy by the two methods and the
+// count; partials[(chunk*NDISS + q)*lx + ix].  f is pulled from L (the
+// current lattice); the fields come from the arrays (SRC 0) or from the
+// same pull (SRC 1).
+constexpr int NDISS = 5;
+struct DissArgs {
+    Geom g;
+    const void *L;
+    const double *rho, *ux, *uy, *uz;
+    const int *solid;
+    ForceSpec f;
+    double rho_off;
+    double s1, s9, coef2, coef3, coef5, ww1, ww2, c15tau; // c15tau = 1.5/tau
+    double *partials;
+};
+
 template <int P>
 struct Launch {
     static hipError_t step(const StepArgs &a, int mode, hipStream_t s);
@@ -281,6 +314,8 @@ struct Launch {
     static hipError_t mass_partials(const SumArgs &a, uint32_t grid, hipStream_t s);
     static hipError_t plane_partials(const PlaneArgs &a, int src, uint32_t chunks, hipStream_t s);
     static hipError_t filtered_plane_partials(const PlaneArgs &a, uint32_t chunks, hipStream_t s);
+    static hipError_t diag_partials(const DiagArgs &a, int src, uint32_t grid, hipStream_t s);
+    static hipError_t dissipation_partials(const DissArgs &a, int src, uint32_t chunks, hipStream_t s);
     // one launch of the two-step blocked sweep along axis 2 (z) or 1 (y)
     static hipError_t step_planes(const BlockArgs &a, int axis, hipStream_t s);
 };

@@ -957,6 +957,223 @@ static inline uint32_t nblocks(uint64_t n)
 }
 
 // Does the vector-run step kernel apply to this launch?
+// The fields of node n: from the arrays (SRC 0) or macrovar of the pulled
+// populations with the fields' force and rho - rho_off (SRC 1), the values
+// k_macro stores; f9 receives the pulled populations when wanted.
+template <int P, int SRC>
+__device__ __forceinline__ void node_fields(const void *L, const double *rho_a, const double *ux_a,
+                                            const double *uy_a, const double *uz_a, const ForceSpec &fs,
+                                            double rho_off, const Geom &g, uint32_t n, double &rho, double &ux,
+                                            double &uy, double &uz, double *f9)
+{
+    using S = typename Prec<P>::S;
+    using T = typename Prec<P>::T;
+    if constexpr (SRC == 0) {
+        rho = rho_a[n];
+        ux = ux_a[n];
+        uy = uy_a[n];
+        uz = uz_a[n];
+        if (f9) {
+            const Nbr b = make_nbr(n, g);
+            T f[19];
+            pull_all<P>((const S *)L, g.N, b, f);
+#pragma unroll
+            for (int i = 0; i < 19; i++)
+                f9[i] = (double)f[i];
+        }
+    } else {
+        const Nbr b = make_nbr(n, g);
+        T f[19];
+        pull_all<P>((const S *)L, g.N, b, f);
+        T rho9, ux9, uy9, uz9;
+        macro_sums(f, rho9, ux9, uy9, uz9);
+        uint32_t cx, cy, cz;
+        node_coords(n, g, cx, cy, cz);
+        double fx, fy, fz;
+        node_force(fs, g, cx, cy, cz, fx, fy, fz);
+        ux = (double)(ux9 + (T)fx / T(2.));
+        uy = (double)(uy9 + (T)fy / T(2.));
+        uz = (double)(uz9 + (T)fz / T(2.));
+        rho = (double)(rho9 - (T)rho_off);
+        if (f9) {
+#pragma unroll
+            for (int i = 0; i < 19; i++)
+                f9[i] = (double)f[i];
+        }
+    }
+}
+
+// Larger |u| wins; on a tie the smaller node index (the reference's loop
+// keeps the first strict maximum in x, y, z order).  idx < 0: none yet.
+__device__ __forceinline__ void vmax_merge(double &v, double &idx, double ov, double oidx)
+{
+    if (ov > v || (ov == v && oidx >= 0.0 && (idx < 0.0 || oidx < idx))) {
+        v = ov;
+        idx = oidx;
+    }
+}
+
+template <int P, int SRC>
+__global__ void __launch_bounds__(BLOCK) k_diag_partials(DiagArgs a)
+{
+    double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double vmax = 0.0, vidx = -1.0, rmax = -INFINITY, rmin = INFINITY;
+    for (uint64_t n = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; n < a.g.N; n += (uint64_t)gridDim.x * BLOCK) {
+        if (a.solid && a.solid[n] >= 0)
+            continue;
+        double rho, ux, uy, uz;
+        node_fields<P, SRC>(a.L, a.rho, a.ux, a.uy, a.uz, a.f, a.rho_off, a.g, (uint32_t)n, rho, ux, uy, uz,
+                            nullptr);
+        s[0] += ux;
+        s[1] += uy;
+        s[2] += uz;
+        s[3] += ux * ux;
+        s[4] += uy * uy;
+        s[5] += uz * uz;
+        s[6] += 1.0;
+        const double vel = sqrt(ux * ux + uy * uy + uz * uz);
+        if (vel > vmax) { // nodes come in increasing order: the first maximum stays
+            vmax = vel;
+            vidx = (double)n;
+        }
+        rmax = rho > rmax ? rho : rmax;
+        rmin = rho < rmin ? rho : rmin;
+    }
+    double out[DIAG_Q];
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        out[q] = block_sum(s[q]);
+        __syncthreads();
+    }
+    // maxima: lanes by xor butterfly, then the waves in order
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(vmax, o), oi = __shfl_xor(vidx, o);
+        vmax_merge(vmax, vidx, ov, oi);
+        const double orx = __shfl_xor(rmax, o), orn = __shfl_xor(rmin, o);
+        rmax = orx > rmax ? orx : rmax;
+        rmin = orn < rmin ? orn : rmin;
+    }
+    __shared__ double wm[4][BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) {
+        wm[0][threadIdx.x >> 6] = vmax;
+        wm[1][threadIdx.x >> 6] = vidx;
+        wm[2][threadIdx.x >> 6] = rmax;
+        wm[3][threadIdx.x >> 6] = rmin;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = wm[0][0], vi = wm[1][0], rx = wm[2][0], rn = wm[3][0];
+        for (int w = 1; w < BLOCK / 64; w++) {
+            vmax_merge(v, vi, wm[0][w], wm[1][w]);
+            rx = wm[2][w] > rx ? wm[2][w] : rx;
+            rn = wm[3][w] < rn ? wm[3][w] : rn;
+        }
+        out[7] = v;
+        out[8] = vi;
+        out[9] = rx;
+        out[10] = rn;
+#pragma unroll
+        for (int q = 0; q < DIAG_Q; q++)
+            a.partials[(uint64_t)q * gridDim.x + blockIdx.x] = out[q];
+    }
+}
+
+// Per-node strain of dissipation_calc (Channel-Flow/saveload.f90:4677-4776)
+// in the reference's operand order: v[0] = S_ij S_ij and v[1] = S_xy from
+// the non-equilibrium moments, v[2], v[3] the same from the non-equilibrium
+// populations.
+__device__ inline void dissipation_node(const DissArgs &a, const double *f9, double rho9, double ux9, double uy9,
+                                        double uz9, double *v)
+{
+    const double ux9s = ux9 * ux9, uy9s = uy9 * uy9, uz9s = uz9 * uz9;
+    const double eqm1 = -11.0 * rho9 + 19.0 * (ux9s + uy9s + uz9s);
+    const double eqm6 = 2.0 * ux9s - uy9s - uz9s;
+    const double eqm8 = uy9s - uz9s;
+    const double eqm10 = ux9 * uy9;
+    const double eqm11 = uy9 * uz9;
+    const double eqm12 = ux9 * uz9;
+    const double sum1 = f9[1] + f9[2] + f9[3] + f9[4] + f9[5] + f9[6];
+    const double sum2 = f9[7] + f9[8] + f9[9] + f9[10] + f9[11] + f9[12] + f9[13] + f9[14] + f9[15] + f9[16] +
+                        f9[17] + f9[18];
+    const double sum6 = f9[1] + f9[2];
+    const double sum7 = f9[3] + f9[4] + f9[5] + f9[6];
+    const double sum8 = f9[7] + f9[8] + f9[9] + f9[10] + f9[11] + f9[12] + f9[13] + f9[14];
+    const double sum9 = f9[15] + f9[16] + f9[17] + f9[18];
+    const double sum10 = f9[3] + f9[4] - f9[5] - f9[6];
+    const double sum11 = f9[7] + f9[8] + f9[9] + f9[10] - f9[11] - f9[12] - f9[13] - f9[14];
+    const double evlm1 = -30.0 * f9[0] + a.coef2 * sum1 + a.coef3 * sum2;
+    const double evlm6 = a.coef5 * sum6 - sum7 + sum8 - a.coef5 * sum9;
+    const double evlm8 = sum10 + sum11;
+    const double evlm10 = f9[7] - f9[8] - f9[9] + f9[10];
+    const double evlm11 = f9[15] - f9[16] - f9[17] + f9[18];
+    const double evlm12 = f9[11] - f9[12] - f9[13] + f9[14];
+    const double neqm1 = a.s1 * (evlm1 - eqm1);
+    const double neqm9 = a.s9 * (evlm6 - eqm6);
+    const double neqm11 = a.s9 * (evlm8 - eqm8);
+    const double neqm13 = a.s9 * (evlm10 - eqm10);
+    const double neqm14 = a.s9 * (evlm11 - eqm11);
+    const double neqm15 = a.s9 * (evlm12 - eqm12);
+    double Sxx = -(neqm1 + 19.0 * neqm9) / 38.0;
+    double Syy = -(2.0 * neqm1 - 19.0 * (neqm9 - 3.0 * neqm11)) / 76.0;
+    double Szz = -(2.0 * neqm1 - 19.0 * (neqm9 + 3.0 * neqm11)) / 76.0;
+    double Sxy = -1.5 * neqm13;
+    double Syz = -1.5 * neqm14;
+    double Szx = -1.5 * neqm15;
+    v[0] = Sxx * Sxx + Syy * Syy + Szz * Szz + 2.0 * (Sxy * Sxy + Syz * Syz + Szx * Szx);
+    v[1] = Sxy;
+    Sxx = 0.0;
+    Syy = 0.0;
+    Szz = 0.0;
+    Sxy = 0.0;
+    Syz = 0.0;
+    Szx = 0.0;
+    const double usqr = 1.5 * (ux9s + uy9s + uz9s);
+#pragma unroll
+    for (int ip = 1; ip < 19; ip++) {
+        const double wght = ip <= 6 ? a.ww1 : a.ww2;
+        const double edtu = (double)CX[ip] * ux9 + (double)CY[ip] * uy9 + (double)CZ[ip] * uz9;
+        const double feq9 = wght * (rho9 + 3.0 * edtu + 4.5 * (edtu * edtu) - usqr);
+        const double fneq9 = f9[ip] - feq9;
+        Sxx = Sxx + fneq9 * (double)(CX[ip] * CX[ip]);
+        Syy = Syy + fneq9 * (double)(CY[ip] * CY[ip]);
+        Szz = Szz + fneq9 * (double)(CZ[ip] * CZ[ip]);
+        Sxy = Sxy + fneq9 * (double)(CX[ip] * CY[ip]);
+        Syz = Syz + fneq9 * (double)(CY[ip] * CZ[ip]);
+        Szx = Szx + fneq9 * (double)(CZ[ip] * CX[ip]);
+    }
+    const double m2 = Sxx * Sxx + Syy * Syy + Szz * Szz + 2.0 * (Sxy * Sxy + Syz * Syz + Szx * Szx);
+    v[2] = m2 * (a.c15tau * a.c15tau);
+    v[3] = Sxy * a.c15tau;
+}
+
+// Plane sums of dissipation_calc: block (chunk, x tile) strides over rows.
+template <int P, int SRC>
+__global__ void __launch_bounds__(BLOCK) k_diss_partials(DissArgs a)
+{
+    const uint32_t ix = blockIdx.y * blockDim.x + threadIdx.x;
+    if (ix >= a.g.lx)
+        return;
+    double s[NDISS] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const uint32_t rows = a.g.ly * a.g.lz;
+    for (uint32_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const uint32_t n = row * a.g.lx + ix;
+        if (a.solid && a.solid[n] >= 0)
+            continue;
+        double f9[19], rho, ux, uy, uz, v[4];
+        node_fields<P, SRC>(a.L, a.rho, a.ux, a.uy, a.uz, a.f, a.rho_off, a.g, n, rho, ux, uy, uz, f9);
+        dissipation_node(a, f9, rho, ux, uy, uz, v);
+        s[0] += v[0];
+        s[1] += v[2];
+        s[2] += v[1];
+        s[3] += v[3];
+        s[4] += 1.0;
+    }
+    double *out = a.partials + (uint64_t)blockIdx.x * NDISS * a.g.lx;
+#pragma unroll
+    for (int q = 0; q < NDISS; q++)
+        out[(uint64_t)q * a.g.lx + ix] = s[q];
+}
+
 template <int P>
 static inline bool use_runs(const StepArgs &a, int mode)
 {
@@ -1138,6 +1355,28 @@ hipError_t Launch<P>::filtered_plane_partials(const PlaneArgs &a, uint32_t chunk
     const uint32_t bx = lxc >= BLOCK ? BLOCK : ((lxc + 63) / 64) * 64;
     const dim3 grid(chunks, (lxc + bx - 1) / bx), blk(bx);
     hipLaunchKernelGGL((k_filtered_plane_partials<P>), grid, blk, 0, s, a);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t Launch<P>::diag_partials(const DiagArgs &a, int src, uint32_t grid, hipStream_t s)
+{
+    if (src == 0)
+        hipLaunchKernelGGL((k_diag_partials<P, 0>), dim3(grid), dim3(BLOCK), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_diag_partials<P, 1>), dim3(grid), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t Launch<P>::dissipation_partials(const DissArgs &a, int src, uint32_t chunks, hipStream_t s)
+{
+    const uint32_t bx = a.g.lx >= BLOCK ? BLOCK : ((a.g.lx + 63) / 64) * 64;
+    const dim3 grid(chunks, (a.g.lx + bx - 1) / bx), blk(bx);
+    if (src == 0)
+        hipLaunchKernelGGL((k_diss_partials<P, 0>), grid, blk, 0, s, a);
+    else
+        hipLaunchKernelGGL((k_diss_partials<P, 1>), grid, blk, 0, s, a);
     return hipGetLastError();
 }
 

@@ -1288,6 +1288,227 @@ int lbm_statistc2(lbm_channel *h, double ustar, double ystar, double *out)
     return LBM_OK;
 }
 
+namespace {
+
+// diag's block partials (q*grid + block) to totals in out[0..DIAG_Q): the sums
+// in increasing block order from 0.0, the maxima merged in block order.
+__global__ void k_diag_final(const double *__restrict__ part, uint32_t grid, double *out)
+{
+    if (threadIdx.x != 0)
+        return;
+    double t[DIAG_Q];
+    for (int q = 0; q < 7; q++) {
+        double s = 0.0;
+        for (uint32_t b = 0; b < grid; b++)
+            s += part[(uint64_t)q * grid + b];
+        t[q] = s;
+    }
+    double v = part[7 * (uint64_t)grid], vi = part[8 * (uint64_t)grid];
+    double rx = part[9 * (uint64_t)grid], rn = part[10 * (uint64_t)grid];
+    for (uint32_t b = 1; b < grid; b++) {
+        const double ov = part[7 * (uint64_t)grid + b], oi = part[8 * (uint64_t)grid + b];
+        if (ov > v || (ov == v && oi >= 0.0 && (vi < 0.0 || oi < vi))) {
+            v = ov;
+            vi = oi;
+        }
+        const double ox = part[9 * (uint64_t)grid + b], on = part[10 * (uint64_t)grid + b];
+        rx = ox > rx ? ox : rx;
+        rn = on < rn ? on : rn;
+    }
+    t[7] = v;
+    t[8] = vi;
+    t[9] = rx;
+    t[10] = rn;
+    for (int q = 0; q < DIAG_Q; q++)
+        out[q] = t[q];
+}
+
+// vortcalc (Channel-Flow/saveload.f90:3929-4023) from the field arrays.
+__global__ void k_vortcalc(Geom g, const double *__restrict__ ux, const double *__restrict__ uy,
+                           const double *__restrict__ uz, const int *__restrict__ solid, const double *__restrict__ omg,
+                           double *ox, double *oy, double *oz)
+{
+    const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.N)
+        return;
+    if (solid && solid[n] >= 0) { // twice the particle's angular velocity
+        const double *w = omg + 3 * (uint64_t)solid[n];
+        ox[n] = 2.0 * w[0];
+        oy[n] = 2.0 * w[1];
+        oz[n] = 2.0 * w[2];
+        return;
+    }
+    const uint64_t row = n / g.lx;
+    const uint32_t i = (uint32_t)(n - row * g.lx);
+    const uint32_t k = (uint32_t)(row / g.ly), j = (uint32_t)(row - (uint64_t)k * g.ly);
+    const uint64_t plane = (uint64_t)g.lx * g.ly;
+    double pwx, pvx;
+    if (i == 0) { // wall forms of :3970-3975
+        pwx = (3.0 * uz[n] + uz[n + 1]) / 3.0;
+        pvx = (3.0 * uy[n] + uy[n + 1]) / 3.0;
+    } else if (i == g.lx - 1) {
+        pwx = -(3.0 * uz[n] + uz[n - 1]) / 3.0;
+        pvx = -(3.0 * uy[n] + uy[n - 1]) / 3.0;
+    } else {
+        pwx = (uz[n + 1] - uz[n - 1]) / 2.0;
+        pvx = (uy[n + 1] - uy[n - 1]) / 2.0;
+    }
+    // periodic images in y and z (exchng8, :4027-4054, on one rank)
+    const uint64_t yp = j + 1 == g.ly ? n - (uint64_t)(g.ly - 1) * g.lx : n + g.lx;
+    const uint64_t ym = j == 0 ? n + (uint64_t)(g.ly - 1) * g.lx : n - g.lx;
+    const uint64_t zp = k + 1 == g.lz ? n - (uint64_t)(g.lz - 1) * plane : n + plane;
+    const uint64_t zm = k == 0 ? n + (uint64_t)(g.lz - 1) * plane : n - plane;
+    const double pwy = (uz[yp] - uz[ym]) / 2.0;
+    const double puy = (ux[yp] - ux[ym]) / 2.0;
+    const double puz = (ux[zp] - ux[zm]) / 2.0;
+    const double pvz = (uy[zp] - uy[zm]) / 2.0;
+    ox[n] = pwy - pvz;
+    oy[n] = puz - pwx;
+    oz[n] = pvx - puy;
+}
+
+} // namespace
+
+int lbm_diag(lbm_channel *h, double ustar, double *out)
+{
+    CHECK_H(h);
+    if (!out)
+        return fail(LBM_ERR_ARG, "lbm_diag: null output");
+    if (h->ms == FROM_PREV) {
+        const int rc = materialize(h);
+        if (rc)
+            return rc;
+    }
+    DiagArgs a;
+    a.g = h->g;
+    a.L = h->lat[h->cur];
+    a.rho = h->mac[0];
+    a.ux = h->mac[1];
+    a.uy = h->mac[2];
+    a.uz = h->mac[3];
+    a.solid = solid_of(h);
+    a.f = h->fm;
+    a.rho_off = h->rho_off;
+    a.partials = h->partials;
+    // DIAG_Q partials per block; the scratch holds at least 16 384 doubles
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(1024, (h->g.N + 255) / 256);
+    int rc = dispatch_err(PREC_CALL(h, diag_partials, a, h->ms == ARRAYS ? 0 : 1, grid, h->stream), "diag");
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, h->stream, (const double *)h->partials, grid,
+                       h->red_out + 2);
+    HIP_TRY(hipGetLastError());
+    double t[DIAG_Q];
+    HIP_TRY(hipMemcpyAsync(t, h->red_out + 2, sizeof(t), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    // saveload.f90:1627-1640, in its order
+    const double rn = t[6];
+    double um = t[0] / rn, vm = t[1] / rn, wm = t[2] / rn;
+    const double ur = sqrt(t[3] / rn - um * um), vr = sqrt(t[4] / rn - vm * vm), wr = sqrt(t[5] / rn - wm * wm);
+    const long long idx = t[8] < 0.0 ? -1 : (long long)t[8];
+    out[0] = t[7];
+    out[1] = idx < 0 ? 0.0 : (double)(idx % h->g.lx + 1);
+    out[2] = idx < 0 ? 0.0 : (double)((idx / h->g.lx) % h->g.ly + 1);
+    out[3] = idx < 0 ? 0.0 : (double)(idx / ((long long)h->g.lx * h->g.ly) + 1);
+    out[4] = um / ustar;
+    out[5] = vm / ustar;
+    out[6] = wm / ustar;
+    out[7] = ur / ustar;
+    out[8] = vr / ustar;
+    out[9] = wr / ustar;
+    out[10] = 1. - rn / ((double)h->g.lx * (double)h->g.ly * (double)h->g.lz);
+    out[11] = t[9];
+    out[12] = t[10];
+    return LBM_OK;
+}
+
+int lbm_vortcalc(lbm_channel *h, double *ox, double *oy, double *oz)
+{
+    CHECK_H(h);
+    if (!ox || !oy || !oz)
+        return fail(LBM_ERR_ARG, "lbm_vortcalc: null output");
+    int rc = materialize(h);
+    if (rc)
+        return rc;
+    const size_t bytes = h->g.N * sizeof(double);
+    double *d = nullptr;
+    hipError_t e = hipMalloc((void **)&d, 3 * bytes);
+    if (e != hipSuccess)
+        return fail(LBM_ERR_NOMEM, "lbm_vortcalc: hipMalloc of %zu bytes failed: %s", 3 * bytes, hipGetErrorString(e));
+    const double *omg = h->part ? h->part->ps.W : nullptr;
+    hipLaunchKernelGGL(k_vortcalc, dim3((unsigned)((h->g.N + 255) / 256)), dim3(256), 0, h->stream, h->g,
+                       (const double *)h->mac[1], (const double *)h->mac[2], (const double *)h->mac[3], solid_of(h),
+                       omg, d, d + h->g.N, d + 2 * h->g.N);
+    e = hipGetLastError();
+    double *dst[3] = {ox, oy, oz};
+    for (int k = 0; k < 3 && e == hipSuccess; k++)
+        e = hipMemcpyAsync(dst[k], d + (size_t)k * h->g.N, bytes, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return fail(LBM_ERR_HIP, "lbm_vortcalc: %s", hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int lbm_dissipation(lbm_channel *h, double tau, double ystar, double *out)
+{
+    CHECK_H(h);
+    if (!out || !(tau > 0.0))
+        return fail(LBM_ERR_ARG, "lbm_dissipation: need an output and tau > 0");
+    if (h->ms == FROM_PREV) { // f is the current lattice's, the fields the previous macrovar's
+        const int rc = materialize(h);
+        if (rc)
+            return rc;
+    }
+    const lbm_config &c = h->cfg;
+    DissArgs a;
+    a.g = h->g;
+    a.L = h->lat[h->cur];
+    a.rho = h->mac[0];
+    a.ux = h->mac[1];
+    a.uy = h->mac[2];
+    a.uz = h->mac[3];
+    a.solid = solid_of(h);
+    a.f = h->fm;
+    a.rho_off = h->rho_off;
+    a.s1 = c.s1;
+    a.s9 = c.s9;
+    a.coef2 = c.coef2;
+    a.coef3 = c.coef3;
+    a.coef5 = c.coef5;
+    a.ww1 = c.ww1;
+    a.ww2 = c.ww2;
+    a.c15tau = 1.5 / tau;
+    a.partials = h->partials + 2 * RED_BLOCKS;
+    const uint32_t rows = h->g.ly * h->g.lz;
+    const uint32_t chunks = std::min<uint32_t>(PLANE_CHUNKS, rows);
+    int rc = dispatch_err(PREC_CALL(h, dissipation_partials, a, h->ms == ARRAYS ? 0 : 1, chunks, h->stream),
+                          "dissipation");
+    if (rc)
+        return rc;
+    const uint32_t width = NDISS * h->g.lx;
+    hipLaunchKernelGGL(k_sum_columns, dim3((width + 255) / 256), dim3(256), 0, h->stream, a.partials, chunks, width,
+                       h->red_out + 2, (double *)nullptr);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> t(width);
+    HIP_TRY(hipMemcpyAsync(t.data(), h->red_out + 2, width * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const int lx = h->g.lx;
+    for (int i = 1; i <= lx; i++) { // saveload.f90:4857-4875
+        const double nf = t[4 * (size_t)lx + i - 1];
+        double yplus = ((double)i - 0.5) / ystar;
+        if (i > lx / 2)
+            yplus = ((double)(lx - i) + 0.5) / ystar;
+        double *o = out + 6 * (size_t)(i - 1);
+        o[0] = (double)i - 0.5;
+        o[1] = yplus;
+        for (int q = 0; q < 4; q++)
+            o[2 + q] = t[(size_t)q * lx + i - 1] / nf;
+    }
+    return LBM_OK;
+}
+
 int lbm_stats_accumulate(lbm_channel *h)
 {
     CHECK_H(h);

@@ -76,6 +76,9 @@ _SIGS = {
     "lbm_mass_sum": [_H, ctypes.POINTER(ctypes.c_double)],
     "lbm_mass_sum_noalloc": [_H, _D],
     "lbm_statistc2": [_H, ctypes.c_double, ctypes.c_double, _D],
+    "lbm_diag": [_H, ctypes.c_double, _D],
+    "lbm_vortcalc": [_H, _D, _D, _D],
+    "lbm_dissipation": [_H, ctypes.c_double, ctypes.c_double, _D],
     "lbm_stats_accumulate": [_H],
     "lbm_stats_read": [_H, ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_longlong)],
     "lbm_stats_reset": [_H],
@@ -359,6 +362,26 @@ class Channel:
         """Plane profiles of Channel-Flow/saveload.f90:1348-1502; array (lx, 15)."""
         out = np.empty((self.lx, 15))
         _call("lbm_statistc2", self._h, self.p["ustar"], self.p["ystar"], _ptr(out))
+        return out
+
+    def diag(self):
+        """Run monitor of Channel-Flow/saveload.f90:1507-1676 on the device:
+        vmax, imout, jmout, kmout, umean, vmean, wmean, urms, vrms, wrms, volf,
+        rhomax, rhomin (means and rms / u*)."""
+        out = np.empty(13)
+        _call("lbm_diag", self._h, self.p["ustar"], _ptr(out))
+        return out
+
+    def vortcalc(self):
+        """Vorticity (ox, oy, oz), each (lz, ly, lx), of Channel-Flow/saveload.f90:3929-4023."""
+        out = [np.empty((self.lz, self.ly, self.lx)) for _ in range(3)]
+        _call("lbm_vortcalc", self._h, *[_ptr(a) for a in out])
+        return tuple(out)
+
+    def dissipation(self):
+        """dissipation_calc of Channel-Flow/saveload.f90:4644-4882; array (lx, 6)."""
+        out = np.empty((self.lx, 6))
+        _call("lbm_dissipation", self._h, self.p["tau"], self.p["ystar"], _ptr(out))
         return out
 
     def stats_accumulate(self):

@@ -25,6 +25,18 @@
       logical :: savecntd = .true.
       integer :: istpload = 0
       character(len=200) :: dirgenr = './'
+      ! checkpoint sets of an nprocY x nprocZ run of the reference
+      ! (para.f90:218-262): loadcntdflow reads nprocy_in x nprocz_in files,
+      ! savecntdflow writes nprocy_out x nprocz_out files
+      integer :: nprocy_in = 1, nprocz_in = 1, nprocy_out = 1, nprocz_out = 1
+      ! dissipation_calc cadence (saveload.f90:4644-4882; 0 = never)
+      integer :: ndiss = 0
+      ! particles (para.f90:332-379): positions from partfile, one "x y z"
+      ! line per sphere in node coordinates (node i at i - 0.5)
+      logical :: ipart = .false.
+      integer :: npart = 0
+      real(8) :: rad = 5.0d0, rhopart = 1.0d0, mingap = 3.0d0, mingap_w = 3.0d0
+      character(len=200) :: partfile = 'partpos.dat'
 
       ! time stepping
       integer :: istep, istep0, istat, imovie
@@ -47,10 +59,16 @@
       ! host copies (allocated by the routines that need them)
       real(8), allocatable, target :: f(:,:,:,:)
       real(8), allocatable, target :: rho(:,:,:), ux(:,:,:), uy(:,:,:), uz(:,:,:)
+      real(8), allocatable, target :: ox(:,:,:), oy(:,:,:), oz(:,:,:)
+      ! particle state (var_inc.f90:142-150): centres, velocities, angular
+      ! velocities, hydrodynamic force and torque, lubrication force
+      real(8), allocatable, target :: ypglb(:,:), wp(:,:), omgp(:,:), fHIp(:,:), torqp(:,:), flubp(:,:)
 
       namelist /channel/ lx, ly, lz, nsteps, ndiag, nflowout, nstat, npforcing, precision, &
                          device, mrttype_in, laminarflow, newrun, newinitflow,  &
-                         savecntd, istpload, dirgenr, rhoepsl
+                         savecntd, istpload, dirgenr, rhoepsl, nprocy_in, nprocz_in, &
+                         nprocy_out, nprocz_out, ndiss, ipart, npart, rad, rhopart, &
+                         mingap, mingap_w, partfile
 
       contains
 

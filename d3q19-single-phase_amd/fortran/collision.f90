@@ -82,3 +82,43 @@
       implicit none
       call lbm_check(lbm_forcingp(lbm, int(istep, c_int), force_in_y), 'FORCINGP')
       end subroutine FORCINGP
+
+      ! Particles (the beads_* routines of var_inc.f90:166-168): create npart
+      ! spheres at the centres listed in partfile, at rest, with the contact
+      ! constants of para.f90:357-360, the lubrication correction at the
+      ! fluid viscosity, and avedensity after every 100th step
+      ! (main.f90:163-167).
+      subroutine beads_init
+      use channel_vars
+      implicit none
+      type(lbm_particle_config) :: pc
+      integer :: u, i
+      allocate(ypglb(3,npart), wp(3,npart), omgp(3,npart), fHIp(3,npart), torqp(3,npart), flubp(3,npart))
+      open(newunit=u, file=trim(partfile), status='old', action='read')
+      do i = 1, npart
+        read(u, *) ypglb(:, i)
+      end do
+      close(u)
+      wp = 0.0d0
+      omgp = 0.0d0
+      pc = lbm_particle_config(npart, 0, rad, rhopart, mingap, mingap_w, 0.025d0, 0.002d0, 0.025d0, 0.002d0, &
+                               visc, 2.0d0/3.0d0, 0.01d0, 1, 100)
+      call lbm_check(lbm_particles_create(lbm, pc, ypglb, wp, omgp), 'beads_init')
+      end subroutine beads_init
+
+      ! n particle-laden main-loop steps from istep (beads_collision,
+      ! collision_MRT, macrovar, beads_move, beads_links, beads_filling).
+      subroutine run_particle_steps(n)
+      use channel_vars
+      implicit none
+      integer, intent(in) :: n
+      call lbm_check(lbm_run_particles(lbm, int(istep - 1, c_int), int(n, c_int)), 'run_particle_steps')
+      end subroutine run_particle_steps
+
+      ! The particle state into ypglb, wp, omgp, fHIp, torqp, flubp.
+      subroutine fetch_particles
+      use channel_vars
+      implicit none
+      call lbm_check(lbm_particles_get(lbm, c_loc(ypglb), c_loc(wp), c_loc(omgp), c_loc(fHIp), c_loc(torqp), &
+                     c_null_ptr, c_loc(flubp)), 'fetch_particles')
+      end subroutine fetch_particles

@@ -140,6 +140,27 @@
           type(c_ptr), value :: h
         end function
 
+        ! run monitor, vorticity, dissipation on the device (saveload.f90:1507-1676,
+        ! :3929-4023, :4644-4882)
+        integer(c_int) function lbm_diag(h, ustar, out) bind(C, name="lbm_diag")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double), value :: ustar
+          real(c_double), intent(out) :: out(13)
+        end function
+
+        integer(c_int) function lbm_vortcalc(h, ox, oy, oz) bind(C, name="lbm_vortcalc")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h, ox, oy, oz
+        end function
+
+        integer(c_int) function lbm_dissipation(h, tau, ystar, out) bind(C, name="lbm_dissipation")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double), value :: tau, ystar
+          real(c_double), intent(out) :: out(*)
+        end function
+
         ! time-averaged statistics (running device sums)
         integer(c_int) function lbm_stats_accumulate(h) bind(C, name="lbm_stats_accumulate")
           import :: c_int, c_ptr

@@ -60,6 +60,13 @@
       ! main.f90:132-137
       call FORCING
       call macrovar
+      if (ipart) then
+        if (npforcing > 0) then
+          write(*,*) 'the perturbation forcing is not used with particles (main.f90:148)'
+          error stop 2
+        end if
+        call beads_init
+      end if
       call lbm_check(lbm_sync(lbm), 'sync')
       call system_clock(t0, rate)
 
@@ -81,7 +88,14 @@
           nrun = istep0 + nsteps - istep + 1
           if (ndiag > 0) nrun = min(nrun, ndiag - mod(istep - 1, ndiag))
           if (nstat > 0) nrun = min(nrun, nstat - mod(istep - 1, nstat))
-          call run_steps(nrun)
+          if (ndiss > 0) nrun = min(nrun, ndiss - mod(istep - 1, ndiss))
+          if (ipart) then
+            ! the particle-laden loop body, avedensity after every 100th
+            ! step included (main.f90:157-167)
+            call run_particle_steps(nrun)
+          else
+            call run_steps(nrun)
+          end if
         end if
         istep = istep + nrun - 1
         if (ndiag > 0) then
@@ -89,6 +103,9 @@
         end if
         if (nstat > 0) then
           if (mod(istep, nstat) == 0) call statistc2
+        end if
+        if (ndiss > 0) then
+          if (mod(istep, ndiss) == 0) call dissipation_calc
         end if
         istep = istep + 1
       end do
@@ -99,6 +116,14 @@
       time_max = dble(t1 - t0)/dble(rate)
       call lbm_check(lbm_mass_sum(lbm, mass), 'mass')
       if (savecntd) call savecntdflow
+      if (ipart) then
+        call fetch_particles
+        open(newunit=u, file=trim(dirgenr)//'partfinal.dat', status='unknown', form='formatted')
+        do ios = 1, npart
+          write(u, '(9(1pe24.16))') ypglb(:, ios), wp(:, ios), omgp(:, ios)
+        end do
+        close(u)
+      end if
       write(*,*) 'time_max = ', time_max
       write(*,'(a,f12.2)') ' MLUPS = ', dble(lx)*dble(ly)*dble(lz)*dble(nsteps)/time_max/1.0d6
       write(*,*) 'mass = ', mass

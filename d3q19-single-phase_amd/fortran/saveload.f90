@@ -3,11 +3,14 @@
 !
 ! Files are Fortran unformatted sequential, one per rank (suffix .NNN):
 !   initflow/finit.000                      istat ; f          (saveload.f90:102-124)
-!   cntdflow/endrunflow2D16x8.SSSSSSS.000   istep, istat, imovie ; f
+!   cntdflow/endrunflow2D16x8.SSSSSSS.NNN   istep, istat, imovie ; f
 !                                                              (saveload.f90:196-231)
 ! f is the reference-layout f(0:18,lx,ly,lz) in FP64, moved to and from the
-! device with lbm_download_f / lbm_upload_f.
-! diag.dat and profiles2.dat follow saveload.f90:1507-1676 and :1348-1502.
+! device with lbm_download_f / lbm_upload_f.  Checkpoints are read from and
+! written as the set of files of an nprocY x nprocZ run (each rank its y/z
+! block of f, para.f90:218-262), one file when both are 1.
+! diag.dat, diag2.dat, profiles2.dat and dissipation.dat follow
+! saveload.f90:1507-1676, :1348-1502 and :4644-4882.
 !=============================================================================
       subroutine saveinitflow
       use channel_vars
@@ -31,78 +34,159 @@
       call lbm_check(lbm_upload_f(lbm, f), 'loadinitflow')
       end subroutine loadinitflow
 
-      function cntd_name(step) result(fnm)
+      function cntd_name(step, myid) result(fnm)
       use channel_vars
       implicit none
-      integer, intent(in) :: step
+      integer, intent(in) :: step, myid
       character(len=300) :: fnm
-      write(fnm, '(a,a,i7.7,a)') trim(dirgenr), 'cntdflow/endrunflow2D16x8.', step, '.000'
+      write(fnm, '(a,a,i7.7,a,i3.3)') trim(dirgenr), 'cntdflow/endrunflow2D16x8.', step, '.', myid
       end function cntd_name
 
+      ! The y/z block of rank myid of an npy x npz run (para.f90:218-262):
+      ! indy = mod(myid, npy), indz = myid/npy; the first mod(ly, npy)
+      ! columns (mod(lz, npz) rows) hold one node more.
+      subroutine rank_block(myid, npy, npz, y0, nyr, z0, nzr)
+      use channel_vars
+      implicit none
+      integer, intent(in) :: myid, npy, npz
+      integer, intent(out) :: y0, nyr, z0, nzr
+      integer :: indy, indz, i
+      indy = mod(myid, npy)
+      indz = myid/npy
+      y0 = 0
+      do i = 0, indy - 1
+        y0 = y0 + ly/npy
+        if (i < mod(ly, npy)) y0 = y0 + 1
+      end do
+      nyr = ly/npy
+      if (indy < mod(ly, npy)) nyr = nyr + 1
+      z0 = 0
+      do i = 0, indz - 1
+        z0 = z0 + lz/npz
+        if (i < mod(lz, npz)) z0 = z0 + 1
+      end do
+      nzr = lz/npz
+      if (indz < mod(lz, npz)) nzr = nzr + 1
+      end subroutine rank_block
+
+      ! savecntdflow (saveload.f90:196-231): one file per rank of an
+      ! nprocy_out x nprocz_out run, so that run can resume from it.
       subroutine savecntdflow
       use channel_vars
       implicit none
       character(len=300) :: cntd_name
+      integer :: myid, y0, nyr, z0, nzr
       call ensure_host_f
       call lbm_check(lbm_download_f(lbm, f), 'savecntdflow')
-      open(12, file=trim(cntd_name(istep)), status='unknown', form='unformatted')
-      write(12) istep, istat, imovie
-      write(12) f
-      close(12)
+      do myid = 0, nprocy_out*nprocz_out - 1
+        call rank_block(myid, nprocy_out, nprocz_out, y0, nyr, z0, nzr)
+        open(12, file=trim(cntd_name(istep, myid)), status='unknown', form='unformatted')
+        write(12) istep, istat, imovie
+        write(12) f(:, :, y0+1:y0+nyr, z0+1:z0+nzr)
+        close(12)
+      end do
       end subroutine savecntdflow
 
+      ! loadcntdflow (saveload.f90:296-332) from the checkpoint set of an
+      ! nprocy_in x nprocz_in run, assembled into the one device lattice (the
+      ! re-decomposition of loadcntdflow_frmmore, :336-381, for any y/z split).
       subroutine loadcntdflow
       use channel_vars
       implicit none
       character(len=300) :: cntd_name
+      integer :: myid, y0, nyr, z0, nzr, st0, st1, st2
+      real(8), allocatable :: blk(:,:,:,:)
       call ensure_host_f
-      open(12, file=trim(cntd_name(istpload)), status='old', form='unformatted')
-      read(12) istep0, istat, imovie
-      read(12) f
-      close(12)
+      do myid = 0, nprocy_in*nprocz_in - 1
+        call rank_block(myid, nprocy_in, nprocz_in, y0, nyr, z0, nzr)
+        allocate(blk(0:npop-1, lx, nyr, nzr))
+        open(12, file=trim(cntd_name(istpload, myid)), status='old', form='unformatted')
+        read(12) st0, st1, st2
+        read(12) blk
+        close(12)
+        if (myid == 0) then
+          istep0 = st0
+          istat = st1
+          imovie = st2
+        else if (st0 /= istep0) then
+          write(*,*) 'loadcntdflow: rank', myid, ' holds step', st0, ', rank 0 step', istep0
+          error stop 3
+        end if
+        f(:, :, y0+1:y0+nyr, z0+1:z0+nzr) = blk
+        deallocate(blk)
+      end do
       call lbm_check(lbm_upload_f(lbm, f), 'loadcntdflow')
       end subroutine loadcntdflow
 
-      ! Run monitor, saveload.f90:1535-1668 (all nodes fluid, one rank).
+      ! Run monitor, saveload.f90:1507-1676, from device reductions over the
+      ! fluid nodes (no field download): diag.dat, and diag2.dat with the
+      ! fastest and fastest-spinning particle when there are particles.
       subroutine diag
       use channel_vars
       implicit none
-      real(8) :: umean, vmean, wmean, urms, vrms, wrms, vmax, vel, rmax, rmin, rn
-      integer :: i, j, k, im, jm, km
-      call fetch_fields
-      rn = dble(lx)*dble(ly)*dble(lz)
-      umean = sum(ux)/rn
-      vmean = sum(uy)/rn
-      wmean = sum(uz)/rn
-      urms = sqrt(sum(ux*ux)/rn - umean**2)
-      vrms = sqrt(sum(uy*uy)/rn - vmean**2)
-      wrms = sqrt(sum(uz*uz)/rn - wmean**2)
-      vmax = 0.0d0
-      im = 0
-      jm = 0
-      km = 0
-      do k = 1, lz
-        do j = 1, ly
-          do i = 1, lx
-            vel = sqrt(ux(i,j,k)*ux(i,j,k) + uy(i,j,k)*uy(i,j,k) + uz(i,j,k)*uz(i,j,k))
-            if (vel > vmax) then
-              vmax = vel
-              im = i
-              jm = j
-              km = k
-            end if
-          end do
-        end do
-      end do
-      rmax = maxval(rho)
-      rmin = minval(rho)
+      real(8) :: out(13), wpmax, omgpmax, mag
+      integer :: i, idwp, idomgp
+      call lbm_check(lbm_diag(lbm, ustar, out), 'diag')
       open(26, file=trim(dirgenr)//'diag/diag.dat', status='unknown', form='formatted', position='append')
-      write(26, 260) dble(istep), vmax, im, jm, km, umean/ustar, vmean/ustar, wmean/ustar,  &
-                     urms/ustar, vrms/ustar, wrms/ustar, 0.0d0, rmax, rmin
+      write(26, 260) dble(istep), out(1), nint(out(2)), nint(out(3)), nint(out(4)), out(5:13)
       close(26)
-      write(*,*) istep, rmax, rmin
+      write(*,*) istep, out(12), out(13)
+      if (ipart) then
+        call fetch_particles
+        wpmax = 0.0d0
+        omgpmax = 0.0d0
+        idwp = 0
+        idomgp = 0
+        do i = 1, npart
+          mag = sqrt(wp(1,i)**2 + wp(2,i)**2 + wp(3,i)**2)
+          if (mag > wpmax) then
+            wpmax = mag
+            idwp = i
+          end if
+          mag = sqrt(omgp(1,i)**2 + omgp(2,i)**2 + omgp(3,i)**2)
+          if (mag > omgpmax) then
+            omgpmax = mag
+            idomgp = i
+          end if
+        end do
+        open(58, file=trim(dirgenr)//'diag/diag2.dat', status='unknown', form='formatted', position='append')
+        if (idwp > 0 .and. idomgp > 0) then
+          write(58, 370) dble(istep), wpmax, idwp, ypglb(:, idwp), omgpmax, idomgp, ypglb(:, idomgp)
+        else
+          write(58, 370) dble(istep), wpmax, idwp, 0.0d0, 0.0d0, 0.0d0, omgpmax, idomgp, 0.0d0, 0.0d0, 0.0d0
+        end if
+        close(58)
+      end if
 260   format(2x, 2(1pe16.6), 3I6, 9(1pe16.6))
+370   format(2x, 2(1pe16.6), 1I6, 3(1pe16.6), 1(1pe16.6), 1I6, 3(1pe16.6))
       end subroutine diag
+
+      ! dissipation_calc, saveload.f90:4644-4882, on the device: plane means
+      ! of S_ij S_ij and S_xy by the moment and population methods.
+      subroutine dissipation_calc
+      use channel_vars
+      implicit none
+      real(8), allocatable :: prof(:,:)
+      integer :: i
+      allocate(prof(6, lx))
+      call lbm_check(lbm_dissipation(lbm, tau, ystar, prof), 'dissipation_calc')
+      open(127, file=trim(dirgenr)//'stat/dissipation.dat', status='unknown', form='formatted', position='append')
+      write(127,*) istep
+      do i = 1, lx
+        write(127, 985) prof(:, i)
+      end do
+      close(127)
+      deallocate(prof)
+985   format(2x, 6(1pe16.6))
+      end subroutine dissipation_calc
+
+      ! vortcalc, saveload.f90:3929-4023, on the device into ox, oy, oz.
+      subroutine vortcalc
+      use channel_vars
+      implicit none
+      if (.not. allocated(ox)) allocate(ox(lx,ly,lz), oy(lx,ly,lz), oz(lx,ly,lz))
+      call lbm_check(lbm_vortcalc(lbm, c_loc(ox), c_loc(oy), c_loc(oz)), 'vortcalc')
+      end subroutine vortcalc
 
       ! Plane profiles, saveload.f90:1348-1502, computed on the device.
       subroutine statistc2

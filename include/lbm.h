@@ -197,6 +197,28 @@ int lbm_mass_sum_noalloc(lbm_channel *h, double *mass_dev);
  * (para.f90:64-66). */
 int lbm_statistc2(lbm_channel *h, double ustar, double ystar, double *out);
 
+/* Run monitor of Channel-Flow/saveload.f90:1507-1676 (diag) from deterministic
+ * device reductions over the fluid nodes of the current fields: out[13] =
+ * vmax, its node (imout, jmout, kmout, 1-based, the first in x, y, z order),
+ * umean, vmean, wmean, urms, vrms, wrms (each / ustar), the solid volume
+ * fraction, rhomax, rhomin -- the values diag.dat carries after the step
+ * number.  Synchronises; no field download. */
+int lbm_diag(lbm_channel *h, double ustar, double *out);
+
+/* vortcalc, Channel-Flow/saveload.f90:3929-4023: the vorticity of the current
+ * fields (central differences, periodic in y and z, the reference's one-sided
+ * forms at the walls; twice the angular velocity inside particles) into the
+ * host arrays ox, oy, oz (lx,ly,lz each).  Synchronises. */
+int lbm_vortcalc(lbm_channel *h, double *ox, double *oy, double *oz);
+
+/* dissipation_calc, Channel-Flow/saveload.f90:4644-4882: per wall-normal plane
+ * the fluid-node means of S_ij S_ij and S_xy from the non-equilibrium moments
+ * and from the non-equilibrium populations of the current f and fields.
+ * out is (6, lx) column-major: x, y+, <SS>_moments, <SS>_populations,
+ * <S_xy>_moments, <S_xy>_populations (the columns of dissipation.dat).  tau
+ * and ystar are the driver's (para.f90:66,106).  Synchronises. */
+int lbm_dissipation(lbm_channel *h, double tau, double ystar, double *out);
+
 /* Time-averaged statistics: lbm_stats_accumulate adds the current plane sums
  * of u_i, u_i u_j and rho, rho^2 (FP64) to running device sums;
  * lbm_stats_read returns the statistc2 columns of the running average

@@ -23,6 +23,8 @@
  *   orc_para            Channel-Flow/para.f90:59-214
  *   orc_statistc2       Channel-Flow/saveload.f90:1348-1502 (profile moments)
  *   orc_diag            Channel-Flow/saveload.f90:1507-1676 (run monitor)
+ *   orc_vortcalc        Channel-Flow/saveload.f90:3929-4023 (vorticity)
+ *   orc_dissipation     Channel-Flow/saveload.f90:4644-4882 (dissipation_calc)
  *
  * plus one function that is not a restatement but an equivalent schedule:
  *
@@ -833,4 +835,151 @@ void orc_diag(const orc_params *p, const double *rho, const double *ux, const do
     out[10] = 1. - rn / rn;
     out[11] = rmax;
     out[12] = rmin;
+}
+
+/* Channel-Flow/saveload.f90:3929-4023 (vortcalc) on one rank: vorticity by
+ * central differences in y and z (periodic, the exchng8 images of
+ * :4027-4054), in x central inside and the one-sided wall forms of
+ * :3970-3975 at ix = 1 and lx.  Nodes with solid[m] >= 0 (nullable) carry
+ * twice the angular velocity of particle solid[m] (omg, 3 per particle). */
+void orc_vortcalc(const orc_params *p, const double *ux, const double *uy, const double *uz, const int *solid,
+                  const double *omg, double *ox, double *oy, double *oz)
+{
+    const int lx = p->lx, ly = p->ly, lz = p->lz;
+    for (int k = 1; k <= lz; k++)
+        for (int j = 1; j <= ly; j++)
+            for (int i = 1; i <= lx; i++) {
+                const size_t m = MI(p, i, j, k);
+                if (solid && solid[m] >= 0) {
+                    const double *w = omg + 3 * (size_t)solid[m];
+                    ox[m] = 2.0 * w[0];
+                    oy[m] = 2.0 * w[1];
+                    oz[m] = 2.0 * w[2];
+                    continue;
+                }
+                double pwx, pvx, pwy, puy, puz, pvz;
+                if (i == 1) {
+                    pwx = (3.0 * uz[m] + uz[MI(p, i + 1, j, k)]) / 3.0;
+                    pvx = (3.0 * uy[m] + uy[MI(p, i + 1, j, k)]) / 3.0;
+                } else if (i == lx) {
+                    pwx = -(3.0 * uz[m] + uz[MI(p, i - 1, j, k)]) / 3.0;
+                    pvx = -(3.0 * uy[m] + uy[MI(p, i - 1, j, k)]) / 3.0;
+                } else {
+                    pwx = (uz[MI(p, i + 1, j, k)] - uz[MI(p, i - 1, j, k)]) / 2.0;
+                    pvx = (uy[MI(p, i + 1, j, k)] - uy[MI(p, i - 1, j, k)]) / 2.0;
+                }
+                const int jp = j == ly ? 1 : j + 1, jm = j == 1 ? ly : j - 1;
+                const int kp = k == lz ? 1 : k + 1, km = k == 1 ? lz : k - 1;
+                pwy = (uz[MI(p, i, jp, k)] - uz[MI(p, i, jm, k)]) / 2.0;
+                puy = (ux[MI(p, i, jp, k)] - ux[MI(p, i, jm, k)]) / 2.0;
+                puz = (ux[MI(p, i, j, kp)] - ux[MI(p, i, j, km)]) / 2.0;
+                pvz = (uy[MI(p, i, j, kp)] - uy[MI(p, i, j, km)]) / 2.0;
+                ox[m] = pwy - pvz;
+                oy[m] = puz - pwx;
+                oz[m] = pvx - puy;
+            }
+}
+
+/* The per-node part of Channel-Flow/saveload.f90:4677-4776 (dissipation_calc):
+ * the strain-rate magnitude S_ij S_ij and S_xy from the non-equilibrium
+ * moments (method 1) and from the non-equilibrium populations (method 2). */
+void orc_dissipation_node(const orc_params *p, const double *f9, double rho9, double ux9, double uy9, double uz9,
+                          double *out4)
+{
+    const double ux9s = ux9 * ux9, uy9s = uy9 * uy9, uz9s = uz9 * uz9;
+    const double eqm1 = -11.0 * rho9 + 19.0 * (ux9s + uy9s + uz9s);
+    const double eqm6 = 2.0 * ux9s - uy9s - uz9s;
+    const double eqm8 = uy9s - uz9s;
+    const double eqm10 = ux9 * uy9;
+    const double eqm11 = uy9 * uz9;
+    const double eqm12 = ux9 * uz9;
+    const double sum1 = f9[1] + f9[2] + f9[3] + f9[4] + f9[5] + f9[6];
+    const double sum2 = f9[7] + f9[8] + f9[9] + f9[10] + f9[11] + f9[12] + f9[13] + f9[14] + f9[15] + f9[16] +
+                        f9[17] + f9[18];
+    const double sum6 = f9[1] + f9[2];
+    const double sum7 = f9[3] + f9[4] + f9[5] + f9[6];
+    const double sum8 = f9[7] + f9[8] + f9[9] + f9[10] + f9[11] + f9[12] + f9[13] + f9[14];
+    const double sum9 = f9[15] + f9[16] + f9[17] + f9[18];
+    const double sum10 = f9[3] + f9[4] - f9[5] - f9[6];
+    const double sum11 = f9[7] + f9[8] + f9[9] + f9[10] - f9[11] - f9[12] - f9[13] - f9[14];
+    const double evlm1 = -30.0 * f9[0] + p->coef2 * sum1 + p->coef3 * sum2;
+    const double evlm6 = p->coef5 * sum6 - sum7 + sum8 - p->coef5 * sum9;
+    const double evlm8 = sum10 + sum11;
+    const double evlm10 = f9[7] - f9[8] - f9[9] + f9[10];
+    const double evlm11 = f9[15] - f9[16] - f9[17] + f9[18];
+    const double evlm12 = f9[11] - f9[12] - f9[13] + f9[14];
+    const double neqm1 = p->s1 * (evlm1 - eqm1);
+    const double neqm9 = p->s9 * (evlm6 - eqm6);
+    const double neqm11 = p->s9 * (evlm8 - eqm8);
+    const double neqm13 = p->s9 * (evlm10 - eqm10);
+    const double neqm14 = p->s9 * (evlm11 - eqm11);
+    const double neqm15 = p->s9 * (evlm12 - eqm12);
+    double Sxx = -(neqm1 + 19.0 * neqm9) / 38.0;
+    double Syy = -(2.0 * neqm1 - 19.0 * (neqm9 - 3.0 * neqm11)) / 76.0;
+    double Szz = -(2.0 * neqm1 - 19.0 * (neqm9 + 3.0 * neqm11)) / 76.0;
+    double Sxy = -1.5 * neqm13;
+    double Syz = -1.5 * neqm14;
+    double Szx = -1.5 * neqm15;
+    out4[0] = Sxx * Sxx + Syy * Syy + Szz * Szz + 2.0 * (Sxy * Sxy + Syz * Syz + Szx * Szx);
+    out4[1] = Sxy;
+    Sxx = 0.0;
+    Syy = 0.0;
+    Szz = 0.0;
+    Sxy = 0.0;
+    Syz = 0.0;
+    Szx = 0.0;
+    const double usqr = 1.5 * (ux9s + uy9s + uz9s);
+    for (int ip = 1; ip < NPOP; ip++) {
+        const double wght = ip <= 6 ? p->ww1 : p->ww2;
+        const double edtu = (double)cix[ip] * ux9 + (double)ciy[ip] * uy9 + (double)ciz[ip] * uz9;
+        const double feq9 = wght * (rho9 + 3.0 * edtu + 4.5 * (edtu * edtu) - usqr);
+        const double fneq9 = f9[ip] - feq9;
+        Sxx = Sxx + fneq9 * (double)(cix[ip] * cix[ip]);
+        Syy = Syy + fneq9 * (double)(ciy[ip] * ciy[ip]);
+        Szz = Szz + fneq9 * (double)(ciz[ip] * ciz[ip]);
+        Sxy = Sxy + fneq9 * (double)(cix[ip] * ciy[ip]);
+        Syz = Syz + fneq9 * (double)(ciy[ip] * ciz[ip]);
+        Szx = Szx + fneq9 * (double)(ciz[ip] * cix[ip]);
+    }
+    const double Sij2m2_tmp = Sxx * Sxx + Syy * Syy + Szz * Szz + 2.0 * (Sxy * Sxy + Syz * Syz + Szx * Szx);
+    out4[2] = Sij2m2_tmp * ((1.5 / p->tau) * (1.5 / p->tau));
+    out4[3] = Sxy * (1.5 / p->tau);
+}
+
+/* Channel-Flow/saveload.f90:4644-4882 (dissipation_calc) on one rank: per
+ * wall-normal plane i the fluid-node means of S_ij S_ij and S_xy by the two
+ * methods, from the populations f (reference layout) and the fields.
+ * out is (6, lx) column-major: x, y+, <S_ij S_ij>_1, <S_ij S_ij>_2,
+ * <S_xy>_1, <S_xy>_2 (the columns written at :4873). */
+void orc_dissipation(const orc_params *p, const double *f, const double *rho, const double *ux, const double *uy,
+                     const double *uz, const int *solid, double *out)
+{
+    const int lx = p->lx, ly = p->ly, lz = p->lz;
+    for (int i = 1; i <= lx; i++) {
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        int nf = 0;
+        for (int k = 1; k <= lz; k++)
+            for (int j = 1; j <= ly; j++) {
+                const size_t m = MI(p, i, j, k);
+                if (solid && solid[m] >= 0)
+                    continue;
+                double v[4];
+                orc_dissipation_node(p, &f[FI(p, 0, i, j, k)], rho[m], ux[m], uy[m], uz[m], v);
+                s[0] = s[0] + v[0];
+                s[1] = s[1] + v[2];
+                s[2] = s[2] + v[1];
+                s[3] = s[3] + v[3];
+                nf++;
+            }
+        double yplus = ((double)i - 0.5) / p->ystar;
+        if (i > lx / 2)
+            yplus = ((double)(lx - i) + 0.5) / p->ystar;
+        double *o = &out[6 * (size_t)(i - 1)];
+        o[0] = (double)i - 0.5;
+        o[1] = yplus;
+        o[2] = s[0] / (double)nf;
+        o[3] = s[1] / (double)nf;
+        o[4] = s[2] / (double)nf;
+        o[5] = s[3] / (double)nf;
+    }
 }

@@ -40,6 +40,12 @@ void orc_statistc2(const orc_params *p, const double *rho, const double *ux, con
                    const double *uz, double *out);
 void orc_diag(const orc_params *p, const double *rho, const double *ux, const double *uy, const double *uz,
               double *out);
+void orc_vortcalc(const orc_params *p, const double *ux, const double *uy, const double *uz, const int *solid,
+                  const double *omg, double *ox, double *oy, double *oz);
+void orc_dissipation_node(const orc_params *p, const double *f9, double rho9, double ux9, double uy9, double uz9,
+                          double *out4);
+void orc_dissipation(const orc_params *p, const double *f, const double *rho, const double *ux, const double *uy,
+                     const double *uz, const int *solid, double *out);
 
 #ifdef __cplusplus
 }

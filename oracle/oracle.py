@@ -17,6 +17,8 @@ under their own names, each calling the C restatement in lbm_oracle.c:
     initvel         Channel-Flow/initial.f90:75-147
     statistc2       Channel-Flow/saveload.f90:1348-1502
     diag            Channel-Flow/saveload.f90:1507-1676
+    vortcalc        Channel-Flow/saveload.f90:3929-4023
+    dissipation     Channel-Flow/saveload.f90:4644-4882 (dissipation_calc)
 
 Array layout is the reference's: f(0:18,lx,ly,lz) column-major, held here as a
 C-ordered numpy array of shape (lz, ly, lx, 19); macroscopic fields (lz, ly, lx).
@@ -82,6 +84,9 @@ def lib():
         L.orc_initvel.argtypes = [P, D, D, D]
         L.orc_statistc2.argtypes = [P] + [D] * 5
         L.orc_diag.argtypes = [P] + [D] * 5
+        L.orc_vortcalc.argtypes = [P] + [D] * 8
+        L.orc_dissipation_node.argtypes = [P, D, _DBL, _DBL, _DBL, _DBL, D]
+        L.orc_dissipation.argtypes = [P] + [D] * 7
         _lib = L
     return _lib
 
@@ -170,6 +175,23 @@ class OracleChannel:
         out = np.empty(13)
         lib().orc_diag(ctypes.byref(self.p), _ptr(self.rho), _ptr(self.ux), _ptr(self.uy),
                        _ptr(self.uz), _ptr(out))
+        return out
+
+    def vortcalc(self, solid=None, omg=None):
+        """Channel-Flow/saveload.f90:3929-4023; (ox, oy, oz) of the current fields."""
+        out = [np.empty_like(self.ux) for _ in range(3)]
+        sp = None if solid is None else np.ascontiguousarray(solid, dtype=np.int32).ctypes.data_as(ctypes.c_void_p)
+        op = None if omg is None else _ptr(np.ascontiguousarray(omg, dtype=np.float64))
+        lib().orc_vortcalc(ctypes.byref(self.p), _ptr(self.ux), _ptr(self.uy), _ptr(self.uz), sp, op,
+                           *(_ptr(a) for a in out))
+        return tuple(out)
+
+    def dissipation(self, solid=None):
+        """Channel-Flow/saveload.f90:4644-4882; array (lx, 6)."""
+        out = np.empty((self.p.lx, 6))
+        sp = None if solid is None else np.ascontiguousarray(solid, dtype=np.int32).ctypes.data_as(ctypes.c_void_p)
+        lib().orc_dissipation(ctypes.byref(self.p), _ptr(self.f), _ptr(self.rho), _ptr(self.ux), _ptr(self.uy),
+                              _ptr(self.uz), sp, _ptr(out))
         return out
 
     # --- driver sequences (Channel-Flow/main.f90) --------------------------

@@ -149,3 +149,160 @@ def test_driver_batched_steps_between_outputs_bitwise(tmp_path):
     assert np.array_equal(f.view(np.uint64), ref.f.view(np.uint64))
     diag = (tmp_path / "diag" / "diag.dat").read_text().split("\n")
     assert len([l for l in diag if l.strip()]) == 3
+
+
+def test_rank_set_assembles_and_splits_bitwise(tmp_path):
+    """A checkpoint set of a 2 x 4 run (nprocY x nprocZ, para.f90:218-262),
+    uneven blocks included, assembles into one lattice and splits back into
+    the same files, bit for bit."""
+    lx, ny, nz = 6, 13, 10
+    f = workloads.uniform(5, 19 * lx * ny * nz).reshape(nz, ny, lx, 19)
+    d = tmp_path / "set"
+    d.mkdir()
+    fortio.write_cntd_ranks(str(d), f, 1200, 2, 4, istat=3, imovie=1)
+    names = sorted(p.name for p in d.iterdir())
+    assert names == [f"endrunflow2D16x8.0001200.{r:03d}" for r in range(8)]
+    blocks = fortio.rank_blocks(ny, nz, 2, 4)
+    assert [b[2] for b in blocks[:2]] == [7, 6] and [b[4] for b in blocks[::2]] == [3, 3, 2, 2]
+    # rank 5: indy 1, indz 2 -> y 7..12, z 6..7
+    (_, g5) = fortio.read_cntd(str(d / "endrunflow2D16x8.0001200.005"), (2, 6, lx, 19))
+    assert np.array_equal(g5, f[6:8, 7:13])
+    head, g = fortio.read_cntd_ranks(str(d), 1200, lx, ny, nz, 2, 4)
+    assert head == (1200, 3, 1) and np.array_equal(g.view(np.uint64), f.view(np.uint64))
+    d2 = tmp_path / "again"
+    d2.mkdir()
+    fortio.write_cntd_ranks(str(d2), g, 1200, 2, 4, istat=3, imovie=1)
+    for n in names:
+        assert (d / n).read_bytes() == (d2 / n).read_bytes()
+    # finit sets and the older z-slab sets
+    fortio.write_finit_ranks(str(d), f, 4, 2, istat=9)
+    assert fortio.read_finit_ranks(str(d), lx, ny, nz, 4, 2)[1].tobytes() == f.tobytes()
+    for r in range(5):
+        fortio.write_cntd(str(d / f"endrunflow.000077.{r:03d}"), f[2 * r:2 * r + 2], 77)
+    assert np.array_equal(fortio.read_endrunflow_zslabs(str(d), 77, lx, ny, nz, 5)[1], f)
+
+
+def _dirs(tmp_path):
+    for d in ("initflow", "cntdflow", "diag", "stat"):
+        (tmp_path / d).mkdir(exist_ok=True)
+
+
+@pytest.mark.gpu
+def test_driver_restart_from_a_rank_set_bitwise(tmp_path):
+    """main.f90:118-121: 30 steps, savecntdflow as the 8 files of a 2 x 4
+    run, then newrun=.false. loads that set into one lattice and runs 30
+    more: equal to 60 straight steps bit for bit."""
+    lx, ly, lz = 16, 12, 8
+    ref = O.OracleChannel(lx, ly, lz)
+    ref.ux[:], ref.uy[:], ref.uz[:] = workloads.laminar_parabola_noise(lx, ly, lz, seed=24)
+    ref.initpop()
+    runs = {}
+    for name in ("straight", "split"):
+        d = tmp_path / name
+        d.mkdir()
+        _dirs(d)
+        fortio.write_finit(str(d / "initflow" / "finit.000"), ref.f)
+        runs[name] = d
+    _nml(runs["straight"] / "run.nml", lx=lx, ly=ly, lz=lz, nsteps=60, newinitflow=".false.", ndiag=0,
+         dirgenr=f"'{runs['straight']}/'")
+    _run(runs["straight"] / "run.nml", runs["straight"])
+    d = runs["split"]
+    _nml(d / "a.nml", lx=lx, ly=ly, lz=lz, nsteps=30, newinitflow=".false.", ndiag=0, nprocy_out=2,
+         nprocz_out=4, dirgenr=f"'{d}/'")
+    _run(d / "a.nml", d)
+    assert len(list((d / "cntdflow").glob("endrunflow2D16x8.0000030.*"))) == 8
+    _nml(d / "b.nml", lx=lx, ly=ly, lz=lz, nsteps=30, newrun=".false.", istpload=30, nprocy_in=2, nprocz_in=4,
+         ndiag=0, dirgenr=f"'{d}/'")
+    _run(d / "b.nml", d)
+    shape = ref.f.shape
+    (istep, _, _), f60 = fortio.read_cntd(str(runs["straight"] / "cntdflow" / "endrunflow2D16x8.0000060.000"), shape)
+    (istep2, _, _), g60 = fortio.read_cntd(str(d / "cntdflow" / "endrunflow2D16x8.0000060.000"), shape)
+    assert istep == istep2 == 60
+    assert np.array_equal(f60.view(np.uint64), g60.view(np.uint64))
+
+
+def _diag_rows(path):
+    rows = []
+    for line in path.read_text().splitlines():
+        if line.strip():
+            v = line.split()
+            rows.append([float(v[0]), float(v[1])] + [int(x) for x in v[2:5]] + [float(x) for x in v[5:]])
+    return rows
+
+
+@pytest.mark.gpu
+def test_driver_diag_values_match_the_oracle(tmp_path):
+    """diag.dat (from lbm_diag on the device, no field download) carries the
+    oracle's run monitor at steps 20 and 40 to its printed digits; the
+    dissipation file has lx rows per output."""
+    lx, ly, lz = 16, 12, 8
+    _dirs(tmp_path)
+    ref = O.OracleChannel(lx, ly, lz)
+    ref.ux[:], ref.uy[:], ref.uz[:] = workloads.laminar_parabola_noise(lx, ly, lz, seed=25)
+    ref.initpop()
+    fortio.write_finit(str(tmp_path / "initflow" / "finit.000"), ref.f)
+    _nml(tmp_path / "run.nml", lx=lx, ly=ly, lz=lz, nsteps=40, newinitflow=".false.", ndiag=20, ndiss=40,
+         dirgenr=f"'{tmp_path}/'")
+    _run(tmp_path / "run.nml", tmp_path)
+    rows = _diag_rows(tmp_path / "diag" / "diag.dat")
+    ref.FORCING()
+    ref.macrovar()
+    want = []
+    for _ in range(2):
+        ref.run(20)
+        want.append(ref.diag())
+    assert len(rows) == 2
+    for r, w, step in zip(rows, want, (20, 40)):
+        assert r[0] == step
+        assert r[2:5] == [int(w[1]), int(w[2]), int(w[3])]
+        got = np.array([r[1]] + r[5:])
+        exp = np.concatenate([[w[0]], w[4:13]])
+        np.testing.assert_allclose(got, exp, rtol=1e-6, atol=1e-12)
+    lines = (tmp_path / "stat" / "dissipation.dat").read_text().splitlines()
+    assert lines[0].strip() == "40" and len(lines) == 1 + lx
+
+
+def _post_from_f(f):
+    """The post-collision lattice whose pull is f (reference layout): P(y)_i
+    = f(y + c_i)_i, or f(y)_opp(i) where y + c_i lies beyond a wall."""
+    lz, ly, lx, _ = f.shape
+    CX = [0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0]
+    CY = [0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1]
+    CZ = [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1]
+    OPP = [0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15]
+    P = np.empty((19, lz, ly, lx))
+    for i in range(19):
+        s = np.roll(f[..., i], (-CZ[i], -CY[i], -CX[i]), axis=(0, 1, 2))
+        if CX[i] == 1:
+            s[:, :, -1] = f[:, :, -1, OPP[i]]
+        elif CX[i] == -1:
+            s[:, :, 0] = f[:, :, 0, OPP[i]]
+        P[i] = s
+    return P.reshape(-1)
+
+
+@pytest.mark.gpu
+def test_driver_particle_laden_run_bitwise(tmp_path):
+    """ipart: three spheres of radius 5 at the turbulent constants on the
+    32x64x32 channel, 150 steps through lbm_run_particles with diag in
+    between (avedensity after step 100 included): the final particle state
+    equals the oracle's bit for bit; diag2.dat reports the particles."""
+    lx, ly, lz, nsteps = 32, 64, 32, 150
+    _dirs(tmp_path)
+    pos = workloads.sphere_positions(lx, ly, lz, 3, 5.0, seed=3)
+    np.savetxt(tmp_path / "partpos.dat", pos, fmt="%.17g")
+    ref = O.OracleChannel(lx, ly, lz)
+    ref.ux[:], ref.uy[:], ref.uz[:] = workloads.laminar_parabola_noise(lx, ly, lz, seed=26)
+    ref.initpop()
+    fortio.write_finit(str(tmp_path / "initflow" / "finit.000"), ref.f)
+    _nml(tmp_path / "run.nml", lx=lx, ly=ly, lz=lz, nsteps=nsteps, newinitflow=".false.", ndiag=50,
+         ipart=".true.", npart=3, rad=5.0, partfile=f"'{tmp_path}/partpos.dat'", dirgenr=f"'{tmp_path}/'")
+    _run(tmp_path / "run.nml", tmp_path)
+    final = np.loadtxt(tmp_path / "partfinal.dat")
+    p = O.para(lx, ly, lz)
+    g = O.particle_geometry(lx, ly, lz, 3, 5.0, visc=p.visc, force=(0.0, p.force_in_y * p.force_mag, 0.0))
+    o = O.OracleParticleChannel(p, g, _post_from_f(ref.f), pos)
+    o.step(nsteps, istep0=0)
+    for got, want in ((final[:, 0:3], o.X), (final[:, 3:6], o.V), (final[:, 6:9], o.W)):
+        assert np.array_equal(np.ascontiguousarray(got).reshape(-1).view(np.uint64), want.view(np.uint64))
+    assert len([l for l in (tmp_path / "diag" / "diag2.dat").read_text().splitlines() if l.strip()]) == 3

@@ -364,3 +364,83 @@ def test_diag_restatement_matches_closed_forms():
     np.testing.assert_allclose(out[4:10], [um / us, vm / us, wm / us, ur / us, vr / us, wr / us], rtol=1e-10)
     assert out[10] == 0.0
     assert (out[11], out[12]) == (rho.max(), rho.min())
+
+
+def test_vortcalc_restatement_matches_closed_forms():
+    """saveload.f90:3929-4023 on ux = b sin(2 pi z/lz), uy = c x (L - x),
+    uz = a sin(2 pi y/ly) (x = i - 1/2 from the lower wall, y = j - 1/2,
+    z = k - 1/2): the central differences of a sine are cos times
+    sin(2 pi / L) exactly; the wall forms (3 u_1 + u_2)/3 and
+    -(3 u_lx + u_lx-1)/3 are the exact derivative of a parabola that vanishes
+    at the walls, and give 4u/3 and -4u/3 for a field constant in x.
+    Nodes inside a particle carry twice its angular velocity."""
+    lx, ly, lz = 12, 10, 8
+    a, b, c = 0.013, -0.021, 0.0007
+    o = O.OracleChannel(lx, ly, lz)
+    x = np.arange(lx) + 0.5
+    y = np.arange(ly) + 0.5
+    z = np.arange(lz) + 0.5
+    shape = (lz, ly, lx)
+    o.ux[:] = np.broadcast_to(b * np.sin(2 * np.pi * z / lz)[:, None, None], shape)
+    o.uy[:] = np.broadcast_to(c * x * (lx - x), shape)
+    o.uz[:] = np.broadcast_to(a * np.sin(2 * np.pi * y / ly)[None, :, None], shape)
+    solid = np.full(shape, -1, dtype=np.int32)
+    solid[3, 4, 5] = 1
+    omg = np.array([[9.0, 9.0, 9.0], [0.01, -0.02, 0.03]])
+    ox, oy, oz = o.vortcalc(solid=solid, omg=omg)
+    pwy = a * np.cos(2 * np.pi * y / ly)[None, :, None] * np.sin(2 * np.pi / ly)
+    puz = b * np.cos(2 * np.pi * z / lz)[:, None, None] * np.sin(2 * np.pi / lz)
+    uz = np.broadcast_to(o.uz, shape)
+    pwx = np.zeros(shape)
+    pwx[:, :, 0] = 4.0 * uz[:, :, 0] / 3.0
+    pwx[:, :, -1] = -4.0 * uz[:, :, -1] / 3.0
+    pvx = np.broadcast_to(c * (lx - 2 * x), shape)
+    want = [np.broadcast_to(pwy, shape).copy(), np.broadcast_to(puz, shape) - pwx, pvx.copy()]
+    for w, k in zip(want, range(3)):
+        w[3, 4, 5] = 2.0 * omg[1, k]
+    for got, w in zip((ox, oy, oz), want):
+        assert np.abs(got - w).max() < 1e-15
+
+
+def _strained_state(lx, ly, lz, seed):
+    """f = equilibrium (initpop form) + w_i (9/2) Q : (c_i c_i - I/3) with a
+    symmetric traceless Q that grows across the channel, Q(x) = Q0 (1 + x/lx):
+    the non-equilibrium second moment is exactly Q, its trace and the
+    non-equilibrium energy and density vanish."""
+    rng = np.random.default_rng(seed)
+    Q0 = rng.uniform(-1e-4, 1e-4, (3, 3))
+    Q0 = 0.5 * (Q0 + Q0.T)
+    Q0 -= np.trace(Q0) / 3 * np.eye(3)
+    c = np.stack([CX, CY, CZ], 1).astype(float)
+    rho = rng.uniform(-1e-3, 1e-3, (lz, ly, lx))
+    u = rng.uniform(-0.03, 0.03, (3, lz, ly, lx))
+    cu = np.einsum("ia,azyx->zyxi", c, u)
+    usq = 1.5 * (u ** 2).sum(0)
+    feq = W * (rho[..., None] + 3 * cu + 4.5 * cu * cu - usq[..., None])
+    ccq = np.einsum("ia,ab,ib->i", c, Q0, c) - np.trace(Q0) / 3 * (c * c).sum(1)
+    scale = 1.0 + (np.arange(lx) + 0.5) / lx
+    fneq = W * 4.5 * ccq * scale[None, None, :, None]
+    return feq + fneq, rho, u, Q0, scale
+
+
+def test_dissipation_restatement_matches_closed_forms():
+    """saveload.f90:4644-4882 on a state whose non-equilibrium stress is a
+    known traceless Q: the moment method gives S = -(3 s9 / 2) Q and the
+    population method S = (3 / (2 tau)) Q, so with s9 = 1/tau both give
+    S_ij S_ij = (1.5/tau)^2 Q:Q and S_xy of opposite signs; the plane means
+    follow Q's growth across the channel; x and y+ columns as statistc2's."""
+    lx, ly, lz = 10, 6, 4
+    o = O.OracleChannel(lx, ly, lz)
+    f, rho, u, Q0, scale = _strained_state(lx, ly, lz, 3)
+    o.f[:] = f
+    o.rho[:], o.ux[:], o.uy[:], o.uz[:] = rho, u[0], u[1], u[2]
+    out = o.dissipation()
+    k = 1.5 / o.p.tau
+    ss = k * k * (Q0 * Q0).sum() * scale ** 2
+    sxy = k * Q0[0, 1] * scale
+    i = np.arange(1, lx + 1)
+    ys = o.p.ystar
+    assert np.array_equal(out[:, 0], i - 0.5)
+    np.testing.assert_allclose(out[:, 1], np.where(i > lx // 2, (lx - i + 0.5) / ys, (i - 0.5) / ys), rtol=1e-15)
+    for col, want in ((2, ss), (3, ss), (4, -sxy), (5, sxy)):
+        np.testing.assert_allclose(out[:, col], want, rtol=1e-9, atol=0, err_msg=f"column {col}")
