@@ -25,12 +25,13 @@ namespace d3q19 {
 constexpr int BLOCK = 256;
 // Threads per block of the step kernels k_step and k_step_closure, measured
 // against each other on one box at a time: 512 for FP64 (18 320 against
-// 18 150 MLUPS with 256 on the 192x768x384 channel), FP16 storage (71 600
-// against 68 100) and the fused closure (22 000 against 21 200 on the
-// 96x384x192 channel); 256 for the plain FP32 step (39 220-39 330 against
-// 37 990-38 800 with 512 on the 2048x1024x512 channel).  128 was slower.
+// 18 150 MLUPS with 256 on the 192x768x384 channel) and FP16 storage (71 600
+// against 68 100); 256 for the plain FP32 step (39 220-39 330 against
+// 37 990-38 800 with 512 on the 2048x1024x512 channel) and the fused closure
+// (96 registers: five 256-thread blocks per CU against two of 512, 248 against
+// 255 us per step on the 96x384x192 channel).  128 was slower.
 template <int P, int MODE>
-constexpr int step_block() { return (P == 1 && (MODE & 8) == 0) ? 256 : 512; }
+constexpr int step_block() { return ((MODE & 8) != 0 || P == 1) ? 256 : 512; }
 
 // Element `node` of a population row, addressed as the row base plus a 32-bit
 // byte offset when that cannot wrap: FP16 storage, whose rows hold fewer than
@@ -216,17 +217,19 @@ __device__ __forceinline__ void step_node(const StepArgs &a)
     // issues with the flag load instead of waiting for it) but not stored
     if constexpr ((MODE & 4) != 0)
         active = active && a.solid[n] < 0;
+    const Nbr b = make_nbr(n, a.g);
+    const S *__restrict__ src = (const S *)a.src;
+    S *__restrict__ dst = (S *)a.dst;
+
+    // the 19 loads issue first; the closure's weights are staged while they
+    // are in flight
+    T f[19];
+    pull_all<P>(src, a.g.N, b, f);
     __shared__ __attribute__((aligned(16))) _Float16 wl[WAVE ? MLP_LDS : 8];
     if constexpr (WAVE) {
         mlp_stage(a.mlp, wl, threadIdx.x, NB);
         __syncthreads();
     }
-    const Nbr b = make_nbr(n, a.g);
-    const S *__restrict__ src = (const S *)a.src;
-    S *__restrict__ dst = (S *)a.dst;
-
-    T f[19];
-    pull_all<P>(src, a.g.N, b, f);
 
     Rates<T> r = rates_as<T>(a.r);
     T fmx = r.fx, fmy = r.fy, fmz = r.fz; // force in u = j + F/2 of the fused fields
@@ -309,8 +312,8 @@ __global__ void __launch_bounds__((step_block<P, MODE>())) k_step(StepArgs a)
     step_node<P, MODE>(a);
 }
 
-// The closure modes, held to 128 registers so that four waves per SIMD keep
-// the lattice traffic in flight while others run the network.
+// The closure modes, held to 128 registers so that at least four waves per
+// SIMD keep the lattice traffic in flight while others run the network.
 template <int P, int MODE>
 __global__ void __launch_bounds__((step_block<P, MODE>()), 4) k_step_closure(StepArgs a)
 {
@@ -465,7 +468,7 @@ __global__ void __launch_bounds__(BLOCK) k_closure_eval(StepArgs a, uint64_t n_b
     closure_inputs(m, eq, neq);
     const float wd = fminf((float)ix + 0.5f, (float)a.g.lx - (float)ix - 0.5f);
     float in[MLP_IN];
-    mlp_inputs(a.mlp, neq, wd * a.inv_ystar / a.yscale, in);
+    mlp_inputs_dev(a.mlp, neq, wd * a.inv_ystar / a.yscale, in);
     if constexpr (WHAT == 0) {
         if (active)
 #pragma unroll

@@ -2018,9 +2018,18 @@ int lbm_closure_set(lbm_channel *h, const float *params, double ystar)
         return fail(LBM_ERR_ARG, "lbm_closure_set: the closure runs in FP32 arithmetic (FP32/FP16 storage)");
     if (h->part)
         return fail(LBM_ERR_STATE, "lbm_closure_set: not combined with particles");
+    for (int j = 0; j < MLP_H; j++)
+        if (!(fabsf(params[MLP_B1 + j]) <= MLP_BIAS_MAX && fabsf(params[MLP_B2 + j]) <= MLP_BIAS_MAX))
+            return fail(LBM_ERR_ARG, "lbm_closure_set: hidden bias %d beyond +-%g (or not finite)", j,
+                        (double)MLP_BIAS_MAX);
+    // the parameters, then the constants and the FP16 weight image of the
+    // fused evaluation (mlp.h, mlp_derive)
+    std::vector<float> dev(MLP_DEV_TOTAL);
+    std::copy(params, params + MLP_TOTAL, dev.begin());
+    mlp_derive(params, dev.data());
     if (!h->mlp)
-        HIP_TRY(hipMalloc((void **)&h->mlp, MLP_TOTAL * sizeof(float)));
-    HIP_TRY(hipMemcpyAsync(h->mlp, params, MLP_TOTAL * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMalloc((void **)&h->mlp, MLP_DEV_TOTAL * sizeof(float)));
+    HIP_TRY(hipMemcpyAsync(h->mlp, dev.data(), MLP_DEV_TOTAL * sizeof(float), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->mlp_inv_ystar = (float)(1.0 / ystar);
     h->mlp_yscale = params[MLP_SCALE + 9];

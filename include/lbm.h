@@ -360,7 +360,8 @@ int lbm_particles_solid(lbm_channel *h, int *solid);
 
 /* Evaluates the closure inside every following collision_MRT (FP32 and FP16
  * storage only; FP64 stays the reference's plain MRT).  ystar = nu/u* of the
- * run (para.f90:66). */
+ * run (para.f90:66).  Hidden-layer biases must lie within +-32 (LBM_ERR_ARG
+ * otherwise): tanh saturates long before that. */
 int lbm_closure_set(lbm_channel *h, const float *params, double ystar);
 /* Back to plain MRT. */
 int lbm_closure_clear(lbm_channel *h);

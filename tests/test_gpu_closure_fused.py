@@ -110,6 +110,41 @@ def test_matrix_core_network_matches_float64_network():
     assert np.max(np.abs(nut - want)) < 5e-3 * np.max(np.abs(want)), np.max(np.abs(nut - want))
 
 
+def test_large_hidden_biases_saturate_as_the_float64_network():
+    """Hidden biases near the +-32 bound: the fused form carries each bias as
+    the factor 2^(c b) on 2^(c a) and relies on IEEE overflow (a product of
+    inf) or a vanishing product for the saturated units; the float64 network
+    gives +-1 there, and the units with small biases still move the output."""
+    lx, ly, lz = 12, 10, 7
+    o, f0 = _state(lx, ly, lz, 44)
+    neq = _oracle_neq(o, f0)
+    scales = np.abs(neq).reshape(-1, 9).std(axis=0) + 1e-30
+    p = _params(scales, o.p.visc)
+    rng = np.random.default_rng(6)
+    p[M.B1:M.W2] = rng.choice([-31.9, -20.0, 0.2, 20.0, 31.9], M.NH)
+    p[M.B2:M.W3] = rng.choice([-31.9, 0.1, 31.9], M.NH)
+    p[M.B2:M.B2 + 4] = 0.0  # four units of layer 2 stay unsaturated
+    p[M.W1:M.B1] *= 4.0     # larger pre-activations for the small-bias units
+    with _channel(lx, ly, lz, f0, p) as ch:
+        feat = ch.closure_inputs().astype(np.float64).reshape(-1, 10)
+        nut = ch.closure_nut().astype(np.float64).reshape(-1)
+    want = M.softplus(M.forward(p, feat))
+    assert np.all(np.isfinite(nut))
+    assert np.std(want) > 0.01
+    assert np.max(np.abs(nut - want)) < 5e-3 * np.max(np.abs(want)), np.max(np.abs(nut - want))
+
+
+def test_closure_set_rejects_biases_beyond_the_bound():
+    with d3q19.Channel(8, 6, 5, "fp32") as ch:
+        p = d3q19.closure_params(seed=3, nu=ch.p["visc"])
+        ch.closure_set(p)
+        for where, bad in ((M.B1 + 3, 32.5), (M.B2 + 7, -33.0), (M.B2, np.nan)):
+            q = p.copy()
+            q[where] = bad
+            with pytest.raises(d3q19.LbmError, match="bias"):
+                ch.closure_set(q)
+
+
 def test_collision_applies_the_reported_eddy_viscosity():
     lx, ly, lz = 8, 6, 5
     o, f0 = _state(lx, ly, lz, 43)
