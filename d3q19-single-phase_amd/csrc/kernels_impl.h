@@ -477,7 +477,7 @@ __global__ void __launch_bounds__(BLOCK) k_closure_eval(StepArgs a, uint64_t n_b
     } else {
         const float o = mlp_forward_wave(a.mlp, wl, in);
         if (active)
-            out[i0] = softplus(o);
+            out[i0] = mlp_nut(a.mlp, o);
     }
 }
 

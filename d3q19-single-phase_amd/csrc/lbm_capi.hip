@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -2018,6 +2019,9 @@ int lbm_closure_set(lbm_channel *h, const float *params, double ystar)
         return fail(LBM_ERR_ARG, "lbm_closure_set: the closure runs in FP32 arithmetic (FP32/FP16 storage)");
     if (h->part)
         return fail(LBM_ERR_STATE, "lbm_closure_set: not combined with particles");
+    if (!(params[MLP_NUTMIN] > -1.0f && params[MLP_NUTMIN] <= 0.0f))
+        return fail(LBM_ERR_ARG, "lbm_closure_set: the nu_t/nu floor %g must lie in (-1, 0]",
+                    (double)params[MLP_NUTMIN]);
     for (int j = 0; j < MLP_H; j++)
         if (!(fabsf(params[MLP_B1 + j]) <= MLP_BIAS_MAX && fabsf(params[MLP_B2 + j]) <= MLP_BIAS_MAX))
             return fail(LBM_ERR_ARG, "lbm_closure_set: hidden bias %d beyond +-%g (or not finite)", j,
@@ -2239,6 +2243,30 @@ int lbm_trainer_set_target(lbm_trainer *t, int mode)
         return fail(LBM_ERR_ARG, "lbm_trainer_set_target: mode %d is neither LBM_TARGET_LOCAL nor LBM_TARGET_PLANE",
                     mode);
     t->target_mode = mode;
+    return LBM_OK;
+}
+
+int lbm_trainer_set_plane_targets(lbm_trainer *t, long long first, long long count, int nx, const float *values)
+{
+    CHECK_T(t);
+    if (!values || nx < 1 || first < 0 || count < 0 || first + count > t->count)
+        return fail(LBM_ERR_ARG, "lbm_trainer_set_plane_targets: need values, nx >= 1 and samples within the %lld held",
+                    t->count);
+    for (int i = 0; i < nx; i++)
+        if (!std::isfinite(values[i]))
+            return fail(LBM_ERR_ARG, "lbm_trainer_set_plane_targets: value %d is not finite", i);
+    if (count == 0)
+        return LBM_OK;
+    float *dv = nullptr;
+    HIP_TRY(hipMalloc((void **)&dv, (size_t)nx * sizeof(float)));
+    hipError_t e = hipMemcpyAsync(dv, values, (size_t)nx * sizeof(float), hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess)
+        e = mlp_fill_plane_targets(t->target, (uint64_t)first, (uint64_t)count, (uint32_t)nx, dv, t->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(t->stream);
+    (void)hipFree(dv);
+    if (e != hipSuccess)
+        return fail(LBM_ERR_HIP, "lbm_trainer_set_plane_targets: %s", hipGetErrorString(e));
     return LBM_OK;
 }
 

@@ -7,12 +7,14 @@
 // p_xy, p_yz, p_xz: evlm1,2,6,7,8,9,10,11,12 minus eqm of
 // Channel-Flow/collision.f90:86-138), each divided by a stored scale, and the
 // wall distance y+ / 100.  Output o gives an eddy viscosity nu_t = nu *
-// softplus(o) >= 0; the node's shear rates become
-// s9 = s13 = 1 / (3 (nu + nu_t) + 1/2)  (tau = 3 nu + 0.5, para.f90:106-108).
+// (softplus(o) + m) with the floor m > -1 (0: nu_t >= 0; negative: the
+// closure may also lower the viscosity, to nu (1 + m) at least); the node's
+// shear rates become s9 = s13 = 1 / (3 (nu + nu_t) + 1/2)  (tau = 3 nu + 0.5,
+// para.f90:106-108), so tau stays above 1/2.
 //
 // Parameters, FP32, in this order (NPARAM = 1441): W1[32][10], b1[32],
-// W2[32][32], b2[32], W3[32], b3; then (not trained) the 10 input scales and
-// nu.
+// W2[32][32], b2[32], W3[32], b3; then (not trained) the 10 input scales, nu
+// and the floor m.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,7 +33,8 @@ constexpr int MLP_B3 = MLP_W3 + MLP_H;
 constexpr int MLP_NPARAM = MLP_B3 + 1; // 1441
 constexpr int MLP_SCALE = MLP_NPARAM;  // 10 input scales
 constexpr int MLP_NU = MLP_SCALE + MLP_IN;
-constexpr int MLP_TOTAL = MLP_NU + 1;
+constexpr int MLP_NUTMIN = MLP_NU + 1;
+constexpr int MLP_TOTAL = MLP_NUTMIN + 1;
 
 // tanh(x) = 1 - 2 / (exp(2x) + 1) with the hardware exp2 and reciprocal; the
 // argument is clamped so exp cannot overflow.
@@ -45,6 +48,12 @@ __device__ __forceinline__ float fast_tanh(float x)
 __device__ __forceinline__ float softplus(float o)
 {
     return o > 20.0f ? o : log1pf(__expf(o));
+}
+
+// nu_t / nu from the network output.
+__device__ __forceinline__ float mlp_nut(const float *__restrict__ w, float o)
+{
+    return softplus(o) + w[MLP_NUTMIN];
 }
 
 // Network output o for one node's inputs x (already scaled), FP32 on the
@@ -76,7 +85,7 @@ __device__ __forceinline__ float mlp_forward(const float *__restrict__ w, const 
 __device__ __forceinline__ float mlp_rate(const float *__restrict__ w, float o)
 {
     const float nu = w[MLP_NU];
-    return 1.0f / (3.0f * nu * (1.0f + softplus(o)) + 0.5f);
+    return 1.0f / (3.0f * nu * (1.0f + mlp_nut(w, o)) + 0.5f);
 }
 
 // ---------------------------------------------------------------------------

@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(WAVE) k_mlp_grad(const float *__restrict__ w, 
         h2[j] = fast_tanh(a);
         o = fmaf(w[MLP_W3 + j], h2[j], o);
     }
-    const float pred = softplus(o);
+    const float pred = mlp_nut(w, o);
     const float y = valid ? target[first + sidx] : 0.0f;
     const float err = pred - y;
     const float sig = 1.0f / (1.0f + __expf(-o));
@@ -247,6 +247,15 @@ __global__ void k_mlp_sum_adam(const float *__restrict__ partial, uint32_t nwave
     }
 }
 
+// target[first + i] = values[i mod nx] for i < count.
+__global__ void k_fill_plane_targets(float *__restrict__ target, uint64_t first, uint64_t count, uint32_t nx,
+                                     const float *__restrict__ values)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        target[first + i] = values[i % nx];
+}
+
 // Network outputs for samples [first, first+count) (evaluation).
 __global__ void k_mlp_predict(const float *__restrict__ w, const float *__restrict__ feat, uint32_t first,
                               uint32_t count, float *__restrict__ out)
@@ -257,7 +266,7 @@ __global__ void k_mlp_predict(const float *__restrict__ w, const float *__restri
     float x[MLP_IN];
     for (int k = 0; k < MLP_IN; k++)
         x[k] = feat[(uint64_t)(first + s) * MLP_IN + k];
-    out[s] = softplus(mlp_forward(w, x));
+    out[s] = mlp_nut(w, mlp_forward(w, x));
 }
 
 } // namespace
@@ -324,6 +333,14 @@ hipError_t mlp_grad(const float *w, const float *feat, const float *target, uint
     hipLaunchKernelGGL(k_mlp_grad, dim3(nw), dim3(WAVE), 0, s, w, feat, target, first, count, partial);
     hipLaunchKernelGGL(k_mlp_sum_adam, dim3((MLP_NPARAM + 1 + 255) / 256), dim3(256), 0, s, partial, nw, grad, wmut,
                        m, v, step, lr, adam);
+    return hipGetLastError();
+}
+
+hipError_t mlp_fill_plane_targets(float *target, uint64_t first, uint64_t count, uint32_t nx, const float *values,
+                                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_plane_targets, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, s, target, first,
+                       count, nx, values);
     return hipGetLastError();
 }
 

@@ -45,6 +45,8 @@ hipError_t mlp_features(const FeatArgs &a, hipStream_t s);
 hipError_t mlp_grad(const float *w, const float *feat, const float *target, uint32_t first, uint32_t count,
                     float *partial, float *grad, float *wmut, float *m, float *v, int step, float lr, int adam,
                     hipStream_t s);
+hipError_t mlp_fill_plane_targets(float *target, uint64_t first, uint64_t count, uint32_t nx, const float *values,
+                                  hipStream_t s);
 hipError_t mlp_predict(const float *w, const float *feat, uint32_t first, uint32_t count, float *out, hipStream_t s);
 
 extern template struct MlpLaunch<0>;

@@ -109,6 +109,7 @@ _SIGS = {
     "lbm_trainer_add_filtered": [_H, _H, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_longlong)],
     "lbm_trainer_add_samples": [_H, _D, _D, ctypes.c_longlong],
+    "lbm_trainer_set_plane_targets": [_H, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int, _D],
     "lbm_trainer_samples": [_H, ctypes.POINTER(ctypes.c_longlong), _D, _D],
     "lbm_trainer_grad": [_H, ctypes.c_longlong, ctypes.c_longlong, _D, ctypes.POINTER(ctypes.c_float)],
     "lbm_trainer_train": [_H, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int, ctypes.c_longlong, ctypes.c_float,
@@ -117,7 +118,7 @@ _SIGS = {
     "lbm_trainer_params": [_H, _D],
 }
 
-MLP_NPARAM, MLP_TOTAL, MLP_IN, MLP_H = 1441, 1452, 10, 32
+MLP_NPARAM, MLP_TOTAL, MLP_IN, MLP_H = 1441, 1453, 10, 32
 
 _lib = None
 
@@ -166,10 +167,14 @@ def _fptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def closure_params(seed=0, scales=None, yscale=100.0, nu=0.0018, init="small"):
+def closure_params(seed=0, scales=None, yscale=100.0, nu=0.0018, init="small", nut_min=0.0):
     """Parameter vector of the learned closure (include/lbm.h: W1, b1, W2, b2, W3, b3,
-    9 input scales, y+ divisor, nu).  init="small": Xavier-uniform weights from
-    SplitMix64 stream `seed`, output bias -4 (nu_t ~ 0.018 nu at start)."""
+    9 input scales, y+ divisor, nu, floor m of nu_t/nu = softplus(o) + m).
+    init="small": Xavier-uniform weights from SplitMix64 stream `seed`, output
+    bias -4 (nu_t ~ 0.018 nu at start) for m = 0, or softplus(b3) = -m
+    (nu_t ~ 0 at start) for a negative floor."""
+    if not -1.0 < nut_min <= 0.0:
+        raise ValueError(f"nut_min {nut_min} outside (-1, 0]")
     from .workloads import uniform
     p = np.zeros(MLP_TOTAL, dtype=np.float32)
     r = uniform(1000 + seed, MLP_NPARAM)
@@ -186,10 +191,11 @@ def closure_params(seed=0, scales=None, yscale=100.0, nu=0.0018, init="small"):
         p[0:320] = take(320, 10, 32)
         p[352:352 + 1024] = take(1024, 32, 32)
         p[1408:1440] = take(32, 32, 1)
-        p[1440] = -4.0
+        p[1440] = -4.0 if nut_min == 0.0 else np.log(np.expm1(-nut_min))
     p[1441:1450] = 1.0 if scales is None else np.asarray(scales, dtype=np.float32)
     p[1450] = yscale
     p[1451] = nu
+    p[1452] = nut_min
     return p
 
 
@@ -574,6 +580,15 @@ class Trainer:
         f = np.ascontiguousarray(features, dtype=np.float32)
         t = np.ascontiguousarray(targets, dtype=np.float32)
         _call("lbm_trainer_add_samples", self._h, _fptr(f), _fptr(t), int(t.size))
+
+    def set_plane_targets(self, values, first=0, count=None):
+        """Targets of samples [first, first + count) (default: all from first)
+        become values[i mod len(values)]: with first at a snapshot start and
+        len(values) the coarse lx, each sample gets its plane's value."""
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        if count is None:
+            count = self.count() - first
+        _call("lbm_trainer_set_plane_targets", self._h, int(first), int(count), int(v.size), _fptr(v))
 
     def samples(self):
         n = ctypes.c_longlong()

@@ -39,6 +39,7 @@ import time
 import numpy as np
 
 from . import Channel, workloads
+from .params import para
 
 # statistc2 columns (saveload.f90:1492): 3 = <u_y>/u* (streamwise mean, U+),
 # 5, 6, 7 = u'u' variances (x, y, z), 9 = u_x'u_y' shear stress.
@@ -199,8 +200,98 @@ def _coarse_run(lx, ly, lz, nu_c, steps, stats_from, stats_every, perturb, seed,
         ch.sync()
         wall = time.perf_counter() - t0
         stats, n = ch.stats_read()
+        nut_plane = None
+        if closure is not None:  # the closure's plane-mean nu_t/nu on the last state
+            ch.macrovar()
+            nut_plane = ch.closure_nut().astype(np.float64).mean(axis=(0, 1))
         return {"stats": stats, "first_half": first_half, "samples": n, "wall_s": wall,
-                "mlups": lx * ly * lz * steps / wall / 1e6}
+                "mlups": lx * ly * lz * steps / wall / 1e6, "nut_plane": nut_plane}
+
+
+def _fine_reference(lx, ly, lz, spinup, window, stats_every, snapshots, seed, perturb, make_trainers, say):
+    """The fine run of the closure studies: plain MRT at full resolution (FP32
+    storage, rate set 3) from a start that becomes turbulent; statistics and
+    their 2x2x2 box-filtered counterparts averaged over the window after the
+    spin-up; `snapshots` filtered snapshots (from half the spin-up on) go to
+    the trainers make_trainers(input_scales, capacity) returns (a dict)."""
+    from . import Trainer, closure_params
+    total = spinup + window
+    # end of the first half of the statistics window (a multiple of stats_every)
+    half_fine = spinup + (window // 2) // stats_every * stats_every
+    snap_at = sorted({int(round(s / stats_every)) * stats_every
+                      for s in np.linspace(spinup // 2, total, snapshots)})
+    Nc = lx * ly * lz // 8
+    trainers = {}
+    out = {"half_at": half_fine, "Nc": Nc}
+    with Channel(lx, ly, lz, "fp32", mrttype=3) as fine:
+        _start(fine, perturb, seed)
+        nu_c = fine.p["visc"] / 2.0
+        s9_c = 1.0 / (3.0 * nu_c + 0.5)
+        ystar_c = fine.p["ystar"] / 2.0
+        out.update(nu_c=nu_c, s9_c=s9_c, ystar_c=ystar_c)
+        fine.stats_reset()
+        fine.sync()
+        t0 = time.perf_counter()
+        t_snap = 0.0
+        done = 0
+        while done < total:
+            fine.run(stats_every)
+            done += stats_every
+            if done > spinup:
+                fine.stats_accumulate()
+                fine.stats_accumulate_filtered()
+            if done == half_fine:
+                out["filtered_half"], _ = fine.stats_read_filtered()
+            if done in snap_at:
+                fine.sync()
+                ts = time.perf_counter()
+                if not trainers:
+                    # input scales: spread of each feature over the first snapshot
+                    with Trainer(Nc, closure_params(nu=nu_c)) as probe:
+                        probe.add_filtered(fine, nu_c, s9_c, ystar_c)
+                        f, _ = probe.samples()
+                    sc = f[:, :9].astype(np.float64).std(axis=0)
+                    sc[~(sc > 1e-12)] = 1.0
+                    del f
+                    trainers.update(make_trainers(sc, len(snap_at) * Nc))
+                    out["input_scales"] = [float(v) for v in sc]
+                for tr in trainers.values():
+                    tr.add_filtered(fine, nu_c, s9_c, ystar_c)
+                t_snap += time.perf_counter() - ts
+            if done % max(total // 10, 1) == 0:
+                st = fine.statistc2()
+                say(f"fine: step {done}/{total} U+c {st[lx // 2, 3]:.3f} "
+                    f"uu(y+~20) {st[int(20 * fine.p['ystar']), 6]:.3f}")
+        fine.sync()
+        out["wall_s"] = time.perf_counter() - t0 - t_snap
+        out["snapshot_s"] = t_snap
+        out["stats"], out["samples"] = fine.stats_read()
+        out["filtered"], _ = fine.stats_read_filtered()
+    return out, trainers
+
+
+def _fine_report(fr, lx, ly, lz, total):
+    fine_stats, fine_filt = fr["stats"], fr["filtered"]
+    return {"wall_s": round(fr["wall_s"], 2), "samples": fr["samples"], "snapshot_s": round(fr["snapshot_s"], 2),
+            "mlups": round(lx * ly * lz * total / fr["wall_s"] / 1e6, 1),
+            # the two walls of the filtered fine statistics against each other
+            "wall_to_wall_asymmetry": _asymmetry(fine_filt),
+            # the first half of the window against the whole: its sampling drift
+            "first_half_vs_window": _stat_errors(fr["filtered_half"], fine_filt),
+            # filtered against plane-pair means of the unfiltered statistics:
+            # zero for U+ (the filter is linear), the subfilter share for the rest
+            "filter_effect": _stat_errors(fine_filt, 0.5 * (fine_stats[0::2] + fine_stats[1::2]))}
+
+
+def _coarse_report(r, fr, wall_fine):
+    fine_stats = fr["stats"]
+    return {"wall_s": round(r["wall_s"], 2), "samples": r["samples"], "mlups": round(r["mlups"], 1),
+            "errors_vs_filtered_fine": _stat_errors(r["stats"], fr["filtered"]),
+            # the same comparison for the first half of both windows
+            "first_half_errors_vs_filtered_fine": _stat_errors(r["first_half"], fr["filtered_half"]),
+            "errors_vs_fine": _stat_errors(r["stats"], 0.5 * (fine_stats[0::2] + fine_stats[1::2])),
+            "wall_to_wall_asymmetry": _asymmetry(r["stats"]),
+            "time_to_stats_speedup": round(wall_fine / r["wall_s"], 2)}
 
 
 def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_every=100, snapshots=8,
@@ -222,71 +313,27 @@ def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_ev
     from . import Trainer, closure_params
     say = log or (lambda m: None)
     total = spinup + window
-    # end of the first half of the statistics window (a multiple of stats_every)
-    half_fine = spinup + (window // 2) // stats_every * stats_every
-    snap_at = sorted({int(round(s / stats_every)) * stats_every
-                      for s in np.linspace(spinup // 2, total, snapshots)})
-    Nc = lx * ly * lz // 8
     modes = ("local", "plane")
-    trainers = {}
     report = {"grid": [lx, ly, lz], "coarse_grid": [lx // 2, ly // 2, lz // 2], "spinup": spinup, "window": window,
               "stats_every": stats_every, "perturbation": {k: list(v) if isinstance(v, tuple) else v
                                                           for k, v in perturb.items()}}
     trained = {}
+    trainers = {}
+
+    def make(sc, capacity):
+        made = {}
+        for mode in modes:
+            made[mode] = Trainer(capacity, closure_params(seed=1, scales=sc, nu=nu_of[0]))
+            made[mode].set_target(mode)
+        return made
+
+    # the coarse run halves the fine viscosity (dx and dt double)
+    nu_of = [para(lx, mrttype=3)["visc"] / 2.0]
     try:
-        with Channel(lx, ly, lz, "fp32", mrttype=3) as fine:
-            _start(fine, perturb, seed)
-            nu_c = fine.p["visc"] / 2.0
-            s9_c = 1.0 / (3.0 * nu_c + 0.5)
-            ystar_c = fine.p["ystar"] / 2.0
-            fine.stats_reset()
-            fine.sync()
-            t0 = time.perf_counter()
-            t_snap = 0.0
-            done = 0
-            while done < total:
-                fine.run(stats_every)
-                done += stats_every
-                if done > spinup:
-                    fine.stats_accumulate()
-                    fine.stats_accumulate_filtered()
-                if done == half_fine:
-                    fine_half, _ = fine.stats_read_filtered()
-                if done in snap_at:
-                    fine.sync()
-                    ts = time.perf_counter()
-                    if not trainers:
-                        # input scales: spread of each feature over the first snapshot
-                        with Trainer(Nc, closure_params(nu=nu_c)) as probe:
-                            probe.add_filtered(fine, nu_c, s9_c, ystar_c)
-                            f, _ = probe.samples()
-                        sc = f[:, :9].astype(np.float64).std(axis=0)
-                        sc[~(sc > 1e-12)] = 1.0
-                        del f
-                        for mode in modes:
-                            trainers[mode] = Trainer(len(snap_at) * Nc, closure_params(seed=1, scales=sc, nu=nu_c))
-                            trainers[mode].set_target(mode)
-                        report["input_scales"] = [float(v) for v in sc]
-                    for tr in trainers.values():
-                        tr.add_filtered(fine, nu_c, s9_c, ystar_c)
-                    t_snap += time.perf_counter() - ts
-                if done % max(total // 10, 1) == 0:
-                    s = fine.statistc2()
-                    say(f"fine: step {done}/{total} U+c {s[lx // 2, 3]:.3f} "
-                        f"uu(y+~20) {s[int(20 * fine.p['ystar']), 6]:.3f}")
-            fine.sync()
-            wall_fine = time.perf_counter() - t0 - t_snap
-            fine_stats, n_fine = fine.stats_read()
-            fine_filt, _ = fine.stats_read_filtered()
-        report["fine"] = {"wall_s": round(wall_fine, 2), "samples": n_fine, "snapshot_s": round(t_snap, 2),
-                          "mlups": round(lx * ly * lz * total / wall_fine / 1e6, 1),
-                          # the two walls of the filtered fine statistics against each other
-                          "wall_to_wall_asymmetry": _asymmetry(fine_filt),
-                          # the first half of the window against the whole: its sampling drift
-                          "first_half_vs_window": _stat_errors(fine_half, fine_filt),
-                          # filtered against plane-pair means of the unfiltered statistics:
-                          # zero for U+ (the filter is linear), the subfilter share for the rest
-                          "filter_effect": _stat_errors(fine_filt, 0.5 * (fine_stats[0::2] + fine_stats[1::2]))}
+        fr, trainers = _fine_reference(lx, ly, lz, spinup, window, stats_every, snapshots, seed, perturb, make, say)
+        report["input_scales"] = fr["input_scales"]
+        report["fine"] = _fine_report(fr, lx, ly, lz, total)
+        Nc = fr["Nc"]
         # mean nu_t/nu per coarse plane (samples are ordered x fastest), both walls folded
         plane = lambda v: v.reshape(lz // 2, ly // 2, lx // 2).mean(axis=(0, 1))  # noqa: E731
         half = lambda v: 0.5 * (v[:lx // 4] + v[::-1][:lx // 4])  # noqa: E731
@@ -316,27 +363,103 @@ def closure_study(lx=192, ly=768, lz=384, spinup=100000, window=100000, stats_ev
     finally:
         for tr in trainers.values():
             tr.close()
-    ref = fine_filt
     runs = [("coarse_plain", None)] + [("coarse_closure_" + m, trained[m]) for m in modes]
     for name, closure in runs:
-        r = _coarse_run(lx // 2, ly // 2, lz // 2, nu_c, total // 2, spinup // 2, stats_every // 2, perturb, seed,
-                        closure=closure, log=say, half_at=half_fine // 2)
-        report[name] = {"wall_s": round(r["wall_s"], 2), "samples": r["samples"], "mlups": round(r["mlups"], 1),
-                        "errors_vs_filtered_fine": _stat_errors(r["stats"], ref),
-                        # the same comparison for the first half of both windows
-                        "first_half_errors_vs_filtered_fine": _stat_errors(r["first_half"], fine_half),
-                        "errors_vs_fine": _stat_errors(r["stats"], 0.5 * (fine_stats[0::2] + fine_stats[1::2])),
-                        "wall_to_wall_asymmetry": _asymmetry(r["stats"]),
-                        "time_to_stats_speedup": round(wall_fine / r["wall_s"], 2)}
+        r = _coarse_run(lx // 2, ly // 2, lz // 2, fr["nu_c"], total // 2, spinup // 2, stats_every // 2, perturb,
+                        seed, closure=closure, log=say, half_at=fr["half_at"] // 2)
+        report[name] = _coarse_report(r, fr, fr["wall_s"])
         report[name + "_profile"] = r["stats"]
-    report["fine_filtered_profile"] = ref
-    report["fine_profile"] = fine_stats
+    report["fine_filtered_profile"] = fr["filtered"]
+    report["fine_profile"] = fr["stats"]
     return report, trained
+
+
+def closure_balance_study(lx=192, ly=768, lz=384, spinup=100000, window=300000, stats_every=100, snapshots=8,
+                          iterations=4000, batch=65536, lr=2e-3, seed=2, perturb=TURBULENT_START, rounds=2,
+                          nut_min=-0.5, core_frac=0.05, log=None):
+    """Learned closure trained on momentum-balance targets (d3q19/calibrate.py).
+
+    1. The fine run of closure_study (filtered statistics, filtered snapshots
+       whose features are the training inputs).
+    2. The plain coarse run; its mean profile against the filtered fine one
+       gives the per-plane target g_0 = U_plain'/U_ref' - 1 (the closure the
+       mean momentum balance asks for at that run's Reynolds stress).
+    3. `rounds` times: train on the current per-plane target (all snapshots,
+       lbm_trainer_set_plane_targets), run the coarse channel with the fused
+       closure (floor nut_min: the closure may lower the viscosity), update
+       the target from that run (g_{k+1} = (1 + g_k) U_k'/U_ref' - 1, g_k the
+       run's own plane-mean nu_t/nu).
+    4. The last network and plain MRT, each on a second coarse realisation
+       (another seed of the start perturbation): the held-out a-posteriori
+       comparison with the filtered fine statistics.
+    """
+    from . import Trainer, closure_params
+    from . import calibrate
+    say = log or (lambda m: None)
+    total = spinup + window
+    lxc, lyc, lzc = lx // 2, ly // 2, lz // 2
+    report = {"grid": [lx, ly, lz], "coarse_grid": [lxc, lyc, lzc], "spinup": spinup, "window": window,
+              "stats_every": stats_every, "rounds": rounds, "nut_min": nut_min, "core_frac": core_frac,
+              "seed": seed, "held_out_seed": seed + 1,
+              "perturbation": {k: list(v) if isinstance(v, tuple) else v for k, v in perturb.items()}}
+    nu_c = para(lx, mrttype=3)["visc"] / 2.0
+    trainers = {}
+    make = lambda sc, cap: {"balance": Trainer(cap, closure_params(seed=1, scales=sc, nu=nu_c,  # noqa: E731
+                                                                   nut_min=nut_min))}
+    arrays = {}
+    coarse = lambda s, closure: _coarse_run(lxc, lyc, lzc, nu_c, total // 2, spinup // 2,  # noqa: E731
+                                            stats_every // 2, perturb, s, closure=closure, log=say,
+                                            half_at=fr["half_at"] // 2)
+    try:
+        fr, trainers = _fine_reference(lx, ly, lz, spinup, window, stats_every, snapshots, seed, perturb, make, say)
+        tr = trainers["balance"]
+        report["input_scales"] = fr["input_scales"]
+        report["fine"] = _fine_report(fr, lx, ly, lz, total)
+        U_ref = fr["filtered"][:, COLS["U+"]]
+        arrays["fine_filtered_profile"] = fr["filtered"]
+        arrays["fine_profile"] = fr["stats"]
+        r = coarse(seed, None)
+        report["coarse_plain"] = _coarse_report(r, fr, fr["wall_s"])
+        arrays["coarse_plain_profile"] = r["stats"]
+        say(f"plain: {report['coarse_plain']['errors_vs_filtered_fine']}")
+        g = calibrate.balance_update(r["stats"][:, COLS["U+"]], U_ref, 0.0, core_frac=core_frac)
+        params = None
+        for k in range(rounds):
+            tr.set_plane_targets(g.astype(np.float32))
+            t1 = time.perf_counter()
+            loss = tr.train(0, tr.count(), iterations=iterations, batch=min(batch, tr.count()), lr=lr)
+            t_train = time.perf_counter() - t1
+            pred = tr.predict(tr.count() - fr["Nc"], fr["Nc"]).astype(np.float64)
+            pred_plane = pred.reshape(lzc, lyc, lxc).mean(axis=(0, 1))
+            params = tr.params()
+            r = coarse(seed, params)
+            name = f"coarse_closure_round{k}"
+            report[name] = _coarse_report(r, fr, fr["wall_s"])
+            report[name]["target"] = [round(float(v), 4) for v in g]
+            report[name]["trained_plane_mean"] = [round(float(v), 4) for v in pred_plane]
+            report[name]["applied_plane_mean"] = [round(float(v), 4) for v in r["nut_plane"]]
+            report[name]["training"] = {"last_batch_loss": float(loss), "wall_s": round(t_train, 2)}
+            arrays[name + "_profile"] = r["stats"]
+            arrays["params_round%d" % k] = params
+            say(f"round {k}: {report[name]['errors_vs_filtered_fine']}")
+            g = calibrate.balance_update(r["stats"][:, COLS["U+"]], U_ref, r["nut_plane"], core_frac=core_frac)
+        report["next_target"] = [round(float(v), 4) for v in g]
+    finally:
+        for t in trainers.values():
+            t.close()
+    for name, closure in (("held_out_plain", None), ("held_out_closure", params)):
+        r = coarse(seed + 1, closure)
+        report[name] = _coarse_report(r, fr, fr["wall_s"])
+        arrays[name + "_profile"] = r["stats"]
+        say(f"{name}: {report[name]['errors_vs_filtered_fine']}")
+    return report, arrays
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("study", choices=["precision", "closure"])
+    ap.add_argument("study", choices=["precision", "closure", "closure-balance"])
+    ap.add_argument("--rounds", type=int, default=2, help="closure-balance: target updates from coarse runs")
+    ap.add_argument("--nut-min", type=float, default=-0.5, help="closure-balance: floor of nu_t/nu")
     ap.add_argument("--spinup", type=int, default=100000, help="closure: fine steps before the statistics window")
     ap.add_argument("--window", type=int, default=100000, help="closure: fine steps of the statistics window")
     ap.add_argument("--snapshots", type=int, default=8, help="closure: filtered fine snapshots for training")
@@ -356,6 +479,16 @@ def main(argv=None):
     args = ap.parse_args(argv)
     lx, ly, lz = (int(v) for v in args.grid.split(","))
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.study == "closure-balance":
+        report, arrays = closure_balance_study(lx, ly, lz, args.spinup, args.window, args.stats_every,
+                                               args.snapshots, args.iterations, args.batch, args.lr,
+                                               rounds=args.rounds, nut_min=args.nut_min, log=log)
+        print(json.dumps(report, indent=1))
+        if args.out:
+            with open(args.out + ".json", "w") as fh:
+                json.dump(report, fh, indent=1)
+            np.savez(args.out + ".npz", **arrays)
+        return
     if args.study == "closure":
         report, trained = closure_study(lx, ly, lz, args.spinup, args.window, args.stats_every, args.snapshots,
                                         args.iterations, args.batch, args.lr, log=log)

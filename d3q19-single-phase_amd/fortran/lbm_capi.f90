@@ -32,7 +32,7 @@
         integer(c_int) :: pressure_force, nave
       end type lbm_particle_config
 
-      integer(c_int), parameter :: LBM_MLP_TOTAL = 1452
+      integer(c_int), parameter :: LBM_MLP_TOTAL = 1453
 
       interface
         function lbm_last_error() bind(C, name="lbm_last_error")

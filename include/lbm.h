@@ -350,18 +350,20 @@ int lbm_particles_solid(lbm_channel *h, int *solid);
  * target of a 2x-coarsened channel that reproduces filtered fine statistics).
  * A 10-32-32-1 tanh network maps the nine non-equilibrium moments the MRT
  * relaxes with s1, s2, s9, s10, s13 (scaled) and y+ to an eddy viscosity
- * nu_t = nu * softplus(o); the node's shear rates become
+ * nu_t = nu * (softplus(o) + m); the node's shear rates become
  * s9 = s13 = 1 / (3 (nu + nu_t) + 0.5).  Parameters, FP32, LBM_MLP_TOTAL
  * values: W1[32][10], b1[32], W2[32][32], b2[32], W3[32], b3 (the
- * LBM_MLP_NPARAM trained values), then 9 input scales, the y+ divisor, nu.
+ * LBM_MLP_NPARAM trained values), then 9 input scales, the y+ divisor, nu,
+ * and the floor m of nu_t/nu, in (-1, 0] (0: a non-negative eddy viscosity;
+ * negative: the closure may also lower the viscosity, tau staying above 1/2).
  */
 #define LBM_MLP_NPARAM 1441
-#define LBM_MLP_TOTAL 1452
+#define LBM_MLP_TOTAL 1453
 
 /* Evaluates the closure inside every following collision_MRT (FP32 and FP16
  * storage only; FP64 stays the reference's plain MRT).  ystar = nu/u* of the
- * run (para.f90:66).  Hidden-layer biases must lie within +-32 (LBM_ERR_ARG
- * otherwise): tanh saturates long before that. */
+ * run (para.f90:66).  Hidden-layer biases must lie within +-32 and the floor
+ * within (-1, 0] (LBM_ERR_ARG otherwise). */
 int lbm_closure_set(lbm_channel *h, const float *params, double ystar);
 /* Back to plain MRT. */
 int lbm_closure_clear(lbm_channel *h);
@@ -369,7 +371,7 @@ int lbm_closure_clear(lbm_channel *h);
 /* What the closure sees and applies at every node of the current lattice, for
  * the next collision_MRT (the fields must come from macrovar): the ten scaled
  * network inputs (feat[10*n + k], n = ix + lx*(iy + ly*iz)), or the eddy
- * viscosity ratio nu_t/nu = softplus(o) (nut[n]) from the same matrix-core
+ * viscosity ratio nu_t/nu = softplus(o) + m (nut[n]) from the same matrix-core
  * evaluation the fused collision uses. */
 int lbm_closure_inputs(lbm_channel *h, float *feat);
 int lbm_closure_nut(lbm_channel *h, float *nut);
@@ -380,8 +382,14 @@ int lbm_closure_nut(lbm_channel *h, float *nut);
  * (target: the eddy viscosity nu_t = -tau:S/(2 S:S) of the subgrid stress,
  * features: the first-order non-equilibrium moments of the coarse MRT with
  * shear rate s9_coarse, scaled by the parameters' input scales).  Gradients
- * are of mean((softplus(o) - target)^2) over a sample range; training uses
- * Adam (beta 0.9, 0.999) over consecutive batches.
+ * are of mean((softplus(o) + m - target)^2) over a sample range; training
+ * uses Adam (beta 0.9, 0.999) over consecutive batches.
+ *
+ * lbm_trainer_set_plane_targets overwrites the targets of samples
+ * [first, first + count): sample first + i gets values[i mod nx].  With first
+ * at the start of an add_filtered snapshot (its samples run x fastest) and nx
+ * the coarse lx, every sample gets the value of its wall-normal plane: a
+ * target fixed by the mean momentum balance of the coarse run, for example.
  *
  * lbm_trainer_set_target chooses the target of later add_filtered calls:
  * LBM_TARGET_LOCAL (default) the node's own -tau:S/(2 S:S)/nu, clipped to
@@ -396,6 +404,7 @@ int lbm_trainer_destroy(lbm_trainer *t);
 int lbm_trainer_set_target(lbm_trainer *t, int mode);
 int lbm_trainer_add_filtered(lbm_trainer *t, lbm_channel *fine, double nu_coarse, double s9_coarse,
                              double ystar_coarse, long long *added);
+int lbm_trainer_set_plane_targets(lbm_trainer *t, long long first, long long count, int nx, const float *values);
 int lbm_trainer_add_samples(lbm_trainer *t, const float *features, const float *targets, long long n);
 int lbm_trainer_samples(lbm_trainer *t, long long *n, float *features, float *targets);
 int lbm_trainer_grad(lbm_trainer *t, long long first, long long count, float *grad, float *loss);

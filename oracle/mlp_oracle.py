@@ -2,7 +2,8 @@
 the GPU results against it).
 
 Float64 numpy restatements of:
-  * the 10-32-32-1 tanh network and its loss mean((softplus(o) - y)^2),
+  * the 10-32-32-1 tanh network, nu_t/nu = softplus(o) + m (m: the floor
+    stored after nu) and the loss mean((softplus(o) + m - y)^2),
     with the analytic gradient (backpropagation), pinned against central
     finite differences in tests/test_mlp_oracle.py;
   * the 2x2x2 box filter of a fine channel (means of rho, u, u_a u_b from the
@@ -18,7 +19,7 @@ NPARAM, NIN, NH = 1441, 10, 32
 W1, B1 = 0, 320
 W2, B2 = 352, 1376
 W3, B3 = 1408, 1440
-SCALE, YSCALE, NU = 1441, 1450, 1451
+SCALE, YSCALE, NU, NUTMIN = 1441, 1450, 1451, 1452
 
 
 def unpack(p):
@@ -30,6 +31,17 @@ def softplus(o):
     return np.where(o > 20, o, np.log1p(np.exp(np.minimum(o, 20))))
 
 
+def floor(p):
+    """The floor m of nu_t/nu (0 for a vector without one)."""
+    p = np.asarray(p)
+    return float(p[NUTMIN]) if p.size > NUTMIN else 0.0
+
+
+def nut(p, X):
+    """nu_t/nu of the network for inputs X."""
+    return softplus(forward(p, X)) + floor(p)
+
+
 def forward(p, X):
     w1, b1, w2, b2, w3, b3 = unpack(p)
     h1 = np.tanh(X @ w1.T + b1)
@@ -38,7 +50,7 @@ def forward(p, X):
 
 
 def loss_and_grad(p, X, y):
-    """Mean squared error of softplus(o) against y and its gradient (NPARAM)."""
+    """Mean squared error of softplus(o) + m against y and its gradient (NPARAM)."""
     X = np.asarray(X, dtype=np.float64)
     y = np.asarray(y, dtype=np.float64)
     w1, b1, w2, b2, w3, b3 = unpack(p)
@@ -48,7 +60,7 @@ def loss_and_grad(p, X, y):
     a2 = h1 @ w2.T + b2
     h2 = np.tanh(a2)
     o = h2 @ w3 + b3
-    pred = softplus(o)
+    pred = softplus(o) + floor(p)
     err = pred - y
     loss = np.mean(err * err)
     g_o = 2.0 * err / n / (1.0 + np.exp(-o))

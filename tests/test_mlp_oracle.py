@@ -3,14 +3,16 @@ against central finite differences, and the feature/target restatement
 against closed forms (a pure shear flow has no subgrid stress and one
 non-zero shear feature; a solid-body mean gives zero targets)."""
 import numpy as np
+import pytest
 
 import mlp_oracle as M
 from d3q19 import closure_params
 
 
-def test_gradient_matches_finite_differences():
+@pytest.mark.parametrize("floor", [0.0, -0.4])
+def test_gradient_matches_finite_differences(floor):
     rng = np.random.default_rng(0)
-    p = closure_params(seed=3).astype(np.float64)
+    p = closure_params(seed=3, nut_min=floor).astype(np.float64)
     p[M.B3] = 0.3
     X = rng.standard_normal((37, 10))
     y = np.abs(rng.standard_normal(37))
