@@ -47,11 +47,14 @@ COLS = {"U+": 3, "uxux": 5, "uyuy": 6, "uzuz": 7, "uxuy": 9}
 
 
 def run_channel(prec, lx, ly, lz, steps, stats_from, stats_every, seed=2, mrttype=3, amp=0.10,
-                jitter=0.0, rstar=None, visc=None, log=None):
+                jitter=0.0, rstar=None, visc=None, perturb=None, log=None):
     """One channel run; returns time-averaged profiles, instantaneous U+
-    profiles at every tenth of the run, mass drift and timing."""
+    profiles at every tenth of the run, mass drift and timing.  perturb: the
+    keyword arguments of workloads.loglaw_divfree for the start perturbation
+    (default: amplitude amp, the module's other defaults)."""
     with Channel(lx, ly, lz, prec, mrttype=mrttype, rstar=rstar, visc=visc) as ch:
-        ux, uy, uz = workloads.loglaw_divfree(lx, ly, lz, ch.p["ustar"], ch.p["ystar"], seed=seed, amp=amp)
+        pert = dict(amp=amp) if perturb is None else dict(perturb)
+        ux, uy, uz = workloads.loglaw_divfree(lx, ly, lz, ch.p["ustar"], ch.p["ystar"], seed=seed, **pert)
         if jitter:
             uc = float(np.max(uy))
             uz = uz + jitter * uc * (2.0 * workloads.uniform(99, uz.size).reshape(uz.shape) - 1.0)
@@ -75,7 +78,7 @@ def run_channel(prec, lx, ly, lz, steps, stats_from, stats_every, seed=2, mrttyp
             if done % tenth == 0:
                 inst.append((done, ch.statistc2()[:, COLS["U+"]].copy()))
                 if log:
-                    log(f"{prec}{' jitter' if jitter else ''} amp {amp}: step {done}/{steps} "
+                    log(f"{prec}{' jitter' if jitter else ''} amp {pert['amp']}: step {done}/{steps} "
                         f"mass/node {ch.mass() / (lx * ly * lz):.3e}")
         ch.sync()
         wall = time.perf_counter() - t0
@@ -99,9 +102,16 @@ def _inst_errors(run, ref):
 
 def precision_study(lx=192, ly=768, lz=384, steps=20000, stats_from=10000, stats_every=100,
                     precs=("fp64", "fp32", "fp16"), mrttype=3, jitter=1e-7, deterministic=True, rstar=None,
-                    visc=None, log=None):
+                    visc=None, perturb=None, log=None):
+    """Each precision on the same seeded start; errors of its time averages
+    against FP64.  With perturb=TURBULENT_START the flow is turbulent
+    (chaotic): trajectories of different precisions separate, and only the
+    statistics compare.  The FP64 run with a round-off-sized disturbance
+    (jitter) is then a second realisation of the same flow, and its distance
+    from FP64 is the sampling scatter that a precision's error is measured
+    against; each run's wall-to-wall asymmetry is reported beside it."""
     cases = {}
-    kw = dict(mrttype=mrttype, rstar=rstar, visc=visc, log=log)
+    kw = dict(mrttype=mrttype, rstar=rstar, visc=visc, perturb=perturb, log=log)
     for prec in precs:
         cases[prec] = run_channel(prec, lx, ly, lz, steps, stats_from, stats_every, **kw)
     if jitter:
@@ -109,7 +119,7 @@ def precision_study(lx=192, ly=768, lz=384, steps=20000, stats_from=10000, stats
     if deterministic:
         for prec in precs:
             cases["deterministic_" + prec] = run_channel(prec, lx, ly, lz, steps, stats_from, stats_every,
-                                                         amp=0.0, **kw)
+                                                         **dict(kw, perturb=dict(amp=0.0)))
     report = {"grid": [lx, ly, lz], "steps": steps, "stats_from": stats_from, "stats_every": stats_every,
               "rstar": 180.0 if rstar is None else rstar, "visc": 0.0036 if visc is None else visc,
               "mrttype": mrttype, "jitter": jitter, "variants": {}}
@@ -127,7 +137,11 @@ def precision_study(lx=192, ly=768, lz=384, steps=20000, stats_from=10000, stats
         if name.startswith("deterministic_"):
             # no fluctuations: only the mean profile is meaningful
             entry["errors_vs_fp64"] = None if r is ref else {"U+": entry["errors_vs_fp64"]["U+"]}
+        else:
+            entry["wall_to_wall_asymmetry"] = _asymmetry(r["profile"])
         report["variants"][name] = entry
+    if perturb is not None:
+        report["perturbation"] = {k: list(v) if isinstance(v, tuple) else v for k, v in perturb.items()}
     return report, cases
 
 
@@ -455,9 +469,91 @@ def closure_balance_study(lx=192, ly=768, lz=384, spinup=100000, window=300000, 
     return report, arrays
 
 
+def wall_profile(a, b, lxc, ystar_c, y_split=20.0, width=5.0):
+    """A two-level plane profile of nu_t/nu: a below y+ = y_split, b above,
+    joined by tanh over `width` wall units (coarse plane centres, both walls)."""
+    i = np.arange(lxc)
+    yplus = np.minimum(i + 0.5, lxc - i - 0.5) / ystar_c
+    w = 0.5 * (1.0 - np.tanh((yplus - y_split) / width))
+    return a * w + b * (1.0 - w)
+
+
+def plane_closure_params(g, nu_c, ystar_c, yscale=100.0, nut_min=-0.5, iterations=3000, lr=3e-3, seed=1):
+    """Network parameters whose nu_t/nu is the plane profile g (coarse planes,
+    x order) as a function of the wall-distance input alone: the nine
+    non-equilibrium inputs are scaled out (scales 1e30) and the device's own
+    kernels train on one sample per plane.  Returns the parameters and the
+    largest deviation of the trained output from g."""
+    from . import Trainer, closure_params
+    lxc = len(g)
+    i = np.arange(lxc)
+    X = np.zeros((lxc, 10), dtype=np.float32)
+    X[:, 9] = np.minimum(i + 0.5, lxc - i - 0.5) / ystar_c / yscale
+    reps = max(1, 4096 // lxc)
+    p = closure_params(seed=seed, scales=np.full(9, 1e30), yscale=yscale, nu=nu_c, nut_min=nut_min)
+    with Trainer(reps * lxc, p) as tr:
+        tr.add_samples(np.tile(X, (reps, 1)), np.tile(np.asarray(g, dtype=np.float32), reps))
+        tr.train(0, reps * lxc, iterations=iterations, batch=reps * lxc, lr=lr)
+        pred = tr.predict(0, lxc).astype(np.float64)
+        out = tr.params()
+    return out, float(np.max(np.abs(pred - np.asarray(g))))
+
+
+def closure_plane_study(reference, lx=192, ly=768, lz=384, spinup=100000, window=300000, stats_every=100,
+                        shapes=((0.0, -0.1),), seeds=(2,), plain=False, y_split=20.0, width=5.0, nut_min=-0.5,
+                        perturb=TURBULENT_START, log=None):
+    """Coarse runs with wall-distance closures against a fine reference.
+
+    reference: the .npz a closure study wrote (its fine_filtered_profile and
+    fine_profile).  Each shape (a, b) is the plane profile wall_profile(a, b)
+    of nu_t/nu, learned by the network from the wall-distance input
+    (plane_closure_params); each is run on the coarse channel for every seed
+    of the start perturbation, over the closure studies' window, and its
+    statistics compared with the filtered fine ones.  plain=True adds the
+    plain coarse run of every seed."""
+    say = log or (lambda m: None)
+    ref = np.load(reference)
+    fine_filt, fine_stats = ref["fine_filtered_profile"], ref["fine_profile"]
+    lxc, lyc, lzc = lx // 2, ly // 2, lz // 2
+    if fine_filt.shape[0] != lxc:
+        raise ValueError(f"reference has {fine_filt.shape[0]} planes, the coarse grid {lxc}")
+    pc = para(lxc, mrttype=3, visc=para(lx, mrttype=3)["visc"] / 2.0)
+    nu_c, ystar_c = pc["visc"], pc["ystar"]
+    total = spinup + window
+    report = {"reference": str(reference), "coarse_grid": [lxc, lyc, lzc], "spinup": spinup, "window": window,
+              "y_split": y_split, "width": width, "nut_min": nut_min, "runs": []}
+    cases = ([("plain", None, None)] if plain else []) + [(f"a={a:+.3f} b={b:+.3f}", a, b) for a, b in shapes]
+    profiles = {}
+    for name, a, b in cases:
+        params, fit = (None, None)
+        if a is not None:
+            g = wall_profile(a, b, lxc, ystar_c, y_split, width)
+            params, fit = plane_closure_params(g, nu_c, ystar_c, nut_min=nut_min)
+        for sd in seeds:
+            r = _coarse_run(lxc, lyc, lzc, nu_c, total // 2, spinup // 2, stats_every // 2, perturb, sd,
+                            closure=params, log=None)
+            entry = {"case": name, "seed": sd, "fit": fit, "wall_s": round(r["wall_s"], 2),
+                     "mlups": round(r["mlups"], 1),
+                     "errors_vs_filtered_fine": _stat_errors(r["stats"], fine_filt),
+                     "errors_vs_fine": _stat_errors(r["stats"], 0.5 * (fine_stats[0::2] + fine_stats[1::2])),
+                     "wall_to_wall_asymmetry": _asymmetry(r["stats"])}
+            if r["nut_plane"] is not None:
+                entry["applied_plane_mean"] = [round(float(v), 4) for v in r["nut_plane"][:lxc // 2]]
+            report["runs"].append(entry)
+            profiles[f"{name} seed={sd}"] = r["stats"]
+            say(f"{name} seed {sd}: " + " ".join(f"{k} {v * 100:.2f}%" for k, v in
+                                                    entry["errors_vs_filtered_fine"].items()))
+    return report, profiles
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("study", choices=["precision", "closure", "closure-balance"])
+    ap.add_argument("study", choices=["precision", "closure", "closure-balance", "closure-plane"])
+    ap.add_argument("--reference", default=None, help="closure-plane: .npz of a closure study (fine profiles)")
+    ap.add_argument("--shapes", default="0,-0.1",
+                    help="closure-plane: ';'-separated a,b pairs (nu_t/nu below / above y+ = 20)")
+    ap.add_argument("--seeds", default="2", help="closure-plane: comma list of start-perturbation seeds")
+    ap.add_argument("--plain", action="store_true", help="closure-plane: also the plain coarse run per seed")
     ap.add_argument("--rounds", type=int, default=2, help="closure-balance: target updates from coarse runs")
     ap.add_argument("--nut-min", type=float, default=-0.5, help="closure-balance: floor of nu_t/nu")
     ap.add_argument("--spinup", type=int, default=100000, help="closure: fine steps before the statistics window")
@@ -475,10 +571,25 @@ def main(argv=None):
     ap.add_argument("--visc", type=float, default=None, help="precision: nu (default 0.0036, para.f90:61)")
     ap.add_argument("--jitter", type=float, default=1e-7, help="control run's disturbance / U_c (0: no control)")
     ap.add_argument("--no-deterministic", action="store_true", help="skip the unperturbed runs")
+    ap.add_argument("--turbulent", action="store_true",
+                    help="precision: start from the turbulent-start perturbation (a chaotic flow)")
+    ap.add_argument("--precs", default="fp64,fp32,fp16", help="precision: comma list (fp64 first)")
     ap.add_argument("--out", default=None, help="also write the report (JSON) and profiles (.npz) here")
     args = ap.parse_args(argv)
     lx, ly, lz = (int(v) for v in args.grid.split(","))
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.study == "closure-plane":
+        shapes = [tuple(float(v) for v in sh.split(",")) for sh in args.shapes.split(";") if sh]
+        report, profiles = closure_plane_study(args.reference, lx, ly, lz, args.spinup, args.window,
+                                               args.stats_every, shapes=shapes,
+                                               seeds=[int(v) for v in args.seeds.split(",")], plain=args.plain,
+                                               nut_min=args.nut_min, log=log)
+        print(json.dumps(report, indent=1))
+        if args.out:
+            with open(args.out + ".json", "w") as fh:
+                json.dump(report, fh, indent=1)
+            np.savez(args.out + ".npz", **{k.replace(" ", "_"): v for k, v in profiles.items()})
+        return
     if args.study == "closure-balance":
         report, arrays = closure_balance_study(lx, ly, lz, args.spinup, args.window, args.stats_every,
                                                args.snapshots, args.iterations, args.batch, args.lr,
@@ -500,8 +611,9 @@ def main(argv=None):
             np.savez(args.out + ".npz", **{"params_" + m: v for m, v in trained.items()}, **arrays)
         return
     report, res = precision_study(lx, ly, lz, args.steps, args.stats_from, args.stats_every,
-                                  mrttype=args.mrttype, jitter=args.jitter, rstar=args.rstar, visc=args.visc,
-                                  deterministic=not args.no_deterministic, log=log)
+                                  precs=tuple(args.precs.split(",")), mrttype=args.mrttype, jitter=args.jitter,
+                                  rstar=args.rstar, visc=args.visc, deterministic=not args.no_deterministic,
+                                  perturb=TURBULENT_START if args.turbulent else None, log=log)
     print(json.dumps(report, indent=1))
     if args.out:
         with open(args.out + ".json", "w") as fh:

@@ -6,6 +6,8 @@
 !            zero (initial.f90:84), so none is added.  Laminar runs start at rest.
 !   initpop  Channel-Flow/initial.f90:19-46: rho = 0 and f = equilibrium of u,
 !            evaluated on the device.
+! The law-of-the-wall loop restates initial.f90:104-115 (GPL-3) closely, as
+! bitwise-equal initial fields require; it keeps that file's licence.
 !=============================================================================
       subroutine initvel
       use channel_vars

@@ -46,6 +46,13 @@
  * push streaming with bounce-back; mass conservation, y/z translation
  * and x-reflection invariants; the laminar start-up against the analytic
  * Poiseuille series the reference prints (saveload.f90:920-936).
+ *
+ * Origin and licence: because it restates UDel-CFD/D3Q19-Single-Phase
+ * (Channel-Flow/*.f90, GNU General Public License version 3) statement by
+ * statement, keeping its variable names (evlm*, eqmc*, tl*, sum*) and its
+ * operand order, this file is a derivative of that GPL-3 code and is
+ * distributed under the GPL-3 terms.  It is test infrastructure only: the
+ * product library does not link, load or call it.
  */
 #include <math.h>
 #include <stdlib.h>
