@@ -542,56 +542,86 @@ __global__ void __launch_bounds__(PBLOCK) k_p_solid_fields(PartGeom pg, PartStat
 
 // Refill: nodes uncovered by particle p get the equilibrium of the wall
 // velocity there and the mean density of their settled fluid neighbours.
-template <int P>
-__device__ inline void refill_node(const PartGeom &pg, const PartState &ps, typename Prec<P>::S *L, uint64_t N,
-                                   int p, int k)
-{
-    const uint32_t n = (uint32_t)ps.fill_node[(size_t)p * pg.maxfill + k];
-    if (ps.solid[n] != -2 - p)
-        return; // claimed by another particle since
-    const uint32_t qn = fdiv(n, pg.g.dlx);
-    const int ix = (int)(n - qn * pg.g.lx);
-    const int iz = (int)fdiv(qn, pg.g.dly);
-    const int iy = (int)(qn - (uint32_t)iz * pg.g.ly);
-    double rsum = 0.0;
-    int cnt = 0;
-    for (int j = 1; j < 19; j++) {
-        const int jx = ix + CX[j];
-        if (jx < 0 || jx >= (int)pg.g.lx)
-            continue;
-        const uint32_t m = node_of(pg.g, jx, wrap(iy + CY[j], pg.g.ly), wrap(iz + CZ[j], pg.g.lz));
-        if (ps.solid[m] != -1)
-            continue;
-        double r = ld_rt<P>(L, N, 0, m);
-        for (int i = 1; i < 19; i++)
-            r = r + ld_rt<P>(L, N, i, m);
-        rsum = rsum + r;
-        cnt++;
-    }
-    const double rho = cnt > 0 ? rsum / (double)cnt : 0.0;
-    const double *X = ps.X + 3 * p, *V = ps.V + 3 * p, *W = ps.W + 3 * p;
-    double dx, dy, dz;
-    node_offset(pg, ix, iy, iz, X, dx, dy, dz);
-    const double ux = V[0] + (W[1] * dz - W[2] * dy);
-    const double uy = V[1] + (W[2] * dx - W[0] * dz);
-    const double uz = V[2] + (W[0] * dy - W[1] * dx);
-    double usqr = ux * ux + uy * uy + uz * uz;
-    usqr = 1.5 * usqr;
-    st_rt<P>(L, N, 0, n, (1.0 / 3.0) * (rho - usqr));
-    for (int i = 1; i < 19; i++) {
-        const double G = (double)CX[i] * ux + (double)CY[i] * uy + (double)CZ[i] * uz;
-        const double w = i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0;
-        st_rt<P>(L, N, i, n, w * (rho + 3.0 * G + 4.5 * G * G - usqr));
-    }
-}
+// Refill of the uncovered nodes of particle p, REFILL_NODES nodes per wave
+// pass.  Lane 18 t + (j - 1) of a pass serves node t of the pass and its
+// neighbour x + c_j (j = 1..18): it loads that neighbour's flag and, for a
+// fluid neighbour inside the walls, its 19 populations, and sums them in
+// population order (its density).  Every lane of the node then adds the 18
+// densities in j order from LDS, as one thread walking the neighbours would,
+// so rho, and with it every stored value, does not depend on the lane
+// layout.  The node gets the equilibrium at rho and the wall velocity
+// V + W x r; lane j stores population j, lane 1 also population 0.  A node
+// claimed by another particle since it was listed is left alone.
+constexpr int REFILL_NODES = 3; // 3 x 18 = 54 of the 64 lanes
 
 template <int P>
 __global__ void __launch_bounds__(64) k_p_refill(PartGeom pg, PartState ps, void *Lv)
 {
+    using S = typename Prec<P>::S;
+    S *L = (S *)Lv;
+    const uint64_t N = pg.g.N;
     const int p = blockIdx.y;
     const int nf = ps.fill_count[p];
-    for (int k = blockIdx.x * 64 + threadIdx.x; k < nf; k += gridDim.x * 64)
-        refill_node<P>(pg, ps, (typename Prec<P>::S *)Lv, pg.g.N, p, k);
+    const int l = threadIdx.x;
+    const int t = l / 18, j = l % 18 + 1;
+    __shared__ double rs[REFILL_NODES][18];
+    __shared__ int ok[REFILL_NODES][18];
+    for (int k0 = 0; k0 < nf; k0 += REFILL_NODES) {
+        const int k = k0 + t;
+        const bool lane = t < REFILL_NODES && k < nf;
+        uint32_t n = 0;
+        int ix = 0, iy = 0, iz = 0;
+        bool mine = false;
+        if (lane) {
+            n = (uint32_t)ps.fill_node[(size_t)p * pg.maxfill + k];
+            mine = ps.solid[n] == -2 - p;
+            const uint32_t qn = fdiv(n, pg.g.dlx);
+            ix = (int)(n - qn * pg.g.lx);
+            iz = (int)fdiv(qn, pg.g.dly);
+            iy = (int)(qn - (uint32_t)iz * pg.g.ly);
+        }
+        if (lane && mine) {
+            double r = 0.0;
+            int use = 0;
+            const int jx = ix + CX[j];
+            if (jx >= 0 && jx < (int)pg.g.lx) {
+                const uint32_t m = node_of(pg.g, jx, wrap(iy + CY[j], pg.g.ly), wrap(iz + CZ[j], pg.g.lz));
+                if (ps.solid[m] == -1) {
+                    r = ld_rt<P>(L, N, 0, m);
+                    for (int i = 1; i < 19; i++)
+                        r = r + ld_rt<P>(L, N, i, m);
+                    use = 1;
+                }
+            }
+            rs[t][j - 1] = r;
+            ok[t][j - 1] = use;
+        }
+        __syncthreads();
+        if (lane && mine) {
+            double rsum = 0.0;
+            int cnt = 0;
+            for (int jj = 0; jj < 18; jj++)
+                if (ok[t][jj]) {
+                    rsum = rsum + rs[t][jj];
+                    cnt++;
+                }
+            const double rho = cnt > 0 ? rsum / (double)cnt : 0.0;
+            const double *X = ps.X + 3 * p, *V = ps.V + 3 * p, *W = ps.W + 3 * p;
+            double dx, dy, dz;
+            node_offset(pg, ix, iy, iz, X, dx, dy, dz);
+            const double ux = V[0] + (W[1] * dz - W[2] * dy);
+            const double uy = V[1] + (W[2] * dx - W[0] * dz);
+            const double uz = V[2] + (W[0] * dy - W[1] * dx);
+            double usqr = ux * ux + uy * uy + uz * uz;
+            usqr = 1.5 * usqr;
+            if (j == 1)
+                st_rt<P>(L, N, 0, n, (1.0 / 3.0) * (rho - usqr));
+            const double G = (double)CX[j] * ux + (double)CY[j] * uy + (double)CZ[j] * uz;
+            const double w = j <= 6 ? 1.0 / 18.0 : 1.0 / 36.0;
+            st_rt<P>(L, N, j, n, w * (rho + 3.0 * G + 4.5 * G * G - usqr));
+        }
+        __syncthreads(); // the LDS slots are reused by the next pass
+    }
 }
 
 // Uncovered nodes become ordinary fluid once every refill has read its
@@ -650,7 +680,7 @@ hipError_t PartLaunch<P>::ibb(const PartGeom &pg, const PartState &ps, void *L, 
 template <int P>
 hipError_t PartLaunch<P>::refill(const PartGeom &pg, const PartState &ps, void *L, hipStream_t s)
 {
-    const dim3 grid(1, pg.np); // one 64-lane block per particle strides over its refill list
+    const dim3 grid(1, pg.np); // one 64-lane block per particle passes over its refill list
     hipLaunchKernelGGL((k_p_refill<P>), grid, dim3(64), 0, s, pg, ps, L);
     hipLaunchKernelGGL(k_p_settle, grid, dim3(64), 0, s, pg, ps);
     return hipGetLastError();

@@ -469,13 +469,27 @@ def closure_balance_study(lx=192, ly=768, lz=384, spinup=100000, window=300000, 
     return report, arrays
 
 
-def wall_profile(a, b, lxc, ystar_c, y_split=20.0, width=5.0):
-    """A two-level plane profile of nu_t/nu: a below y+ = y_split, b above,
-    joined by tanh over `width` wall units (coarse plane centres, both walls)."""
+# The wall-distance closure profile calibrated on the 2x-coarse Re_tau = 180
+# channel (96 x 384 x 192, nu = 0.0018) against the filtered statistics of
+# the 192 x 768 x 384 run: nu_t/nu = -0.03 below y+ = 10, -0.08 up to
+# y+ = 30, -0.15 beyond.  Use: plane_closure_params(wall_profile(
+# **WALL_CLOSURE_C5, lxc=96, ystar_c=...), nu_c, ystar_c).
+WALL_CLOSURE_C5 = dict(levels=(-0.03, -0.08, -0.15), splits=(10.0, 30.0), width=5.0)
+
+
+def wall_profile(levels, lxc, ystar_c, splits=(20.0,), width=5.0):
+    """A piecewise-constant plane profile of nu_t/nu in wall units: levels[0]
+    below y+ = splits[0], levels[k] between splits[k-1] and splits[k], the
+    last level beyond the last split; neighbouring levels joined by tanh over
+    `width` wall units (coarse plane centres, both walls)."""
+    if len(levels) != len(splits) + 1:
+        raise ValueError(f"{len(levels)} levels need {len(levels) - 1} splits, not {len(splits)}")
     i = np.arange(lxc)
     yplus = np.minimum(i + 0.5, lxc - i - 0.5) / ystar_c
-    w = 0.5 * (1.0 - np.tanh((yplus - y_split) / width))
-    return a * w + b * (1.0 - w)
+    g = np.full(lxc, float(levels[0]))
+    for k, ys in enumerate(splits):
+        g = g + (levels[k + 1] - levels[k]) * 0.5 * (1.0 + np.tanh((yplus - ys) / width))
+    return g
 
 
 def plane_closure_params(g, nu_c, ystar_c, yscale=100.0, nut_min=-0.5, iterations=3000, lr=3e-3, seed=1):
@@ -500,7 +514,7 @@ def plane_closure_params(g, nu_c, ystar_c, yscale=100.0, nut_min=-0.5, iteration
 
 
 def closure_plane_study(reference, lx=192, ly=768, lz=384, spinup=100000, window=300000, stats_every=100,
-                        shapes=((0.0, -0.1),), seeds=(2,), plain=False, y_split=20.0, width=5.0, nut_min=-0.5,
+                        shapes=((0.0, -0.1),), seeds=(2,), plain=False, splits=(20.0,), width=5.0, nut_min=-0.5,
                         perturb=TURBULENT_START, log=None):
     """Coarse runs with wall-distance closures against a fine reference.
 
@@ -510,7 +524,8 @@ def closure_plane_study(reference, lx=192, ly=768, lz=384, spinup=100000, window
     (plane_closure_params); each is run on the coarse channel for every seed
     of the start perturbation, over the closure studies' window, and its
     statistics compared with the filtered fine ones.  plain=True adds the
-    plain coarse run of every seed."""
+    plain coarse run of every seed.  A shape is a tuple of levels of
+    wall_profile for the given splits."""
     say = log or (lambda m: None)
     ref = np.load(reference)
     fine_filt, fine_stats = ref["fine_filtered_profile"], ref["fine_profile"]
@@ -521,13 +536,13 @@ def closure_plane_study(reference, lx=192, ly=768, lz=384, spinup=100000, window
     nu_c, ystar_c = pc["visc"], pc["ystar"]
     total = spinup + window
     report = {"reference": str(reference), "coarse_grid": [lxc, lyc, lzc], "spinup": spinup, "window": window,
-              "y_split": y_split, "width": width, "nut_min": nut_min, "runs": []}
-    cases = ([("plain", None, None)] if plain else []) + [(f"a={a:+.3f} b={b:+.3f}", a, b) for a, b in shapes]
+              "splits": list(splits), "width": width, "nut_min": nut_min, "runs": []}
+    cases = ([("plain", None)] if plain else []) + [("g=" + ",".join(f"{v:+.3f}" for v in sh), sh) for sh in shapes]
     profiles = {}
-    for name, a, b in cases:
+    for name, sh in cases:
         params, fit = (None, None)
-        if a is not None:
-            g = wall_profile(a, b, lxc, ystar_c, y_split, width)
+        if sh is not None:
+            g = wall_profile(sh, lxc, ystar_c, splits, width)
             params, fit = plane_closure_params(g, nu_c, ystar_c, nut_min=nut_min)
         for sd in seeds:
             r = _coarse_run(lxc, lyc, lzc, nu_c, total // 2, spinup // 2, stats_every // 2, perturb, sd,
@@ -551,7 +566,8 @@ def main(argv=None):
     ap.add_argument("study", choices=["precision", "closure", "closure-balance", "closure-plane"])
     ap.add_argument("--reference", default=None, help="closure-plane: .npz of a closure study (fine profiles)")
     ap.add_argument("--shapes", default="0,-0.1",
-                    help="closure-plane: ';'-separated a,b pairs (nu_t/nu below / above y+ = 20)")
+                    help="closure-plane: ';'-separated nu_t/nu levels between the --splits")
+    ap.add_argument("--splits", default="20", help="closure-plane: comma list of y+ between the levels")
     ap.add_argument("--seeds", default="2", help="closure-plane: comma list of start-perturbation seeds")
     ap.add_argument("--plain", action="store_true", help="closure-plane: also the plain coarse run per seed")
     ap.add_argument("--rounds", type=int, default=2, help="closure-balance: target updates from coarse runs")
@@ -583,6 +599,7 @@ def main(argv=None):
         report, profiles = closure_plane_study(args.reference, lx, ly, lz, args.spinup, args.window,
                                                args.stats_every, shapes=shapes,
                                                seeds=[int(v) for v in args.seeds.split(",")], plain=args.plain,
+                                               splits=[float(v) for v in args.splits.split(",")],
                                                nut_min=args.nut_min, log=log)
         print(json.dumps(report, indent=1))
         if args.out:
