@@ -251,6 +251,30 @@ def test_balance_calibrated_closure_reproduces_a_laminar_reference():
     assert e1 < 0.1 * e0, (e0, e1)
 
 
+def test_wall_distance_closure_applies_its_profile():
+    """The calibrated wall-distance closure of the coarse channel: the
+    network the device's kernels fit to the plane profile (inputs other than
+    the wall distance scaled out) applies that profile at every node of the
+    fused collision, whatever the flow."""
+    from d3q19 import studies
+    lxc, ly, lz = 96, 8, 6
+    nu_c = 0.0018
+    with d3q19.Channel(lxc, ly, lz, "fp32", mrttype=3, visc=nu_c) as ch:
+        ystar_c = ch.p["ystar"]
+        g = studies.wall_profile(lxc=lxc, ystar_c=ystar_c, **studies.WALL_CLOSURE_C5)
+        params, fit = studies.plane_closure_params(g, nu_c, ystar_c)
+        ux, uy, uz = workloads.laminar_parabola_noise(lxc, ly, lz, seed=8, uc=0.1, amp=0.3)
+        ch.set_macros(ux=ux, uy=uy, uz=uz)
+        ch.FORCING()
+        ch.initpop()
+        ch.macrovar()
+        ch.closure_set(params, ystar=ystar_c)
+        nut = ch.closure_nut().astype(np.float64)
+    assert fit < 0.02, fit
+    assert g.min() < -0.14 and g.max() > -0.04
+    assert np.max(np.abs(nut - g[None, None, :])) < 0.02, np.max(np.abs(nut - g[None, None, :]))
+
+
 def test_vanishing_closure_is_plain_mrt():
     lx, ly, lz = 32, 24, 16
     vel = _fine_flow(lx, ly, lz, 5)
