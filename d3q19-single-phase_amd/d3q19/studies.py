@@ -192,7 +192,9 @@ def _asymmetry(p):
 
 
 def _coarse_run(lx, ly, lz, nu_c, steps, stats_from, stats_every, perturb, seed, closure=None, log=None,
-                half_at=None):
+                half_at=None, snap_at=(), on_snap=None):
+    """One coarse run with statistics after stats_from; on_snap(ch) is called
+    after the steps in snap_at (its time is not counted in wall_s)."""
     with Channel(lx, ly, lz, "fp32", mrttype=3, visc=nu_c) as ch:
         _start(ch, perturb, seed)
         if closure is not None:
@@ -202,6 +204,7 @@ def _coarse_run(lx, ly, lz, nu_c, steps, stats_from, stats_every, perturb, seed,
         t0 = time.perf_counter()
         done = 0
         first_half = None
+        t_snap = 0.0
         while done < steps:
             ch.run(stats_every)
             done += stats_every
@@ -209,17 +212,23 @@ def _coarse_run(lx, ly, lz, nu_c, steps, stats_from, stats_every, perturb, seed,
                 ch.stats_accumulate()
             if done == half_at:
                 first_half, _ = ch.stats_read()
+            if done in snap_at:
+                ch.sync()
+                ts = time.perf_counter()
+                on_snap(ch)
+                t_snap += time.perf_counter() - ts
             if log and done % max(steps // 5, 1) == 0:
                 log(f"coarse{' + closure' if closure is not None else ''}: step {done}/{steps}")
         ch.sync()
-        wall = time.perf_counter() - t0
+        wall = time.perf_counter() - t0 - t_snap
         stats, n = ch.stats_read()
-        nut_plane = None
-        if closure is not None:  # the closure's plane-mean nu_t/nu on the last state
+        nut_plane = nut_plane_std = None
+        if closure is not None:  # the closure's nu_t/nu by plane on the last state
             ch.macrovar()
-            nut_plane = ch.closure_nut().astype(np.float64).mean(axis=(0, 1))
+            nut = ch.closure_nut().astype(np.float64)
+            nut_plane, nut_plane_std = nut.mean(axis=(0, 1)), nut.std(axis=(0, 1))
         return {"stats": stats, "first_half": first_half, "samples": n, "wall_s": wall,
-                "mlups": lx * ly * lz * steps / wall / 1e6, "nut_plane": nut_plane}
+                "mlups": lx * ly * lz * steps / wall / 1e6, "nut_plane": nut_plane, "nut_plane_std": nut_plane_std}
 
 
 def _fine_reference(lx, ly, lz, spinup, window, stats_every, snapshots, seed, perturb, make_trainers, say):
@@ -561,9 +570,94 @@ def closure_plane_study(reference, lx=192, ly=768, lz=384, spinup=100000, window
     return report, profiles
 
 
+def closure_local_study(reference, lx=192, ly=768, lz=384, spinup=100000, window=300000, stats_every=100,
+                        levels=WALL_CLOSURE_C5["levels"], splits=WALL_CLOSURE_C5["splits"],
+                        width=WALL_CLOSURE_C5["width"], train_seed=2, seeds=(3, 4), snapshots=4, stride=2,
+                        iterations=4000, batch=65536, lr=2e-3, nut_min=-0.5, perturb=TURBULENT_START, log=None):
+    """The calibrated plane profile learned from all ten network inputs.
+
+    1. A coarse run (start seed train_seed) with the wall-distance network of
+       the profile; at `snapshots` times in its statistics window the nine
+       non-equilibrium inputs and the wall distance of every node of every
+       stride-th row (y and z) are read through lbm_closure_inputs (a probe
+       closure with unit scales, set for the read only).
+    2. The network is trained by the device's kernels on those samples, each
+       with the profile's value at its plane as the target and the inputs
+       scaled by their spread, so it may use the non-equilibrium moments as
+       well as the wall distance.
+    3. Coarse runs on `seeds` with that network, against the filtered fine
+       statistics of `reference`; the spread of nu_t/nu within each plane
+       shows how far the network's output varies from node to node.
+    """
+    from . import Trainer, closure_params
+    say = log or (lambda m: None)
+    ref = np.load(reference)
+    fine_filt, fine_stats = ref["fine_filtered_profile"], ref["fine_profile"]
+    lxc, lyc, lzc = lx // 2, ly // 2, lz // 2
+    pc = para(lxc, mrttype=3, visc=para(lx, mrttype=3)["visc"] / 2.0)
+    nu_c, ystar_c = pc["visc"], pc["ystar"]
+    steps, stats_from, every = (spinup + window) // 2, spinup // 2, stats_every // 2
+    g = wall_profile(levels, lxc, ystar_c, splits, width)
+    params_y, fit_y = plane_closure_params(g, nu_c, ystar_c, nut_min=nut_min)
+    probe = closure_params(init="zero", nu=nu_c, nut_min=nut_min)
+    probe[1440] = np.log(np.expm1(-nut_min))  # nu_t = 0: only its inputs are read
+    snap_at = {int(round(v / every)) * every for v in np.linspace(stats_from + (steps - stats_from) / snapshots,
+                                                                     steps, snapshots)}
+    feats = []
+
+    def grab(ch):
+        ch.macrovar()
+        ch.closure_set(probe, ystar=ystar_c)
+        feats.append(np.ascontiguousarray(ch.closure_inputs()[::stride, ::stride]).reshape(-1, 10))
+        ch.closure_set(params_y, ystar=ystar_c)
+
+    r = _coarse_run(lxc, lyc, lzc, nu_c, steps, stats_from, every, perturb, train_seed, closure=params_y,
+                    snap_at=snap_at, on_snap=grab)
+    report = {"reference": str(reference), "levels": list(levels), "splits": list(splits), "width": width,
+              "train_seed": train_seed, "snapshots": len(feats), "stride": stride,
+              "wall_distance_fit": fit_y,
+              "train_run": {"errors_vs_filtered_fine": _stat_errors(r["stats"], fine_filt)}}
+    X = np.concatenate(feats)
+    del feats
+    sc = X[:, :9].astype(np.float64).std(axis=0)
+    sc[~(sc > 1e-12)] = 1.0
+    X[:, :9] /= sc.astype(np.float32)
+    plane = np.arange(X.shape[0]) % lxc  # rows keep every plane, x fastest
+    t = g[plane].astype(np.float32)
+    n = X.shape[0]
+    with Trainer(n, closure_params(seed=1, scales=sc, nu=nu_c, nut_min=nut_min)) as tr:
+        tr.add_samples(X, t)
+        loss = tr.train(0, n, iterations=iterations, batch=min(batch, n), lr=lr)
+        pred = tr.predict(0, n).astype(np.float64)
+        params_local = tr.params()
+    pm = np.bincount(plane, pred, lxc) / np.bincount(plane, None, lxc)
+    ps = np.sqrt(np.maximum(np.bincount(plane, pred * pred, lxc) / np.bincount(plane, None, lxc) - pm * pm, 0.0))
+    report["training"] = {"samples": int(n), "iterations": iterations, "last_batch_loss": float(loss),
+                          "input_scales": [float(v) for v in sc],
+                          "plane_mean_max_error": float(np.max(np.abs(pm - g))),
+                          "plane_std_max": float(ps.max()), "rms_error": float(np.sqrt(np.mean((pred - t) ** 2)))}
+    say(f"local network: plane-mean error {np.max(np.abs(pm - g)):.4f}, in-plane spread up to {ps.max():.4f}")
+    del X, t, pred
+    report["runs"] = []
+    profiles = {}
+    for sd in seeds:
+        r = _coarse_run(lxc, lyc, lzc, nu_c, steps, stats_from, every, perturb, sd, closure=params_local)
+        e = {"seed": sd, "wall_s": round(r["wall_s"], 2), "mlups": round(r["mlups"], 1),
+             "errors_vs_filtered_fine": _stat_errors(r["stats"], fine_filt),
+             "errors_vs_fine": _stat_errors(r["stats"], 0.5 * (fine_stats[0::2] + fine_stats[1::2])),
+             "wall_to_wall_asymmetry": _asymmetry(r["stats"]),
+             "applied_plane_mean": [round(float(v), 4) for v in r["nut_plane"][:lxc // 2]],
+             "applied_plane_std": [round(float(v), 4) for v in r["nut_plane_std"][:lxc // 2]]}
+        report["runs"].append(e)
+        profiles[f"local_seed={sd}"] = r["stats"]
+        say(f"local closure seed {sd}: " + " ".join(f"{k} {v * 100:.2f}%" for k, v in
+                                                   e["errors_vs_filtered_fine"].items()))
+    return report, profiles, params_local
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("study", choices=["precision", "closure", "closure-balance", "closure-plane"])
+    ap.add_argument("study", choices=["precision", "closure", "closure-balance", "closure-plane", "closure-local"])
     ap.add_argument("--reference", default=None, help="closure-plane: .npz of a closure study (fine profiles)")
     ap.add_argument("--shapes", default="0,-0.1",
                     help="closure-plane: ';'-separated nu_t/nu levels between the --splits")
@@ -594,6 +688,18 @@ def main(argv=None):
     args = ap.parse_args(argv)
     lx, ly, lz = (int(v) for v in args.grid.split(","))
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.study == "closure-local":
+        report, profiles, params = closure_local_study(args.reference, lx, ly, lz, args.spinup, args.window,
+                                                       args.stats_every,
+                                                       seeds=[int(v) for v in args.seeds.split(",")],
+                                                       iterations=args.iterations, batch=args.batch, lr=args.lr,
+                                                       nut_min=args.nut_min, log=log)
+        print(json.dumps(report, indent=1))
+        if args.out:
+            with open(args.out + ".json", "w") as fh:
+                json.dump(report, fh, indent=1)
+            np.savez(args.out + ".npz", params_local=params, **profiles)
+        return
     if args.study == "closure-plane":
         shapes = [tuple(float(v) for v in sh.split(",")) for sh in args.shapes.split(";") if sh]
         report, profiles = closure_plane_study(args.reference, lx, ly, lz, args.spinup, args.window,
