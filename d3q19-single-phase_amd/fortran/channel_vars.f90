@@ -37,6 +37,13 @@
       integer :: npart = 0
       real(8) :: rad = 5.0d0, rhopart = 1.0d0, mingap = 3.0d0, mingap_w = 3.0d0
       character(len=200) :: partfile = 'partpos.dat'
+      ! learned closure (no reference counterpart): the LBM_MLP_TOTAL FP32
+      ! parameters of lbm_closure_set as raw little-endian floats in
+      ! closurefile ('' = plain MRT), FP32/FP16 storage only; visc_in > 0
+      ! replaces the turbulent branch's nu (a 2x-coarse run of the
+      ! Re_tau = 180 channel: lx = 96, visc_in = 0.0018)
+      character(len=200) :: closurefile = ''
+      real(8) :: visc_in = 0.0d0
 
       ! time stepping
       integer :: istep, istep0, istat, imovie
@@ -68,7 +75,7 @@
                          device, mrttype_in, laminarflow, newrun, newinitflow,  &
                          savecntd, istpload, dirgenr, rhoepsl, nprocy_in, nprocz_in, &
                          nprocy_out, nprocz_out, ndiss, ipart, npart, rad, rhopart, &
-                         mingap, mingap_w, partfile
+                         mingap, mingap_w, partfile, closurefile, visc_in
 
       contains
 
@@ -76,6 +83,7 @@
       subroutine channel_para
         if (.not. laminarflow) then
           visc = 0.0036d0
+          if (visc_in > 0.0d0) visc = visc_in
           rstar = 180.0d0
           ustar = 2.0d0*rstar*visc/dble(lx)
           force_in_y = 2.d0*rho0*ustar*ustar/dble(lx)

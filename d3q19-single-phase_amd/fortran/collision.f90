@@ -11,6 +11,8 @@
 !   FORCING        -> lbm_set_force       (collision.f90:515-527)
 !   FORCINGP       -> lbm_forcingp        (collision.f90:529-602)
 !   run_steps(n)   -> lbm_run             (n x collision_MRT + macrovar)
+!   closure_load   -> lbm_closure_set     (the learned closure; no reference
+!                                          counterpart)
 ! macrovar leaves the fields on the device; fetch_fields copies them to the
 ! host arrays when an I/O routine needs them.
 !=============================================================================
@@ -25,6 +27,29 @@
       implicit none
       call lbm_check(lbm_macrovar(lbm, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), 'macrovar')
       end subroutine macrovar
+
+      ! The learned closure inside every following collision_MRT: the
+      ! LBM_MLP_TOTAL FP32 parameters read from closurefile (stream access,
+      ! raw floats), ystar of the run.
+      subroutine closure_load
+      use channel_vars
+      implicit none
+      real(c_float) :: params(LBM_MLP_TOTAL)
+      integer :: u, ios
+      open(newunit=u, file=trim(closurefile), access='stream', form='unformatted', status='old', &
+           action='read', iostat=ios)
+      if (ios /= 0) then
+        write(*,*) 'cannot open closure file ', trim(closurefile)
+        error stop 2
+      end if
+      read(u, iostat=ios) params
+      close(u)
+      if (ios /= 0) then
+        write(*,*) 'closure file ', trim(closurefile), ' holds fewer than ', LBM_MLP_TOTAL, ' values'
+        error stop 2
+      end if
+      call lbm_check(lbm_closure_set(lbm, params, ystar), 'lbm_closure_set')
+      end subroutine closure_load
 
       ! n iterations of collision_MRT followed by macrovar (main.f90:157-161)
       ! in one library call.

@@ -60,6 +60,7 @@
       ! main.f90:132-137
       call FORCING
       call macrovar
+      if (len_trim(closurefile) > 0) call closure_load
       if (ipart) then
         if (npforcing > 0) then
           write(*,*) 'the perturbation forcing is not used with particles (main.f90:148)'

@@ -85,6 +85,44 @@ def test_driver_injected_start_bitwise(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_runs_the_learned_closure_bitwise(tmp_path):
+    """closurefile: a 2x-coarse-style run (FP32 storage, visc_in = 0.0018,
+    rate set 3) with the learned closure read from a raw float file.  The
+    driver's checkpoint equals the Python host's run of the same library with
+    the same parameters and call sequence, bit for bit, and differs from the
+    plain run's."""
+    import mlp_oracle as M
+    lx, ly, lz, nsteps, nu = 32, 12, 8, 40, 0.0018
+    for d in ("initflow", "cntdflow", "diag", "stat"):
+        (tmp_path / d).mkdir()
+    ref = O.OracleChannel(lx, ly, lz)
+    ref.ux[:], ref.uy[:], ref.uz[:] = workloads.laminar_parabola_noise(lx, ly, lz, seed=23, amp=0.3)
+    ref.initpop()
+    fortio.write_finit(str(tmp_path / "initflow" / "finit.000"), ref.f)
+    p = d3q19.closure_params(seed=5, nu=nu, nut_min=-0.5, scales=np.full(9, 1e-4))
+    p[M.B3] = 0.3  # nu_t/nu = softplus(0.3) - 0.5 ~ 0.35 plus the network's variation
+    p.astype("<f4").tofile(tmp_path / "closure.bin")
+    common = dict(lx=lx, ly=ly, lz=lz, nsteps=nsteps, newinitflow=".false.", ndiag=0, nstat=0, precision=1,
+                  mrttype_in=3, visc_in=nu, dirgenr=f"'{tmp_path}/'")
+    _nml(tmp_path / "run.nml", closurefile=f"'{tmp_path}/closure.bin'", **common)
+    _run(tmp_path / "run.nml", tmp_path)
+    name = str(tmp_path / "cntdflow" / f"endrunflow2D16x8.{nsteps:07d}.000")
+    _, f_drv = fortio.read_cntd(name, ref.f.shape)
+    _nml(tmp_path / "plain.nml", **common)
+    _run(tmp_path / "plain.nml", tmp_path)
+    _, f_plain = fortio.read_cntd(name, ref.f.shape)
+    with d3q19.Channel(lx, ly, lz, "fp32", mrttype=3, visc=nu) as ch:
+        ch.upload_f(ref.f)
+        ch.FORCING()
+        ch.macrovar()
+        ch.closure_set(p, ystar=ch.p["ystar"])
+        ch.run(nsteps)
+        f_py = ch.download_f()
+    assert np.array_equal(f_drv.view(np.uint64), f_py.view(np.uint64))
+    assert np.max(np.abs(f_drv - f_plain)) > 1e-6
+
+
+@pytest.mark.gpu
 def test_driver_fresh_start_prerelax_bitwise(tmp_path):
     lx, ly, lz, nsteps = 32, 8, 6, 40
     for d in ("initflow", "cntdflow", "diag", "stat"):
