@@ -189,28 +189,13 @@ __global__ void __launch_bounds__(PBLOCK) k_p_links(PartGeom pg, PartState ps)
         int tot;
         const int off = block_scan(__builtin_popcount(mask), tot);
         if (mask) {
-            double dx, dy, dz;
-            node_offset(pg, ix, iy, iz, X, dx, dy, dz);
-            const double d2 = dx * dx + dy * dy + dz * dz;
-            const double cc = d2 - pg.R2;
             int k = base + off;
-            for (int i = 1; i < 19; i++) {
-                if (!(mask & (1u << i)))
-                    continue;
-                const double cx = (double)CX[i], cy = (double)CY[i], cz = (double)CZ[i];
-                const double a = cx * cx + cy * cy + cz * cz;
-                const double bb = cx * dx + cy * dy + cz * dz;
-                const double disc = bb * bb - a * cc;
-                double q = (disc >= 0.0) ? (-bb - sqrt(disc)) / a : 0.5;
-                if (q < 0.0)
-                    q = 0.0;
-                if (q > 1.0)
-                    q = 1.0;
+            for (uint32_t m = mask; m != 0; m &= m - 1) { // set bits, lowest direction first
+                const int i = __builtin_ctz(m);
                 if (k < pg.maxlink) {
                     const size_t s = (size_t)p * pg.maxlink + k;
                     ps.link_node[s] = (int)n;
                     ps.link_dir[s] = (signed char)i;
-                    ps.link_q[s] = q;
                 } else {
                     atomicOr(ps.overflow, 1);
                 }
@@ -221,6 +206,41 @@ __global__ void __launch_bounds__(PBLOCK) k_p_links(PartGeom pg, PartState ps)
     }
     if (threadIdx.x == 0)
         ps.link_count[p] = base < pg.maxlink ? base : pg.maxlink;
+}
+
+// Phase 4: the cut-link fraction q of every link, one lane per link: the
+// smaller root of |x + t c_i - X|^2 = R^2, clamped to [0, 1] (1/2 if the
+// line misses the sphere).  |c_i|^2 is 1 or 2, so the division by it is the
+// exact product with 1 or 1/2.
+__global__ void __launch_bounds__(PBLOCK) k_p_linkq(PartGeom pg, PartState ps)
+{
+    const int p = blockIdx.y;
+    const int nl = ps.link_count[p];
+    const double *X = ps.X + 3 * p;
+    for (int k = blockIdx.x * PBLOCK + threadIdx.x; k < nl; k += gridDim.x * PBLOCK) {
+        const size_t s = (size_t)p * pg.maxlink + k;
+        const uint32_t n = (uint32_t)ps.link_node[s];
+        const int i = ps.link_dir[s];
+        const uint32_t qn = fdiv(n, pg.g.dlx);
+        const int ix = (int)(n - qn * pg.g.lx);
+        const int iz = (int)fdiv(qn, pg.g.dly);
+        const int iy = (int)(qn - (uint32_t)iz * pg.g.ly);
+        double dx, dy, dz;
+        node_offset(pg, ix, iy, iz, X, dx, dy, dz);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        const double cc = d2 - pg.R2;
+        const double cx = (double)CX[i], cy = (double)CY[i], cz = (double)CZ[i];
+        const double a = cx * cx + cy * cy + cz * cz;
+        const double inv_a = i <= 6 ? 1.0 : 0.5;
+        const double bb = cx * dx + cy * dy + cz * dz;
+        const double disc = bb * bb - a * cc;
+        double q = (disc >= 0.0) ? (-bb - sqrt(disc)) * inv_a : 0.5;
+        if (q < 0.0)
+            q = 0.0;
+        if (q > 1.0)
+            q = 1.0;
+        ps.link_q[s] = q;
+    }
 }
 
 // Storage access with a run-time direction (FP16 scale by direction class).
@@ -646,6 +666,7 @@ hipError_t part_classify(const PartGeom &pg, const PartState &ps, int first, hip
         hipLaunchKernelGGL(k_p_uncover, dim3(pg.np), dim3(PBLOCK), 0, s, pg, ps, ps.X + 3 * (size_t)pg.np);
     hipLaunchKernelGGL(k_p_mark, dim3(pg.np), dim3(PBLOCK), 0, s, pg, ps);
     hipLaunchKernelGGL(k_p_links, dim3(pg.np), dim3(PBLOCK), 0, s, pg, ps);
+    hipLaunchKernelGGL(k_p_linkq, dim3(IBB_BLOCKS, pg.np), dim3(PBLOCK), 0, s, pg, ps);
     return hipGetLastError();
 }
 
