@@ -92,3 +92,22 @@ def test_balance_update_clips_and_checks_shapes():
     assert g.max() == 1.0
     with pytest.raises(ValueError):
         C.balance_update(U_ref[:-1], U_ref, 0.0)
+
+
+def test_wall_profile_levels_and_symmetry():
+    """The piecewise wall-distance profile of the closure studies: its levels
+    far from the splits, mirror symmetry about the centre plane, and the
+    calibrated profile's values at the walls and in the core."""
+    from d3q19 import studies
+    lxc, ystar = 96, 0.26666666666666666  # Re_tau = 180 on the 2x-coarse grid
+    g = studies.wall_profile((-0.03, -0.08, -0.15), lxc, ystar, splits=(10.0, 30.0), width=5.0)
+    np.testing.assert_allclose(g, g[::-1], rtol=0, atol=1e-15)
+    yplus = (np.arange(lxc) + 0.5) / ystar
+    assert abs(g[0] + 0.03) < 2e-3                      # y+ = 1.9
+    assert abs(g[np.argmin(abs(yplus - 20))] + 0.08) < 2e-3
+    assert abs(g[lxc // 2 - 1] + 0.15) < 1e-9           # core
+    assert np.all(np.diff(g[:lxc // 2]) <= 0)           # monotone towards the centre
+    same = studies.wall_profile(lxc=lxc, ystar_c=ystar, **studies.WALL_CLOSURE_C5)
+    np.testing.assert_array_equal(g, same)
+    with pytest.raises(ValueError):
+        studies.wall_profile((0.0, 0.1), lxc, ystar, splits=(10.0, 30.0))
