@@ -47,6 +47,10 @@ defers and pairs them into blocked sweeps), wall clock around the loop.
 "particles": the coupled step with 2000 spheres of radius 5 at the same
 turbulent constants (--particles 0 leaves it out).
 
+"diagnostics": diag, statistc2 and dissipation_calc reduced on the device,
+ms per call, beside the download of the four macroscopic fields into host
+arrays that a host-side evaluation of them starts from.
+
 cpu_baseline: the CPU oracle's out-of-place schedule (same arithmetic as the
 reference, OpenMP over z planes) on the host cores, on a bounded sample.
 """
@@ -218,6 +222,34 @@ def measure_closure(steps, warmup):
     return {"mlups": n * steps / wall / 1e6, "kernel_ms": avg_ms, "nodes": n}
 
 
+def measure_diagnostics(prec, workload, reps=5):
+    """Output-cadence work on the device, ms per call: diag
+    (Channel-Flow/saveload.f90:1507-1676), statistc2 (:1348-1502) and
+    dissipation_calc (:4644-4882), each reduced to a few hundred values on the
+    device; beside them the download of macrovar's four fields (4 x N doubles
+    into preallocated host arrays) that a host-side evaluation starts with."""
+    lx, ly, lz, _ = WORKLOADS[workload]
+    with d3q19.Channel(lx, ly, lz, prec, rstar=RSTAR.get(workload)) as ch:
+        init_channel(ch, lx, ly, lz)
+        ch.run(2)
+        host = [np.empty((lz, ly, lx)) for _ in range(4)]
+
+        def timed(fn, n):
+            fn()  # first call: allocations, page faults
+            ch.sync()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                fn()
+            ch.sync()
+            return round((time.perf_counter() - t0) / n * 1e3, 3)
+
+        out = {"diag_ms": timed(ch.diag, reps), "statistc2_ms": timed(ch.statistc2, reps),
+               "dissipation_calc_ms": timed(ch.dissipation, reps),
+               "fields_download_ms": timed(lambda: ch.macrovar(download=True, out=host), 2),
+               "fields_download_bytes": 4 * 8 * lx * ly * lz}
+    return out
+
+
 def measure_particles(prec, workload, npart, rad, steps, warmup):
     """Coupled particle step (beads_collision, collision_MRT with solid nodes
     skipped, macrovar, beads_move, beads_links, beads_filling) with npart
@@ -366,6 +398,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-closure", action="store_true", help="skip the coarse closure-run throughput")
     ap.add_argument("--no-forcingp", action="store_true", help="skip the perturbation-forcing step throughput")
+    ap.add_argument("--no-diagnostics", action="store_true", help="skip the device diagnostics timing")
     ap.add_argument("--particles", type=int, default=2000, metavar="NPART",
                     help="also time the coupled particle step with NPART spheres of radius 5 (FP64 and FP32)")
     ap.add_argument("--cpu-steps", type=int, default=20)
@@ -426,6 +459,14 @@ def main():
                     "hbm_frac_wall": round(fp["wall_frac"], 4),
                     "vs_uniform_force": round(fp["mlups"] / main_m["mlups"], 3)}
         log(f"bench: forcingp {fp['mlups']:.1f} MLUPS")
+
+    diagnostics = None
+    if not args.no_diagnostics:
+        log(f"bench: device diagnostics, {main_prec}")
+        diagnostics = measure_diagnostics(main_prec, args.workload)
+        log(f"bench: diag {diagnostics['diag_ms']} ms, statistc2 {diagnostics['statistc2_ms']} ms, "
+            f"dissipation_calc {diagnostics['dissipation_calc_ms']} ms, fields download "
+            f"{diagnostics['fields_download_ms']} ms")
 
     closure = None
     if args.workload == "c2" and not args.no_closure:
@@ -488,6 +529,7 @@ def main():
         "variants": variants,
         "closure_coarse": closure,
         "forcingp": forcingp,
+        "diagnostics": diagnostics,
         "particles": particles,
     }
     if rank == 0:

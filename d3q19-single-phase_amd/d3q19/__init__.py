@@ -305,12 +305,17 @@ class Channel:
         """Channel-Flow/collision.f90:24-273 (with collisionExchnge as the periodic wrap)."""
         _call("lbm_collision_mrt", self._h)
 
-    def macrovar(self, download=False):
-        """Channel-Flow/collision.f90:378-463.  With download=True returns (rho, ux, uy, uz)."""
+    def macrovar(self, download=False, out=None):
+        """Channel-Flow/collision.f90:378-463.  With download=True returns (rho, ux, uy, uz),
+        filling the four (lz, ly, lx) float64 arrays of `out` when given."""
         if not download:
             _call("lbm_macrovar", self._h, None, None, None, None)
             return None
-        out = [np.empty((self.lz, self.ly, self.lx)) for _ in range(4)]
+        if out is None:
+            out = [np.empty((self.lz, self.ly, self.lx)) for _ in range(4)]
+        for a in out:
+            if a.shape != (self.lz, self.ly, self.lx) or a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError("macrovar: out needs four C-contiguous (lz, ly, lx) float64 arrays")
         _call("lbm_macrovar", self._h, *[_ptr(a) for a in out])
         return tuple(out)
 
