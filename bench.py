@@ -401,7 +401,7 @@ def main():
     ap.add_argument("--no-diagnostics", action="store_true", help="skip the device diagnostics timing")
     ap.add_argument("--particles", type=int, default=2000, metavar="NPART",
                     help="also time the coupled particle step with NPART spheres of radius 5 (FP64 and FP32)")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=26)
     ap.add_argument("--traffic-probe", metavar="PREC", choices=sorted(BYTES_PER_NODE),
                     help="run the short program profiled by rocprofv3 --pmc (5 step launches + calibration copy)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
