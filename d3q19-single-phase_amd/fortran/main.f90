@@ -80,16 +80,19 @@
         if (istep <= npforcing) then
           ! perturbation forcing until npforcing (the block at main.f90:148-153):
           ! FORCINGP or FORCING, collision_MRT and macrovar per step, batched
-          ! up to npforcing and the next diag/statistc2 step
+          ! up to npforcing and the next output step
           nrun = min(npforcing, istep0 + nsteps) - istep + 1
           if (ndiag > 0) nrun = min(nrun, ndiag - mod(istep - 1, ndiag))
           if (nstat > 0) nrun = min(nrun, nstat - mod(istep - 1, nstat))
+          if (ndiss > 0) nrun = min(nrun, ndiss - mod(istep - 1, ndiss))
+          if (nflowout > 0) nrun = min(nrun, nflowout - mod(istep - 1, nflowout))
           call run_forcingp_steps(nrun)
         else
           nrun = istep0 + nsteps - istep + 1
           if (ndiag > 0) nrun = min(nrun, ndiag - mod(istep - 1, ndiag))
           if (nstat > 0) nrun = min(nrun, nstat - mod(istep - 1, nstat))
           if (ndiss > 0) nrun = min(nrun, ndiss - mod(istep - 1, ndiss))
+          if (nflowout > 0) nrun = min(nrun, nflowout - mod(istep - 1, nflowout))
           if (ipart) then
             ! the particle-laden loop body, avedensity after every 100th
             ! step included (main.f90:157-167)
@@ -107,6 +110,10 @@
         end if
         if (ndiss > 0) then
           if (mod(istep, ndiss) == 0) call dissipation_calc
+        end if
+        ! main.f90:184
+        if (nflowout > 0) then
+          if (mod(istep, nflowout) == 0) call outputflow
         end if
         istep = istep + 1
       end do

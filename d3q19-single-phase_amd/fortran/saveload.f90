@@ -10,7 +10,8 @@
 ! written as the set of files of an nprocY x nprocZ run (each rank its y/z
 ! block of f, para.f90:218-262), one file when both are 1.
 ! diag.dat, diag2.dat, profiles2.dat and dissipation.dat follow
-! saveload.f90:1507-1676, :1348-1502 and :4644-4882.
+! saveload.f90:1507-1676, :1348-1502 and :4644-4882; flowout/uy_profile and
+! flowout/pressCut follow outputflow (:696-714, :845-947, :1025-1120).
 !=============================================================================
       subroutine saveinitflow
       use channel_vars
@@ -187,6 +188,75 @@
       if (.not. allocated(ox)) allocate(ox(lx,ly,lz), oy(lx,ly,lz), oz(lx,ly,lz))
       call lbm_check(lbm_vortcalc(lbm, c_loc(ox), c_loc(oy), c_loc(oz)), 'vortcalc')
       end subroutine vortcalc
+
+      ! outputflow, saveload.f90:696-714: outputuy and outputpress from the
+      ! fields of the current step (downloaded, as the reference gathers them
+      ! on rank 0).
+      subroutine outputflow
+      use channel_vars
+      implicit none
+      call fetch_fields
+      call outputuy
+      call outputpress
+      end subroutine outputflow
+
+      ! outputuy, saveload.f90:845-947: flowout/uy_profileSSSSSSS.dat holds,
+      ! for each wall-normal node of the lower half, its index, its distance
+      ! from the centre plane, the time in viscous units of the half width,
+      ! the streamwise velocity / u* along y at z = lz/2, and the laminar
+      ! start-up and steady solutions / u* (:920-936).
+      subroutine outputuy
+      use channel_vars
+      implicit none
+      character(len=300) :: fnm
+      integer :: i, k, n
+      real(8) :: time1, xx0, uuss, uut
+      write(fnm, '(a,a,i7.7,a)') trim(dirgenr), 'flowout/uy_profile', istep, '.dat'
+      open(17, file=trim(fnm), status='unknown', form='formatted')
+      time1 = istep*visc/((real(lx, 8)/2.d0)**2)
+      k = lz/2
+      do i = 1, lx/2
+        xx0 = abs(real(i, 8) - 0.5d0 - real(lx, 8)/2.d0)
+        uuss = (4*ustar)/(real(lx, 8)**2)*((real(lx, 8)/2)**2 - xx0**2)
+        uut = 0.0d0
+        do n = 0, 25
+          uut = uut + (4*(-1)**n)/(pi*(n + 0.5d0))**3*exp(-((real(n, 8) + 0.5d0)**2)*(pi**2)*time1)* &
+                dcos((real(n, 8) + 0.5d0)*pi*(xx0/(real(lx, 8)/2.d0)))
+        end do
+        uut = (1 - (xx0/(real(lx, 8)/2.d0))**2) - uut
+        write(17, 170) real(i, 8), xx0, time1, uy(i, :, k)/ustar, uut, uuss/ustar
+      end do
+      close(17)
+170   format(2x, 8(1pe16.6))
+      end subroutine outputuy
+
+      ! outputpress, saveload.f90:1025-1120: flowout/pressCutSSSSSSS.dat holds
+      ! (y, z, rho/3) over every x and y node of the plane z = lx/2 (the
+      ! reference's k = nx/2), x outermost; nothing is written when lx/2 > lz,
+      ! where that plane does not exist.
+      subroutine outputpress
+      use channel_vars
+      implicit none
+      character(len=300) :: fnm
+      integer :: i, j, k
+      real(8) :: yy0, zz0
+      k = lx/2
+      if (k < 1 .or. k > lz) then
+        write(*,*) 'outputpress: no plane z = lx/2 =', k, ' in lz =', lz, '; pressCut not written'
+        return
+      end if
+      write(fnm, '(a,a,i7.7,a)') trim(dirgenr), 'flowout/pressCut', istep, '.dat'
+      open(20, file=trim(fnm), status='unknown', form='formatted')
+      zz0 = dble(k) - 0.5d0
+      do i = 1, lx
+        do j = 1, ly
+          yy0 = dble(j) - 0.5d0
+          write(20, 120) yy0, zz0, rho(i, j, k)/3.0d0
+        end do
+      end do
+      close(20)
+120   format(2x, 3(1pe16.6))
+      end subroutine outputpress
 
       ! Plane profiles, saveload.f90:1348-1502, computed on the device.
       subroutine statistc2

@@ -122,6 +122,51 @@ def test_driver_runs_the_learned_closure_bitwise(tmp_path):
     assert np.max(np.abs(f_drv - f_plain)) > 1e-6
 
 
+def _floats(path):
+    return np.array([float(t) for t in path.read_text().split()])
+
+
+@pytest.mark.gpu
+def test_driver_outputflow_files(tmp_path):
+    """outputflow every nflowout steps (main.f90:184, saveload.f90:696-1120).
+    uy_profileSSSSSSS.dat: per lower-half wall-normal node i, then i,
+    |i - 1/2 - lx/2|, the viscous time of the half width, u_y/u* along y at
+    z = lz/2, and the laminar start-up and steady solutions / u*.
+    pressCutSSSSSSS.dat: (y, z, rho/3) over x and y of the plane z = lx/2.
+    The fields against the oracle's at the same steps, to the 7 printed
+    digits; the analytic columns against the series evaluated in numpy."""
+    lx, ly, lz, nsteps, nfo = 16, 12, 8, 40, 20
+    for d in ("initflow", "cntdflow", "diag", "stat", "flowout"):
+        (tmp_path / d).mkdir()
+    ref = O.OracleChannel(lx, ly, lz)
+    ref.ux[:], ref.uy[:], ref.uz[:] = workloads.laminar_parabola_noise(lx, ly, lz, seed=24)
+    ref.initpop()
+    fortio.write_finit(str(tmp_path / "initflow" / "finit.000"), ref.f)
+    _nml(tmp_path / "run.nml", lx=lx, ly=ly, lz=lz, nsteps=nsteps, newinitflow=".false.", ndiag=0, nstat=0,
+         nflowout=nfo, dirgenr=f"'{tmp_path}/'")
+    _run(tmp_path / "run.nml", tmp_path)
+    ref.FORCING()
+    ref.macrovar()
+    us, nu, h = ref.p.ustar, ref.p.visc, lx / 2.0
+    for step in (20, 40):
+        ref.run(nfo)
+        got = _floats(tmp_path / "flowout" / f"uy_profile{step:07d}.dat").reshape(lx // 2, ly + 5)
+        i = np.arange(1, lx // 2 + 1, dtype=float)
+        xx0 = np.abs(i - 0.5 - h)
+        t1 = step * nu / h ** 2
+        n = np.arange(26)[:, None]
+        series = (4.0 * (-1.0) ** n / (np.pi * (n + 0.5)) ** 3 * np.exp(-((n + 0.5) ** 2) * np.pi ** 2 * t1)
+                  * np.cos((n + 0.5) * np.pi * xx0 / h)).sum(axis=0)
+        want = np.column_stack([i, xx0, np.full_like(i, t1), ref.uy[lz // 2 - 1, :, :lx // 2].T / us,
+                                (1.0 - (xx0 / h) ** 2) - series, 4 * us / lx ** 2 * (h ** 2 - xx0 ** 2) / us])
+        np.testing.assert_allclose(got, want, rtol=6e-7, atol=1e-12)
+        got = _floats(tmp_path / "flowout" / f"pressCut{step:07d}.dat").reshape(lx, ly, 3)
+        k = lx // 2
+        want = np.stack(np.broadcast_arrays((np.arange(ly) + 0.5)[None, :], np.full((1, 1), k - 0.5),
+                                            ref.rho[k - 1].T / 3.0), axis=-1)
+        np.testing.assert_allclose(got, want, rtol=6e-7, atol=0)
+
+
 @pytest.mark.gpu
 def test_driver_fresh_start_prerelax_bitwise(tmp_path):
     lx, ly, lz, nsteps = 32, 8, 6, 40
