@@ -49,10 +49,17 @@
           end do
           call saveinitflow
           call macrovar
+          istep0 = 0
+          istep = istep0
+          call statistc
+          call statistc2
         else
           call loadinitflow
+          istep0 = 0
+          istep = istep0
+          call statistc
+          call statistc2
         end if
-        istep0 = 0
       else
         call loadcntdflow
       end if

@@ -9,8 +9,8 @@
 ! device with lbm_download_f / lbm_upload_f.  Checkpoints are read from and
 ! written as the set of files of an nprocY x nprocZ run (each rank its y/z
 ! block of f, para.f90:218-262), one file when both are 1.
-! diag.dat, diag2.dat, profiles2.dat and dissipation.dat follow
-! saveload.f90:1507-1676, :1348-1502 and :4644-4882; flowout/uy_profile and
+! diag.dat, diag2.dat, profiles.dat, profiles2.dat and dissipation.dat
+! follow saveload.f90:1507-1676, :1202-1342, :1348-1502 and :4644-4882; flowout/uy_profile and
 ! flowout/pressCut follow outputflow (:696-714, :845-947, :1025-1120).
 !=============================================================================
       subroutine saveinitflow
@@ -257,6 +257,55 @@
       close(20)
 120   format(2x, 3(1pe16.6))
       end subroutine outputpress
+
+      ! statistc, saveload.f90:1202-1342: plane means of u / u*, the velocity
+      ! variances and shear stresses / u*^2, and the mean and variance of the
+      ! density fluctuation (/ u*^2, / u*^4) into stat/profiles.dat, from the
+      ! fields macrovar forms of the current lattice (the reference calls it
+      ! once, at the start).
+      subroutine statistc
+      use channel_vars
+      implicit none
+      real(8) :: vxave(lx), vyave(lx), vzave(lx), vxsq(lx), vysq(lx), vzsq(lx)
+      real(8) :: sxz(lx), sxy(lx), syz(lx), prave(lx), prsq(lx), yplus, area
+      integer :: i
+      call fetch_fields
+      do i = 1, lx
+        vxave(i) = sum(ux(i,:,:))
+        vyave(i) = sum(uy(i,:,:))
+        vzave(i) = sum(uz(i,:,:))
+        sxz(i) = sum(ux(i,:,:)*uz(i,:,:))
+        sxy(i) = sum(ux(i,:,:)*uy(i,:,:))
+        syz(i) = sum(uy(i,:,:)*uz(i,:,:))
+        vxsq(i) = sum(ux(i,:,:)**2)
+        vysq(i) = sum(uy(i,:,:)**2)
+        vzsq(i) = sum(uz(i,:,:)**2)
+        prave(i) = sum(rho(i,:,:))
+        prsq(i) = sum(rho(i,:,:)*rho(i,:,:))
+      end do
+      area = dble(ly)*dble(lz)
+      vxave = vxave/area/ustar
+      vyave = vyave/area/ustar
+      vzave = vzave/area/ustar
+      vxsq = vxsq/area/ustar**2 - vxave**2
+      vysq = vysq/area/ustar**2 - vyave**2
+      vzsq = vzsq/area/ustar**2 - vzave**2
+      sxz = sxz/area/ustar**2 - vxave*vzave
+      sxy = sxy/area/ustar**2 - vxave*vyave
+      syz = syz/area/ustar**2 - vyave*vzave
+      prave = prave/area/ustar**2
+      prsq = prsq/area/ustar**4 - prave*prave
+      open(27, file=trim(dirgenr)//'stat/profiles.dat', status='unknown', form='formatted', position='append')
+      write(27,*) istep
+      do i = 1, lx
+        yplus = (dble(i) - 0.5d0)/ystar
+        if (i > lx/2) yplus = (dble(lx - i) + 0.5d0)/ystar
+        write(27, 460) dble(i) - 0.5d0, yplus, vxave(i), vyave(i), vzave(i), vxsq(i), vysq(i), vzsq(i), &
+                       sxz(i), sxy(i), syz(i), prave(i), prsq(i)
+      end do
+      close(27)
+460   format(2x, 13(1pe15.6))
+      end subroutine statistc
 
       ! Plane profiles, saveload.f90:1348-1502, computed on the device.
       subroutine statistc2

@@ -183,6 +183,28 @@ def test_driver_fresh_start_prerelax_bitwise(tmp_path):
     _, finit = fortio.read_finit(str(tmp_path / "initflow" / "finit.000"), ref.f.shape)
     assert np.array_equal(finit.view(np.uint64), ref.f.view(np.uint64))
     ref.macrovar()
+    # statistc at the start (main.f90:96, saveload.f90:1202-1342): plane
+    # statistics of the relaxed flow's fields, 13 columns, to the printed digits
+    tok = (tmp_path / "stat" / "profiles.dat").read_text().split()
+    assert tok[0] == "0"
+    got = np.array([float(t) for t in tok[1:]]).reshape(lx, 13)
+    us, area = ref.p.ustar, ly * lz
+    mean = lambda a: a.sum(axis=(0, 1)) / area  # noqa: E731
+    vx, vy, vz = (mean(a) / us for a in (ref.ux, ref.uy, ref.uz))
+    pr = mean(ref.rho) / us ** 2
+    i = np.arange(1, lx + 1)
+    yplus = np.where(i > lx // 2, lx - i + 0.5, i - 0.5) / ref.p.ystar
+    raw = np.column_stack([i - 0.5, yplus, vx, vy, vz, mean(ref.ux ** 2) / us ** 2, mean(ref.uy ** 2) / us ** 2,
+                           mean(ref.uz ** 2) / us ** 2, mean(ref.ux * ref.uz) / us ** 2,
+                           mean(ref.ux * ref.uy) / us ** 2, mean(ref.uy * ref.uz) / us ** 2, pr,
+                           mean(ref.rho ** 2) / us ** 4])
+    want = raw - np.column_stack([0 * i, 0 * i, 0 * i, 0 * i, 0 * i, vx ** 2, vy ** 2, vz ** 2, vx * vz, vx * vy,
+                                  vy * vz, 0 * i, pr * pr])
+    # 7 printed digits of each value, plus the summation-order rounding of the
+    # plane sums, relative to the moment before its mean is taken off (the
+    # log-law start is uniform along the walls: its variances are round-off)
+    tol = 1e-6 * np.abs(want) + 1e-13 * np.abs(raw).max(axis=0)
+    assert np.all(np.abs(got - want) <= tol), np.max(np.abs(got - want) / tol, axis=0)
     ref.FORCING()
     ref.macrovar()
     ref.run(nsteps)
