@@ -223,6 +223,12 @@
           type(c_ptr), value :: h, pos, vel, omg, force, torque, contact, lub
         end function
 
+        ! node flags (N ints): -1 fluid, p >= 0 inside particle p
+        integer(c_int) function lbm_particles_solid(h, solid) bind(C, name="lbm_particles_solid")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: h, solid
+        end function
+
         ! learned closure (FP32/FP16 storage only)
         integer(c_int) function lbm_closure_set(h, params, ystar) bind(C, name="lbm_closure_set")
           import :: c_int, c_ptr, c_float, c_double

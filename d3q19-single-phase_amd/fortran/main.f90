@@ -25,6 +25,7 @@
       close(u)
 
       call channel_para
+      call make_dirs
       call channel_alloc
       istat = 0
       imovie = 0
@@ -130,6 +131,7 @@
       call system_clock(t1)
       time_max = dble(t1 - t0)/dble(rate)
       call lbm_check(lbm_mass_sum(lbm, mass), 'mass')
+      call probe
       if (savecntd) call savecntdflow
       if (ipart) then
         call fetch_particles

@@ -11,7 +11,8 @@
 ! block of f, para.f90:218-262), one file when both are 1.
 ! diag.dat, diag2.dat, profiles.dat, profiles2.dat and dissipation.dat
 ! follow saveload.f90:1507-1676, :1202-1342, :1348-1502 and :4644-4882; flowout/uy_profile and
-! flowout/pressCut follow outputflow (:696-714, :845-947, :1025-1120).
+! flowout/pressCut follow outputflow (:696-714, :845-947, :1025-1120);
+! probe/probeN.dat follows probe (:4059-4100).
 !=============================================================================
       subroutine saveinitflow
       use channel_vars
@@ -257,6 +258,70 @@
       close(20)
 120   format(2x, 3(1pe16.6))
       end subroutine outputpress
+
+      ! The output directories under dirgenr, as para.f90:310-327 makes them
+      ! (libc mkdir; an existing directory is fine).
+      subroutine make_dirs
+      use channel_vars
+      implicit none
+      interface
+        integer(c_int) function c_mkdir(path, mode) bind(C, name="mkdir")
+          import :: c_int, c_char
+          character(kind=c_char), intent(in) :: path(*)
+          integer(c_int), value :: mode
+        end function
+      end interface
+      character(len=8), parameter :: names(6) = [character(len=8) :: 'diag', 'stat', 'probe', 'initflow', &
+                                                 'cntdflow', 'flowout']
+      integer :: k, rc
+      do k = 1, size(names)
+        rc = c_mkdir(trim(dirgenr)//trim(names(k))//c_null_char, int(o'755', c_int))
+      end do
+      end subroutine make_dirs
+
+      ! probe, saveload.f90:4059-4100: probe/probeN.dat holds the velocity at
+      ! the centre node (lx/2, ny_r/2, nz_r/2 of its block) of every rank of
+      ! the nprocy_out x nprocz_out decomposition, one line per rank (one
+      ! line for a single rank), 0 where that node lies inside a particle.
+      subroutine probe
+      use channel_vars
+      implicit none
+      character(len=300) :: fnm
+      character(len=25) :: filenum
+      integer(c_int), allocatable, target :: solid(:)
+      integer :: id, y0, nyr, z0, nzr, jy, kz, n
+      real(8) :: up, vp, wp9
+      call fetch_fields
+      if (ipart) then
+        allocate(solid(int(lx, 8)*ly*lz))
+        call lbm_check(lbm_particles_solid(lbm, c_loc(solid)), 'particles_solid')
+      end if
+      write(filenum, '(i10)') istep
+      fnm = trim(dirgenr)//'probe/probe'//trim(adjustl(filenum))//'.dat'
+      open(60, file=trim(fnm), status='unknown', form='formatted')
+      write(60,*) 'Center probe at istep ', istep
+      write(60,*) 'Processor    x-velocity    y-velocity    z-velocity'
+      do id = 0, nprocy_out*nprocz_out - 1
+        call rank_block(id, nprocy_out, nprocz_out, y0, nyr, z0, nzr)
+        jy = y0 + nyr/2
+        kz = z0 + nzr/2
+        up = ux(lx/2, jy, kz)
+        vp = uy(lx/2, jy, kz)
+        wp9 = uz(lx/2, jy, kz)
+        if (ipart) then
+          n = lx/2 + lx*((jy - 1) + ly*(kz - 1))
+          if (solid(n) >= 0) then
+            up = 0.0d0
+            vp = 0.0d0
+            wp9 = 0.0d0
+          end if
+        end if
+        write(60, 600) id, up, vp, wp9
+      end do
+      close(60)
+      if (allocated(solid)) deallocate(solid)
+600   format(2x, 1i5, 3(1pe16.6))
+      end subroutine probe
 
       ! statistc, saveload.f90:1202-1342: plane means of u / u*, the velocity
       ! variances and shear stresses / u*^2, and the mean and variance of the

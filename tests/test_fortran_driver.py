@@ -316,6 +316,21 @@ def test_driver_restart_from_a_rank_set_bitwise(tmp_path):
          nprocz_out=4, dirgenr=f"'{d}/'")
     _run(d / "a.nml", d)
     assert len(list((d / "cntdflow").glob("endrunflow2D16x8.0000030.*"))) == 8
+    # probe (saveload.f90:4059-4100): the centre velocity of each of the 8
+    # ranks' blocks at the last step, against the oracle's fields
+    o = O.OracleChannel(lx, ly, lz)
+    o.f[:] = ref.f
+    o.FORCING()
+    o.macrovar()
+    o.run(30)
+    lines = (d / "probe" / "probe30.dat").read_text().splitlines()
+    assert "Center probe at istep" in lines[0] and lines[0].split()[-1] == "30"
+    rows = [ln.split() for ln in lines[2:] if ln.strip()]
+    assert [int(r[0]) for r in rows] == list(range(8))
+    for (myid, y0, nyr, z0, nzr), r in zip(fortio.rank_blocks(ly, lz, 2, 4), rows):
+        jy, kz = y0 + nyr // 2, z0 + nzr // 2  # 1-based centre of the block
+        want = [a[kz - 1, jy - 1, lx // 2 - 1] for a in (o.ux, o.uy, o.uz)]
+        np.testing.assert_allclose([float(v) for v in r[1:]], want, rtol=6e-7, atol=1e-300)
     _nml(d / "b.nml", lx=lx, ly=ly, lz=lz, nsteps=30, newrun=".false.", istpload=30, nprocy_in=2, nprocz_in=4,
          ndiag=0, dirgenr=f"'{d}/'")
     _run(d / "b.nml", d)
