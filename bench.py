@@ -8,8 +8,9 @@ Workload (default): turbulent channel Re_tau = 180, nu = 0.0036, MRTtype 1,
 768 streamwise x 384 spanwise x 192 wall-normal nodes (reference axes
 lx=192, ly=768, lz=384; 56.6 M nodes), log-law mean profile plus a seeded
 divergence-free random-phase perturbation of 10% amplitude, populations
-resident in HBM before timing.  `value` is FP64 MLUPS; FP32 and FP16-storage
-variants of the same workload are reported under "variants".  The timed
+resident in HBM before timing.  `value` is FP64 MLUPS; FP32, FP16-storage and
+FP16C-storage (1 sign, 4 exponent, 11 significand bits) variants of the same
+workload are reported under "variants".  The timed
 steps stay far below the ~4k steps after which MRTtype 1's linear
 instability at this centreline speed shows (the rates do not change the
 kernel's cost).
@@ -69,7 +70,7 @@ import d3q19  # noqa: E402
 from d3q19 import workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_NODE = {"fp64": 304, "fp32": 152, "fp16": 76}
+BYTES_PER_NODE = {"fp64": 304, "fp32": 152, "fp16": 76, "fp16c": 76}
 WORKLOADS = {
     # name: (lx, ly, lz, description)
     "c2": (192, 768, 384, "turbulent channel Re_tau=180, 768x384x192 (streamwise x spanwise x wall-normal)"),
@@ -394,7 +395,8 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--grid", default=None, metavar="LX,LY,LZ",
                     help="measure this grid instead of the workload's (shape studies; Re_tau = 180)")
-    ap.add_argument("--variants", default="fp32,fp16", help="comma list of extra storage precisions ('' = none)")
+    ap.add_argument("--variants", default="fp32,fp16,fp16c",
+                    help="comma list of extra storage precisions ('' = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-closure", action="store_true", help="skip the coarse closure-run throughput")
     ap.add_argument("--no-forcingp", action="store_true", help="skip the perturbation-forcing step throughput")
@@ -515,7 +517,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16 storage, f32 compute"}[main_prec],
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16 storage, f32 compute",
+                  "fp16c": "16-bit (1-4-11) storage, f32 compute"}[main_prec],
         "data": "synthetic (seeded log-law + divergence-free perturbation)",
         "config": {"workload": desc, "lx": lx, "ly": ly, "lz": lz, "nodes": lx * ly * lz,
                    "kernel_ms": round(main_m["kernel_ms"], 4),

@@ -1,5 +1,5 @@
-// D3Q19 MRT node arithmetic for gfx950, shared by the FP64 and FP32/FP16
-// kernel translation units.
+// D3Q19 MRT node arithmetic for gfx950, shared by the FP64 and
+// FP32/FP16/FP16C kernel translation units.
 //
 // mrt_collide() follows Channel-Flow/collision.f90:60-220 operation by
 // operation: every sum is evaluated left to right as written there and every
@@ -187,6 +187,9 @@ struct Prec<1> {
 template <int I>
 constexpr float fp16_scale() { return I == 0 ? 4.0f : (I <= 6 ? 16.0f : 32.0f); }
 
+// fp16_scale<I>() for a direction known only at run time.
+__host__ __device__ __forceinline__ float fp16_scale_rt(int i) { return i == 0 ? 4.0f : (i <= 6 ? 16.0f : 32.0f); }
+
 template <>
 struct Prec<2> {
     using S = _Float16;
@@ -199,7 +202,63 @@ struct Prec<2> {
     static constexpr int scale_class() { return I == 0 ? 0 : (I <= 6 ? 1 : 2); }
     template <int I>
     static __device__ __forceinline__ void st(S *p, T v) { __builtin_nontemporal_store(to_S<I>(v), p); }
+    // run-time direction (particle links, refill)
+    static __device__ __forceinline__ T to_T_rt(S v, int i)
+    {
+        return (float)v * (i == 0 ? 0.25f : (i <= 6 ? 1.0f / 16.0f : 1.0f / 32.0f));
+    }
+    static __device__ __forceinline__ S to_S_rt(T v, int i) { return (S)(v * fp16_scale_rt(i)); }
 };
+
+// FP16C storage (P = 3): the same scaled values as FP16 storage (f_i 2^k) in
+// a 16-bit floating-point format with 1 sign, 4 exponent and 11 significand
+// bits ("FP16C" of Lehmann et al., Phys. Rev. E 106, 015308, 2022).  Normal
+// numbers span [2^-13, 4 - 2^-10] with one significand bit more than IEEE
+// FP16; below 2^-13 they continue as denormals down to 2^-24, as FP16's do.
+// The conversions go through FP32's own denormal range: a value times
+// 2^-113 carries the format's exponent in its FP32 exponent field, so the
+// 16-bit word is the FP32 bits shifted right by 12, rounded to nearest even,
+// and decoding shifts them back and multiplies by 2^113 (both exact).
+// Magnitudes above the largest value, and NaN, saturate to it (sign kept).
+constexpr float FP16C_DOWN = 9.629649721936179e-35f; // 2^-113
+constexpr float FP16C_UP = 1.0384593717069655e34f;   // 2^113
+constexpr uint32_t FP16C_MAXBITS = 0x07FFF000u;      // FP32 bits of the largest value times 2^-113
+
+// 16-bit word of y, a value already multiplied by 2^-113.
+__host__ __device__ __forceinline__ uint16_t fp16c_pack(float y)
+{
+    const uint32_t bits = __builtin_bit_cast(uint32_t, y);
+    uint32_t a = bits & 0x7fffffffu;
+    a = a < FP16C_MAXBITS ? a : FP16C_MAXBITS;
+    const uint32_t r = (a + 0x7ffu + ((a >> 12) & 1u)) >> 12;
+    return (uint16_t)(r | ((bits >> 16) & 0x8000u));
+}
+
+// The value of a 16-bit word times 2^-113 (multiply by FP16C_UP for it).
+__host__ __device__ __forceinline__ float fp16c_unpack(uint16_t h)
+{
+    const uint32_t w = h;
+    return __builtin_bit_cast(float, ((w & 0x7fffu) << 12) | ((w & 0x8000u) << 16));
+}
+
+template <>
+struct Prec<3> {
+    using S = uint16_t;
+    using T = float;
+    template <int I>
+    static __device__ __forceinline__ T to_T(S v) { return fp16c_unpack(v) * (FP16C_UP / fp16_scale<I>()); }
+    template <int I>
+    static __device__ __forceinline__ S to_S(T v) { return fp16c_pack(v * (fp16_scale<I>() * FP16C_DOWN)); }
+    template <int I>
+    static constexpr int scale_class() { return I == 0 ? 0 : (I <= 6 ? 1 : 2); }
+    template <int I>
+    static __device__ __forceinline__ void st(S *p, T v) { __builtin_nontemporal_store(to_S<I>(v), p); }
+    static __device__ __forceinline__ T to_T_rt(S v, int i) { return fp16c_unpack(v) * (FP16C_UP / fp16_scale_rt(i)); }
+    static __device__ __forceinline__ S to_S_rt(T v, int i) { return fp16c_pack(v * (fp16_scale_rt(i) * FP16C_DOWN)); }
+};
+
+// 2-byte storage (FP16, FP16C).
+constexpr bool half_storage(int P) { return P == 2 || P == 3; }
 
 // Lattice velocity component times a value: c*a for c in {-1,0,1}.
 template <int C, typename T>

@@ -323,5 +323,6 @@ struct Launch {
 extern template struct Launch<0>;
 extern template struct Launch<1>;
 extern template struct Launch<2>;
+extern template struct Launch<3>;
 
 } // namespace d3q19

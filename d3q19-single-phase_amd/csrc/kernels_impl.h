@@ -34,14 +34,14 @@ template <int P, int MODE>
 constexpr int step_block() { return ((MODE & 8) != 0 || P == 1) ? 256 : 512; }
 
 // Element `node` of a population row, addressed as the row base plus a 32-bit
-// byte offset when that cannot wrap: FP16 storage, whose rows hold fewer than
+// byte offset when that cannot wrap: 2-byte storage, whose rows hold fewer than
 // 2^31 nodes (lbm_create's limit), i.e. fewer than 2^32 bytes.  A wave-uniform
 // row base then stays in scalar registers and the load or store takes the
 // lane's offset directly, with no 64-bit address arithmetic per population.
 template <int P>
 __device__ __forceinline__ typename Prec<P>::S *elem(typename Prec<P>::S *row, uint32_t node)
 {
-    if constexpr (P == 2)
+    if constexpr (half_storage(P))
         return (typename Prec<P>::S *)((char *)row + node * (uint32_t)sizeof(typename Prec<P>::S));
     else
         return row + node;
@@ -1180,7 +1180,7 @@ __global__ void __launch_bounds__(BLOCK) k_diss_partials(DissArgs a)
 template <int P>
 static inline bool use_runs(const StepArgs &a, int mode)
 {
-    if constexpr (P == 2) {
+    if constexpr (half_storage(P)) {
         return false;
     } else {
         return a.runs && (mode == 0 || mode == 1) && a.g.lx % Vec<P>::W == 0 && a.g.N / Vec<P>::W < 0xffffffffull;
@@ -1190,7 +1190,7 @@ static inline bool use_runs(const StepArgs &a, int mode)
 template <int P>
 hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
 {
-    if constexpr (P != 2) {
+    if constexpr (!half_storage(P)) {
         if (use_runs<P>(a, mode)) {
             const dim3 grid(nblocks<P>(a.g.N / Vec<P>::W)), blk(BLOCK);
             if (mode == 0)

@@ -176,7 +176,8 @@ int dispatch_err(hipError_t e, const char *what)
 #define PREC_CALL(h, fn, ...)                                                                   \
     ((h)->prec == LBM_FP64   ? Launch<0>::fn(__VA_ARGS__)                                       \
      : (h)->prec == LBM_FP32 ? Launch<1>::fn(__VA_ARGS__)                                       \
-                             : Launch<2>::fn(__VA_ARGS__))
+     : (h)->prec == LBM_FP16 ? Launch<2>::fn(__VA_ARGS__)                                       \
+                             : Launch<3>::fn(__VA_ARGS__))
 
 RatesD rates_of(const lbm_channel *h)
 {
@@ -481,7 +482,7 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
     if (N >= (1ull << 31))
         return fail(LBM_ERR_ARG, "lbm_create: %llu nodes exceed the 2^31 per-device limit of 32-bit node indices",
                     (unsigned long long)N);
-    if (c.precision < LBM_FP64 || c.precision > LBM_FP16)
+    if (c.precision < LBM_FP64 || c.precision > LBM_FP16C)
         return fail(LBM_ERR_ARG, "lbm_create: unknown precision %d", c.precision);
     if (c.reserved != 0)
         return fail(LBM_ERR_ARG, "lbm_create: reserved field must be 0");
@@ -548,10 +549,10 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         // both FP64 (y: 19 770 MLUPS against 19 110 for z slabs, 5 to a
         // chunk) and FP32 (z: 39 880 against 39 460 with 12 to a chunk).
         // LBM_BLOCK_AXIS=z|y sets the axis, LBM_BLOCK_PLANES=K the chunk (0:
-        // off).  Not by default for FP16 storage, whose step is bound by
-        // arithmetic (measured 60 800 MLUPS blocked against 68 100
+        // off).  Not by default for 2-byte storage, whose step is bound by
+        // arithmetic (FP16: measured 60 800 MLUPS blocked against 68 100
         // single-step).
-        const BlockChoice bc = block_choice(h->g, h->elem, c.precision != LBM_FP16, BLOCK_RING_BYTES,
+        const BlockChoice bc = block_choice(h->g, h->elem, h->elem != 2, BLOCK_RING_BYTES,
                                             BLOCK_MIN_SLABS, BLOCK_PREF_SLABS, getenv("LBM_BLOCK_AXIS"),
                                             getenv("LBM_BLOCK_PLANES"));
         if (bc.bad) {
@@ -1694,7 +1695,7 @@ static int download_post_range(lbm_channel *h, uint64_t n0, uint64_t count, doub
         HIP_TRY(hipStreamSynchronize(h->stream));
         return LBM_OK;
     }
-    // FP32/FP16: widen on the host, population by population
+    // FP32/FP16/FP16C: widen on the host, population by population
     std::vector<unsigned char> buf(count * h->elem);
     for (int i = 0; i < 19; i++) {
         HIP_TRY(hipMemcpyAsync(buf.data(), L + ((size_t)i * N + n0) * h->elem, count * h->elem,
@@ -1705,11 +1706,16 @@ static int download_post_range(lbm_channel *h, uint64_t n0, uint64_t count, doub
             const float *v = (const float *)buf.data();
             for (uint64_t n = 0; n < count; n++)
                 o[n] = v[n];
-        } else {
+        } else if (h->prec == LBM_FP16) {
             const _Float16 *v = (const _Float16 *)buf.data();
             const double sc = i == 0 ? 0.25 : (i <= 6 ? 1.0 / 16.0 : 1.0 / 32.0);
             for (uint64_t n = 0; n < count; n++)
                 o[n] = (double)(float)v[n] * sc;
+        } else {
+            const uint16_t *v = (const uint16_t *)buf.data();
+            const float sc = FP16C_UP / fp16_scale_rt(i);
+            for (uint64_t n = 0; n < count; n++)
+                o[n] = (double)(fp16c_unpack(v[n]) * sc);
         }
     }
     return LBM_OK;
@@ -1895,7 +1901,8 @@ int lbm_beads_filling(lbm_channel *h)
     Part *pt = h->part;
     hipError_t e = h->prec == LBM_FP64   ? PartLaunch<0>::refill(pt->pg, pt->ps, h->lat[h->cur], h->stream)
                    : h->prec == LBM_FP32 ? PartLaunch<1>::refill(pt->pg, pt->ps, h->lat[h->cur], h->stream)
-                                         : PartLaunch<2>::refill(pt->pg, pt->ps, h->lat[h->cur], h->stream);
+                   : h->prec == LBM_FP16 ? PartLaunch<2>::refill(pt->pg, pt->ps, h->lat[h->cur], h->stream)
+                                         : PartLaunch<3>::refill(pt->pg, pt->ps, h->lat[h->cur], h->stream);
     return dispatch_err(e, "beads_filling");
 }
 
@@ -1905,7 +1912,8 @@ int lbm_beads_collision(lbm_channel *h)
     Part *pt = h->part;
     hipError_t e = h->prec == LBM_FP64   ? PartLaunch<0>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream)
                    : h->prec == LBM_FP32 ? PartLaunch<1>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream)
-                                         : PartLaunch<2>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream);
+                   : h->prec == LBM_FP16 ? PartLaunch<2>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream)
+                                         : PartLaunch<3>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream);
     int rc = dispatch_err(e, "beads_collision");
     if (rc)
         return rc;
@@ -2200,7 +2208,8 @@ int lbm_trainer_add_filtered(lbm_trainer *t, lbm_channel *fine, double nu_coarse
     fa.out = fields;
     hipError_t e = fine->prec == LBM_FP64   ? MlpLaunch<0>::filter(fa, t->stream)
                    : fine->prec == LBM_FP32 ? MlpLaunch<1>::filter(fa, t->stream)
-                                            : MlpLaunch<2>::filter(fa, t->stream);
+                   : fine->prec == LBM_FP16 ? MlpLaunch<2>::filter(fa, t->stream)
+                                            : MlpLaunch<3>::filter(fa, t->stream);
     FeatArgs ft;
     ft.lx = lxc;
     ft.ly = lyc;

@@ -353,5 +353,6 @@ hipError_t mlp_predict(const float *w, const float *feat, uint32_t first, uint32
 template struct MlpLaunch<0>;
 template struct MlpLaunch<1>;
 template struct MlpLaunch<2>;
+template struct MlpLaunch<3>;
 
 } // namespace d3q19

@@ -52,5 +52,6 @@ hipError_t mlp_predict(const float *w, const float *feat, uint32_t first, uint32
 extern template struct MlpLaunch<0>;
 extern template struct MlpLaunch<1>;
 extern template struct MlpLaunch<2>;
+extern template struct MlpLaunch<3>;
 
 } // namespace d3q19

@@ -135,5 +135,6 @@ hipError_t part_move(const PartGeom &pg, const PartState &ps, hipStream_t s);
 extern template struct PartLaunch<0>;
 extern template struct PartLaunch<1>;
 extern template struct PartLaunch<2>;
+extern template struct PartLaunch<3>;
 
 } // namespace d3q19

@@ -243,13 +243,14 @@ __global__ void __launch_bounds__(PBLOCK) k_p_linkq(PartGeom pg, PartState ps)
     }
 }
 
-// Storage access with a run-time direction (FP16 scale by direction class).
+// Storage access with a run-time direction (2-byte storage scales by
+// direction class).
 template <int P>
 __device__ inline double ld_rt(const typename Prec<P>::S *L, uint64_t N, int i, uint32_t n)
 {
     const typename Prec<P>::S v = L[(uint64_t)i * N + n];
-    if constexpr (P == 2)
-        return (double)((float)v * (i == 0 ? 0.25f : (i <= 6 ? 1.0f / 16.0f : 1.0f / 32.0f)));
+    if constexpr (half_storage(P))
+        return (double)Prec<P>::to_T_rt(v, i);
     else
         return (double)v;
 }
@@ -258,8 +259,8 @@ template <int P>
 __device__ inline void st_rt(typename Prec<P>::S *L, uint64_t N, int i, uint32_t n, double v)
 {
     using S = typename Prec<P>::S;
-    if constexpr (P == 2)
-        L[(uint64_t)i * N + n] = (S)((float)v * (i == 0 ? 4.0f : (i <= 6 ? 16.0f : 32.0f)));
+    if constexpr (half_storage(P))
+        L[(uint64_t)i * N + n] = Prec<P>::to_S_rt((float)v, i);
     else
         L[(uint64_t)i * N + n] = (S)v;
 }
@@ -710,5 +711,6 @@ hipError_t PartLaunch<P>::refill(const PartGeom &pg, const PartState &ps, void *
 template struct PartLaunch<0>;
 template struct PartLaunch<1>;
 template struct PartLaunch<2>;
+template struct PartLaunch<3>;
 
 } // namespace d3q19

@@ -21,9 +21,9 @@ from .params import para, loglaw_profile, CIX, CIY, CIZ, IPOPP  # noqa: F401
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblbm.so")
 
-FP64, FP32, FP16 = 0, 1, 2
-_PREC = {"fp64": FP64, "f64": FP64, "fp32": FP32, "f32": FP32, "fp16": FP16, "f16": FP16}
-STORAGE_BYTES = {FP64: 8, FP32: 4, FP16: 2}
+FP64, FP32, FP16, FP16C = 0, 1, 2, 3
+_PREC = {"fp64": FP64, "f64": FP64, "fp32": FP32, "f32": FP32, "fp16": FP16, "f16": FP16, "fp16c": FP16C}
+STORAGE_BYTES = {FP64: 8, FP32: 4, FP16: 2, FP16C: 2}
 
 
 class LbmError(RuntimeError):

@@ -9,7 +9,7 @@
       implicit none
 
       integer(c_int), parameter :: LBM_OK = 0
-      integer(c_int), parameter :: LBM_FP64 = 0, LBM_FP32 = 1, LBM_FP16 = 2
+      integer(c_int), parameter :: LBM_FP64 = 0, LBM_FP32 = 1, LBM_FP16 = 2, LBM_FP16C = 3
 
       ! struct lbm_config
       type, bind(C) :: lbm_config

@@ -35,12 +35,17 @@ extern "C" {
 
 /* Storage precision of the populations.  The reference's f is already the
  * shifted population f - w_i*rho0 (Channel-Flow/initial.f90:29-44: rho holds
- * the density fluctuation only); FP32/FP16 keep that same quantity at lower
- * precision.  LBM_FP64 computes in FP64; LBM_FP32 and LBM_FP16 compute in FP32
- * (FP16 is storage only, with per-direction power-of-two scaling). */
+ * the density fluctuation only); FP32/FP16/FP16C keep that same quantity at
+ * lower precision.  LBM_FP64 computes in FP64; the others compute in FP32.
+ * LBM_FP16 stores IEEE half values with per-direction power-of-two scaling;
+ * LBM_FP16C stores the same scaled values in a 16-bit format with 1 sign, 4
+ * exponent and 11 significand bits (normal range [2^-13, 4 - 2^-10],
+ * denormals to 2^-24, saturating above; Lehmann et al., Phys. Rev. E 106,
+ * 015308, 2022): one significand bit more than FP16, for the same traffic. */
 #define LBM_FP64 0
 #define LBM_FP32 1
 #define LBM_FP16 2
+#define LBM_FP16C 3
 
 /* Run-time constants.  Field names and meanings follow
  * Channel-Flow/var_inc.f90:85-93 as filled by Channel-Flow/para.f90:106-174;
@@ -49,7 +54,7 @@ extern "C" {
  * ww*) must equal para.f90's values: lbm_create rejects others. */
 typedef struct lbm_config {
     int lx, ly, lz;     /* node counts: var_inc.f90:51-52 (lx), para.f90:234-245 (ly, lz) */
-    int precision;      /* LBM_FP64 / LBM_FP32 / LBM_FP16 */
+    int precision;      /* LBM_FP64 / LBM_FP32 / LBM_FP16 / LBM_FP16C */
     int device;         /* HIP device ordinal */
     int reserved;       /* must be 0 */
     double s1, s2, s4, s9, s10, s13, s16;             /* para.f90:106-141 */

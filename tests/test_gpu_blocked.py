@@ -52,7 +52,7 @@ def test_blocked_sweep_fp64_is_bitwise_single_steps_and_oracle(monkeypatch, shap
 
 
 @pytest.mark.parametrize("axis", ["z", "y"])
-@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp16c"])
 def test_blocked_sweep_reduced_precision_matches_single_steps(monkeypatch, prec, axis):
     lx, ly, lz = 24, 20, 18
     _, (blk, one) = _pair(monkeypatch, lx, ly, lz, prec, 4, seed=52, axis=axis)
@@ -83,7 +83,8 @@ def test_blocked_sweep_then_other_main_loop_calls(monkeypatch):
         assert blk.mass() == one.mass()
 
 
-@pytest.mark.parametrize("prec,want", [("fp64", ("y", 11)), ("fp32", ("z", 11)), ("fp16", ("z", 0))])
+@pytest.mark.parametrize("prec,want", [("fp64", ("y", 11)), ("fp32", ("z", 11)), ("fp16", ("z", 0)),
+                                       ("fp16c", ("z", 0))])
 def test_default_sweep_on_the_turbulent_channel(monkeypatch, prec, want):
     """lbm_create's choice for the 192x768x384 channel: 11.2 MB slabs, eleven
     to a chunk (a 146 MB ring), along y in FP64 and z in FP32; FP16 storage

@@ -75,7 +75,7 @@ def _params(scales, nu, seed=3):
     return p
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp16c"])
 def test_fused_closure_inputs_are_the_oracle_nonequilibrium_moments(prec):
     lx, ly, lz = 12, 10, 7  # 840 nodes: the last block is partly past the lattice
     o, f0 = _state(lx, ly, lz, 41)

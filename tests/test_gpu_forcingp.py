@@ -96,7 +96,7 @@ def test_forcingp_statistics_and_avedensity():
         assert np.max(np.abs(f - ref.f)) < 1e-17 + 1e-12 * abs(m_ref)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3)])
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3), ("fp16c", 2e-3)])
 def test_forcingp_reduced_precision_tracks_fp64(prec, tol):
     lx, ly, lz = 32, 48, 24
     ref, _, _ = seeded_pair(lx, ly, lz, seed=34, noise=0.0, prec="fp64")
@@ -127,7 +127,7 @@ def test_forcingp_not_combined_with_particles():
             ch.particles_create(np.array([[16.0, 12.0, 12.0]]), rad=4.0)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16", "fp16c"])
 @pytest.mark.parametrize("axis,forced", [("z", True), ("y", True), ("y", False)])
 def test_batched_forcingp_loop_equals_call_by_call(monkeypatch, prec, axis, forced):
     """lbm_run_forcingp against FORCINGP / FORCING, collision_MRT and macrovar

@@ -276,7 +276,7 @@ def test_vector_run_kernel_equals_one_node_kernel(prec, shape, monkeypatch):
         assert np.max(np.abs(outs[1] - outs[0])) < 1e-6 * scale
 
 
-@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3)])
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3), ("fp16c", 2e-3)])
 def test_reduced_precision_tracks_fp64(prec, tol):
     lx, ly, lz = 32, 64, 32
     vel = workloads.laminar_parabola_noise(lx, ly, lz, seed=1)
@@ -295,6 +295,49 @@ def test_reduced_precision_tracks_fp64(prec, tol):
         pa = uy_a.mean(axis=(0, 1))
         pb = uy_b.mean(axis=(0, 1))
         assert np.max(np.abs(pb - pa)) / np.max(np.abs(pa)) < tol
+
+
+def test_fp16c_storage_is_closer_to_fp64_than_fp16():
+    """The 16-bit format with one more significand bit (LBM_FP16C) tracks the
+    FP64 lattice more closely than IEEE FP16 storage of the same scaled
+    values, over the same 200 steps."""
+    lx, ly, lz = 32, 64, 32
+    vel = workloads.laminar_parabola_noise(lx, ly, lz, seed=1)
+    err = {}
+    with d3q19.Channel(lx, ly, lz, "fp64") as a:
+        for c in (a,):
+            c.set_macros(ux=vel[0], uy=vel[1], uz=vel[2])
+            c.FORCING()
+            c.initpop()
+            c.macrovar()
+            c.run(200)
+        ref = a.download_post()
+    for prec in ("fp16", "fp16c"):
+        with d3q19.Channel(lx, ly, lz, prec) as b:
+            b.set_macros(ux=vel[0], uy=vel[1], uz=vel[2])
+            b.FORCING()
+            b.initpop()
+            b.macrovar()
+            b.run(200)
+            err[prec] = np.max(np.abs(b.download_post() - ref)) / np.max(np.abs(ref))
+    assert err["fp16c"] < 0.7 * err["fp16"], err
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp16c"])
+def test_host_widening_equals_device_gather(prec):
+    """lbm_download_post_range widens the stored values on the host; the
+    device gather of lbm_download_post widens them on the GPU: same values."""
+    lx, ly, lz = 16, 12, 10
+    vel = workloads.laminar_parabola_noise(lx, ly, lz, seed=2, amp=0.3)
+    with d3q19.Channel(lx, ly, lz, prec) as c:
+        c.set_macros(ux=vel[0], uy=vel[1], uz=vel[2])
+        c.FORCING()
+        c.initpop()
+        c.macrovar()
+        c.run(7)
+        full = c.download_post()
+        part = c.download_post_range(lx * ly * 3, lx * ly * 4)
+    assert np.array_equal(part, full[:, lx * ly * 3:lx * ly * 7])
 
 
 def test_rhoupdat_reports_a_nan():

@@ -114,7 +114,7 @@ def test_random_call_sequences_match_undeferred_calls(monkeypatch, seed):
         assert np.array_equal(bits(a.download_post()), bits(b.download_post()))
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp16c"])
 def test_per_call_loop_reduced_precision(monkeypatch, prec):
     """FP32 pairs (same node arithmetic as single steps) and FP16 storage
     (no default sweep: nothing deferred) equal the undeferred calls."""

@@ -60,3 +60,20 @@ def test_blocked_two_step_sweep_schedule():
         assert r.returncode == 0, r.stderr + r.stdout
         assert "ok sweep choice" in r.stdout
         assert "ok blocked sweep" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
+                    reason="hipcc not available")
+def test_fp16c_storage_format():
+    """tests/host/fp16c_check.hip: every FP16C word decodes to the format's
+    value and encodes back; words are ordered; floats encode to the nearest
+    word (brute force, ties to even), saturating above the largest."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "fp16c_check")
+        subprocess.run([hipcc, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "d3q19-single-phase_amd", "csrc"),
+                        os.path.join(ROOT, "tests", "host", "fp16c_check.hip"), "-o", exe],
+                       check=True, capture_output=True)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr + r.stdout
+        assert "ok fp16c" in r.stdout
