@@ -114,23 +114,25 @@ inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const Step
     return out;
 }
 
-// Ring of the two-step blocked sweep: about 150 MB of the 256 MB last-level
-// cache (13 slots of 11.2 MB measured best on the 192x768x384 channel in
-// FP64 and FP32; larger rings are partly evicted by the streams through the
-// same cache)
-constexpr double BLOCK_RING_BYTES = 150e6;
+// Ring of the two-step blocked sweep: about 225 MB of the 256 MB last-level
+// cache.  The step-t launches read the lattice with non-temporal loads, so
+// that stream does not evict the ring; on the 192x768x384 channel rings of
+// 202-269 MB then measured best in FP64 and FP32 (18-24 slots of 11.2 MB), and
+// 291 MB lost 12 %.
+constexpr double BLOCK_RING_BYTES = 225e6;
 constexpr long BLOCK_MIN_SLABS = 4;
-constexpr long BLOCK_PREF_SLABS = 8;
+constexpr long BLOCK_MAX_SLABS = 32;
 
 // The sweep lbm_create chooses for a lattice of elem-byte values: axis 2 (z)
-// or 1 (y) and K slabs per chunk (0: single steps only).  K is the number of
-// slabs whose ring of K + 2 fits ring_bytes (at most 16 and the number of
-// slabs).  The axis with the larger slabs is taken when its chunk holds at
-// least pref_slabs, else the other one on that condition, else whichever
-// holds at least min_slabs, else none.  axis_env / planes_env (may be null)
-// are the values of LBM_BLOCK_AXIS ("z" or "y") and LBM_BLOCK_PLANES (K, 0
-// turns the sweep off), which override the choice; any other value of either
-// is reported in `bad`.  blocked = false (FP16 storage, whose step is bound by
+// or 1 (y) and K slabs per chunk (0: single steps only).  Along each axis K is
+// the number of slabs whose ring of K + 2 fits ring_bytes, at most
+// BLOCK_MAX_SLABS and the number of slabs.  The axis whose chunk holds more
+// nodes is taken (the larger slabs on a tie): the two extra ring slots are
+// then the smallest share of the ring.  A chunk of fewer than min_slabs
+// turns the sweep off.  axis_env / planes_env (may be null) are the values of
+// LBM_BLOCK_AXIS ("z" or "y") and LBM_BLOCK_PLANES (K, 0 turns the sweep
+// off), which override the choice; any other value of either is reported in
+// `bad`.  blocked = false (2-byte storage, whose step is bound by
 // arithmetic) makes the default K zero.
 struct BlockChoice {
     int axis;
@@ -139,18 +141,20 @@ struct BlockChoice {
 };
 
 inline BlockChoice block_choice(const Geom &g, uint32_t elem, bool blocked, double ring_bytes, long min_slabs,
-                                long pref_slabs, const char *axis_env, const char *planes_env)
+                                const char *axis_env, const char *planes_env)
 {
     auto chunk = [&](int axis) -> long {
         const BlockGeom bg = block_geom(g, axis);
         const double slab_bytes = (double)g.lx * bg.nr * 19.0 * (double)elem;
-        return (long)(ring_bytes / slab_bytes) - 2;
+        const long k = (long)(ring_bytes / slab_bytes) - 2;
+        return std::max<long>(0, std::min<long>(std::min<long>(k, BLOCK_MAX_SLABS), (long)bg.ns));
+    };
+    auto chunk_nodes = [&](int axis) -> uint64_t {
+        return (uint64_t)chunk(axis) * g.lx * block_geom(g, axis).nr;
     };
     // z slabs (lx * ly nodes) are the larger ones when ly >= lz
     const int big = g.ly >= g.lz ? 2 : 1, small = 3 - big;
-    int axis = big;
-    if (chunk(big) < pref_slabs)
-        axis = chunk(small) >= pref_slabs ? small : (chunk(big) >= min_slabs ? big : small);
+    int axis = chunk_nodes(small) > chunk_nodes(big) ? small : big;
     if (axis_env) {
         if (strcmp(axis_env, "y") == 0)
             axis = 1;
@@ -170,7 +174,7 @@ inline BlockChoice block_choice(const Geom &g, uint32_t elem, bool blocked, doub
         kb = v;
     }
     const BlockGeom bg = block_geom(g, axis);
-    kb = std::min<long>(std::min<long>(kb, 16), (long)bg.ns);
+    kb = std::min<long>(std::min<long>(kb, BLOCK_MAX_SLABS), (long)bg.ns);
     if (kb < 1 || (uint64_t)slab_padded(g.lx * bg.nr) * (uint64_t)kb >= (1ull << 31))
         kb = 0;
     return BlockChoice{axis, (uint32_t)kb, nullptr};

@@ -541,20 +541,18 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         h->runs = (k && strcmp(k, "runs") == 0) ? 1 : 0;
         // two-step blocked sweep (blocked.h, block_choice): chunks of K slabs
         // whose ring of K + 2 slabs stays within BLOCK_RING_BYTES, along the
-        // axis with the larger slabs when a chunk of those holds at least
-        // BLOCK_PREF_SLABS, else the other; below BLOCK_MIN_SLABS the sweep
+        // axis whose chunk holds more nodes; below BLOCK_MIN_SLABS the sweep
         // is off (it measured slower than single steps: the 2048x1024x512
-        // channel's 40 MB y slabs allow two, 33 900 against 38 200 MLUPS).  On the
-        // 192x768x384 channel this picks 11.2 MB slabs, 11 to a chunk, in
-        // both FP64 (y: 19 770 MLUPS against 19 110 for z slabs, 5 to a
-        // chunk) and FP32 (z: 39 880 against 39 460 with 12 to a chunk).
+        // channel's 40 MB y slabs allow two, 33 900 against 38 200 MLUPS).  On
+        // the 192x768x384 channel this picks 11.2 MB slabs, 18 to a chunk, in
+        // both FP64 (y: 20 650-20 700 MLUPS against 20 000-20 090 for z
+        // slabs, 8 to a chunk) and FP32 (z: about 42 000).
         // LBM_BLOCK_AXIS=z|y sets the axis, LBM_BLOCK_PLANES=K the chunk (0:
         // off).  Not by default for 2-byte storage, whose step is bound by
-        // arithmetic (FP16: measured 60 800 MLUPS blocked against 68 100
-        // single-step).
-        const BlockChoice bc = block_choice(h->g, h->elem, h->elem != 2, BLOCK_RING_BYTES,
-                                            BLOCK_MIN_SLABS, BLOCK_PREF_SLABS, getenv("LBM_BLOCK_AXIS"),
-                                            getenv("LBM_BLOCK_PLANES"));
+        // arithmetic (FP16: measured 66 300 MLUPS blocked, 32 to a chunk,
+        // against 72 000 single-step).
+        const BlockChoice bc = block_choice(h->g, h->elem, h->elem != 2, BLOCK_RING_BYTES, BLOCK_MIN_SLABS,
+                                            getenv("LBM_BLOCK_AXIS"), getenv("LBM_BLOCK_PLANES"));
         if (bc.bad) {
             delete h;
             return fail(LBM_ERR_ARG, "lbm_create: invalid value of %s", bc.bad);

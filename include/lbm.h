@@ -176,8 +176,8 @@ int lbm_avedensity(lbm_channel *h, double *rhomean);
  * HBM carries one lattice read and write per two steps.  The results equal
  * the single-step path's bit for bit.  The environment variables
  * LBM_BLOCK_AXIS=z|y and LBM_BLOCK_PLANES (read by lbm_create) set the axis
- * and the slabs per chunk, 0 turns the sweep off; lbm_block_info reports the
- * choice. */
+ * and the slabs per chunk (at most 32), 0 turns the sweep off; lbm_block_info
+ * reports the choice. */
 int lbm_run(lbm_channel *h, int nsteps);
 
 /* Sum of all populations in FP64 (mass monitor; the sum is invariant under

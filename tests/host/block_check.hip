@@ -218,8 +218,7 @@ static bool check_choice(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t elem, b
                          const char *planes_env, int want_axis, uint32_t want_K)
 {
     const Geom g = geom(lx, ly, lz);
-    const BlockChoice c = block_choice(g, elem, blocked, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, BLOCK_PREF_SLABS,
-                                       axis_env, planes_env);
+    const BlockChoice c = block_choice(g, elem, blocked, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, axis_env, planes_env);
     if (c.axis != want_axis || c.K != want_K) {
         fprintf(stderr, "%ux%ux%u elem %u: sweep axis %d K %u, want axis %d K %u\n", lx, ly, lz, elem, c.axis, c.K,
                 want_axis, want_K);
@@ -230,31 +229,40 @@ static bool check_choice(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t elem, b
 
 int main()
 {
-    // 192x768x384: y slabs, 11 to a chunk, in FP64; z slabs, 11, in FP32;
-    // FP16 storage single steps; the environment overrides
-    bool choice = check_choice(192, 768, 384, 8, true, nullptr, nullptr, 1, 11) &&
-                  check_choice(192, 768, 384, 4, true, nullptr, nullptr, 2, 11) &&
+    // 192x768x384: 11.2 MB slabs, 18 to a chunk (224 MB rings): y slabs in
+    // FP64, z slabs in FP32; FP16 storage single steps; the environment
+    // overrides (z slabs of 22.4 MB in FP64: 8 to a chunk)
+    bool choice = check_choice(192, 768, 384, 8, true, nullptr, nullptr, 1, 18) &&
+                  check_choice(192, 768, 384, 4, true, nullptr, nullptr, 2, 18) &&
                   check_choice(192, 768, 384, 2, false, nullptr, nullptr, 2, 0) &&
-                  check_choice(192, 768, 384, 8, true, "z", nullptr, 2, 4) &&
+                  check_choice(192, 768, 384, 8, true, "z", nullptr, 2, 8) &&
                   check_choice(192, 768, 384, 8, true, nullptr, "0", 1, 0) &&
                   check_choice(192, 768, 384, 2, false, "y", "12", 1, 12) &&
-                  // 2048x1024x512 in FP32: 80 MB z and 40 MB y slabs, no sweep
+                  check_choice(192, 768, 384, 8, true, nullptr, "40", 1, 32) &&
+                  // 2048x1024x512 in FP32: 80 MB z and 40 MB y slabs (three
+                  // to a ring), no sweep
                   check_choice(512, 2048, 1024, 4, true, nullptr, nullptr, 1, 0) &&
-                  // small grids: the larger slabs, at most 16 and at most the slab count
-                  check_choice(32, 64, 32, 8, true, nullptr, nullptr, 2, 16) &&
+                  // 96x384x192 in FP32 (2.8 MB z slabs, 1.4 MB y slabs): 32
+                  // z slabs hold more nodes than 32 y slabs
+                  check_choice(96, 384, 192, 4, true, nullptr, nullptr, 2, 32) &&
+                  // small grids: at most 32 slabs and at most the slab count;
+                  // equal chunks take the larger slabs
+                  check_choice(32, 64, 32, 8, true, nullptr, nullptr, 2, 32) &&
                   check_choice(12, 10, 6, 8, true, nullptr, nullptr, 2, 6) &&
-                  check_choice(12, 6, 10, 8, true, nullptr, nullptr, 1, 6);
+                  check_choice(12, 6, 10, 8, true, nullptr, nullptr, 1, 6) &&
+                  check_choice(8, 80, 50, 8, true, nullptr, nullptr, 2, 32) &&
+                  check_choice(12, 6, 40, 8, true, nullptr, nullptr, 1, 6);
     if (!choice)
         return 1;
     // invalid environment values are reported, not taken as a default
     const Geom g0 = geom(192, 768, 384);
     for (const char *bad_axis : {"Y", "x", "", "yz"})
-        if (!block_choice(g0, 8, true, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, BLOCK_PREF_SLABS, bad_axis, nullptr).bad) {
+        if (!block_choice(g0, 8, true, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, bad_axis, nullptr).bad) {
             fprintf(stderr, "LBM_BLOCK_AXIS=\"%s\" accepted\n", bad_axis);
             return 1;
         }
     for (const char *bad_k : {"", "abc", "4x", "-1", " "})
-        if (!block_choice(g0, 8, true, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, BLOCK_PREF_SLABS, nullptr, bad_k).bad) {
+        if (!block_choice(g0, 8, true, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, nullptr, bad_k).bad) {
             fprintf(stderr, "LBM_BLOCK_PLANES=\"%s\" accepted\n", bad_k);
             return 1;
         }
@@ -262,7 +270,8 @@ int main()
 
     const uint32_t cases[][4] = {{4, 3, 1, 2},  {4, 3, 2, 2},  {5, 4, 3, 2},  {6, 5, 7, 2},   {5, 3, 9, 4},
                                  {4, 4, 8, 4},  {3, 5, 10, 3}, {8, 6, 16, 1}, {8, 6, 17, 16}, {7, 5, 33, 8},
-                                 {2, 1, 5, 2},  {6, 2, 4, 1},  {9, 7, 12, 5}, {4, 3, 3, 16}};
+                                 {2, 1, 5, 2},  {6, 2, 4, 1},  {9, 7, 12, 5}, {4, 3, 3, 16},
+                                 {4, 37, 41, 18}, {3, 70, 4, 32}, {2, 5, 75, 32}};
     bool ok = true;
     for (auto &k : cases)
         for (int axis : {2, 1})

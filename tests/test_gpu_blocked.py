@@ -33,7 +33,8 @@ def _pair(monkeypatch, lx, ly, lz, prec, planes, seed, axis="z"):
 @pytest.mark.parametrize("axis", ["z", "y"])
 @pytest.mark.parametrize("shape,planes", [
     ((8, 6, 5), 2), ((12, 10, 9), 4), ((16, 12, 17), 16), ((6, 5, 1), 2), ((6, 5, 2), 2),
-    ((10, 8, 7), 1), ((9, 7, 12), 5), ((32, 64, 32), 4), ((6, 1, 5), 2),
+    ((10, 8, 7), 1), ((9, 7, 12), 5), ((32, 64, 32), 4), ((6, 1, 5), 2), ((8, 41, 45), 18),
+    ((6, 70, 66), 32),
 ])
 def test_blocked_sweep_fp64_is_bitwise_single_steps_and_oracle(monkeypatch, shape, planes, axis):
     lx, ly, lz = shape
@@ -83,11 +84,11 @@ def test_blocked_sweep_then_other_main_loop_calls(monkeypatch):
         assert blk.mass() == one.mass()
 
 
-@pytest.mark.parametrize("prec,want", [("fp64", ("y", 11)), ("fp32", ("z", 11)), ("fp16", ("z", 0)),
+@pytest.mark.parametrize("prec,want", [("fp64", ("y", 18)), ("fp32", ("z", 18)), ("fp16", ("z", 0)),
                                        ("fp16c", ("z", 0))])
 def test_default_sweep_on_the_turbulent_channel(monkeypatch, prec, want):
-    """lbm_create's choice for the 192x768x384 channel: 11.2 MB slabs, eleven
-    to a chunk (a 146 MB ring), along y in FP64 and z in FP32; FP16 storage
+    """lbm_create's choice for the 192x768x384 channel: 11.2 MB slabs, 18 to
+    a chunk (a 224 MB ring), along y in FP64 and z in FP32; 2-byte storage
     runs single steps.  Only the handle is made (lattices allocated and
     cleared; no step runs)."""
     monkeypatch.delenv("LBM_BLOCK_PLANES", raising=False)
@@ -101,10 +102,10 @@ def test_default_sweep_on_the_turbulent_channel(monkeypatch, prec, want):
             assert ch.block_info()[1] == 0
 
 
-@pytest.mark.parametrize("prec,want", [("fp64", ("y", 11)), ("fp32", ("z", 11))])
+@pytest.mark.parametrize("prec,want", [("fp64", ("y", 18)), ("fp32", ("z", 18))])
 def test_default_sweep_bitwise_at_full_size(monkeypatch, prec, want):
     """The defaults behind the bench numbers (FP64: y slabs, FP32: z slabs,
-    eleven to a chunk) on the full 192x768x384 channel from the bench's start
+    18 to a chunk) on the full 192x768x384 channel from the bench's start
     (log-law + divergence-free perturbation): two blocked pairs equal four
     single steps bit for bit, populations and the FP64 mass.  (The FP32 unit
     allows FMA contraction; the sweep and the single step inline the same
