@@ -224,21 +224,26 @@ constexpr float FP16C_DOWN = 9.629649721936179e-35f; // 2^-113
 constexpr float FP16C_UP = 1.0384593717069655e34f;   // 2^113
 constexpr uint32_t FP16C_MAXBITS = 0x07FFF000u;      // FP32 bits of the largest value times 2^-113
 
-// 16-bit word of y, a value already multiplied by 2^-113.
+// 16-bit word of y, a value already multiplied by 2^-113.  The magnitude is
+// saturated as a float (one v_min_f32 with the |y| source modifier; FP32
+// denormals are kept, and minNum turns NaN into the largest value); the
+// rounded 15 bits are formed in the upper half of a 32-bit word beside the
+// sign, so the store takes that half (global_store_short_d16_hi).
 __host__ __device__ __forceinline__ uint16_t fp16c_pack(float y)
 {
     const uint32_t bits = __builtin_bit_cast(uint32_t, y);
-    uint32_t a = bits & 0x7fffffffu;
-    a = a < FP16C_MAXBITS ? a : FP16C_MAXBITS;
-    const uint32_t r = (a + 0x7ffu + ((a >> 12) & 1u)) >> 12;
-    return (uint16_t)(r | ((bits >> 16) & 0x8000u));
+    const uint32_t a = __builtin_bit_cast(uint32_t, fminf(fabsf(y), __builtin_bit_cast(float, FP16C_MAXBITS)));
+    const uint32_t t = a + 0x7ffu + ((a >> 12) & 1u);
+    return (uint16_t)(((t << 4) | (bits & 0x80000000u)) >> 16);
 }
 
-// The value of a 16-bit word times 2^-113 (multiply by FP16C_UP for it).
+// The value of a 16-bit word times 2^-113 (multiply by FP16C_UP for it).  The
+// word is sign-extended (a global_load_sshort), so after the shift by 12 the
+// sign is in bit 31 and its copies in bits 27-30 are masked off.
 __host__ __device__ __forceinline__ float fp16c_unpack(uint16_t h)
 {
-    const uint32_t w = h;
-    return __builtin_bit_cast(float, ((w & 0x7fffu) << 12) | ((w & 0x8000u) << 16));
+    const uint32_t s = (uint32_t)(int32_t)(int16_t)h;
+    return __builtin_bit_cast(float, (s << 12) & 0x87ffffffu);
 }
 
 template <>
