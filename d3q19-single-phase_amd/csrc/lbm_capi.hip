@@ -101,6 +101,7 @@ struct lbm_channel {
     float *mlp;         // learned closure parameters on the device (nullptr: plain MRT)
     float mlp_inv_ystar, mlp_yscale;
     int runs;           // step kernel: 1 = vector runs where the shape allows, 0 = one node per thread
+    int w32;            // step kernel, 2-byte storage: 1 = 32-bit pull offsets where N allows (default)
     uint32_t bK;        // two-step blocked sweep: slabs per chunk (0 = single steps only)
     int baxis;          // its sweep axis: 2 = z (slabs lx*ly), 1 = y (slabs lx*lz)
     int bforced;        // lbm_run_forcingp also pairs steps whose force varies over the nodes
@@ -321,6 +322,7 @@ int launch_step(lbm_channel *h)
     a.inv_ystar = h->mlp_inv_ystar;
     a.yscale = h->mlp_yscale;
     a.runs = h->runs;
+    a.w32 = h->w32;
     if (h->mlp) {
         if (mode & 3) // the closure runs with fused fields only
             return fail(LBM_ERR_STATE, "collision: the learned closure needs fields from macrovar");
@@ -539,6 +541,11 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         // the 192x768x384 channel: 64.6 % vs 69.2 % of HBM in FP64)
         const char *k = getenv("LBM_STEP_KERNEL");
         h->runs = (k && strcmp(k, "runs") == 0) ? 1 : 0;
+        // LBM_STEP_W32=0: 2-byte storage addresses its walled pulls from two
+        // 64-bit row bases even where 32-bit offsets would do (the same values;
+        // for comparing the two forms)
+        const char *w = getenv("LBM_STEP_W32");
+        h->w32 = (w && strcmp(w, "0") == 0) ? 0 : 1;
         // two-step blocked sweep (blocked.h, block_choice): chunks of K slabs
         // whose ring of K + 2 slabs stays within BLOCK_RING_BYTES, along the
         // axis whose chunk holds more nodes; below BLOCK_MIN_SLABS the sweep
