@@ -358,7 +358,8 @@ PORT_VS_REFERENCE_1CORE = round(5.35 / 5.45, 3)
 
 def host_cores():
     """Cores this process may use: its affinity set, capped by OMP_NUM_THREADS
-    when that is set (the GPU box gives one GPU's job 16)."""
+    when that is set (a job scheduler that gives one GPU's job 16 host
+    threads sets it to 16)."""
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     cap = os.environ.get("OMP_NUM_THREADS", "")
     return max(1, min(n, int(cap))) if cap.isdigit() and int(cap) > 0 else n
