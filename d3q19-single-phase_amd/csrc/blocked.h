@@ -46,13 +46,30 @@ inline BlockGeom block_geom(const Geom &g, int axis)
     return BlockGeom{g.ly, g.lz, g.lx, (uint64_t)g.lx * g.ly};
 }
 
-// Ring slots and buffer sizes in elements (slabs of lx * nr nodes).
-inline uint32_t block_ring_slots(uint32_t K) { return K + 2; }
-inline uint64_t block_ring_elems(const Geom &g, int axis, uint32_t K)
+// Row windows.  W = 0 (or W >= nr) sweeps whole slabs.  Otherwise the rows
+// are taken W at a time (the last window may be narrower), and each window is
+// swept on its own: step t of rows r0 - 1 .. r0 + W (wrapping along the
+// rows) goes to ring slabs of W + 2 rows, step t+1 of rows r0 .. r0 + W - 1
+// reads them.  The two extra rows are computed twice, by this window and the
+// neighbouring one; the values are the same, since each is a function of the
+// step-t lattice alone.  Windows keep the ring's slabs small when the
+// lattice's are too large for a useful chunk (the 2048x1024x512 channel).
+inline bool block_windowed(const Geom &g, int axis, uint32_t W) { return W > 0 && W < block_geom(g, axis).nr; }
+inline uint32_t block_ring_rows(const Geom &g, int axis, uint32_t W)
 {
-    return 19ull * block_ring_slots(K) * g.lx * block_geom(g, axis).nr;
+    return block_windowed(g, axis, W) ? W + 2 : block_geom(g, axis).nr;
 }
-inline uint64_t block_head_elems(const Geom &g, int axis) { return 19ull * 2 * g.lx * block_geom(g, axis).nr; }
+
+// Ring slots and buffer sizes in elements (slabs of lx * ring rows nodes).
+inline uint32_t block_ring_slots(uint32_t K) { return K + 2; }
+inline uint64_t block_ring_elems(const Geom &g, int axis, uint32_t K, uint32_t W = 0)
+{
+    return 19ull * block_ring_slots(K) * g.lx * block_ring_rows(g, axis, W);
+}
+inline uint64_t block_head_elems(const Geom &g, int axis, uint32_t W = 0)
+{
+    return 19ull * 2 * g.lx * block_ring_rows(g, axis, W);
+}
 
 // The force of one of the two steps of a pair: rates with the uniform force
 // (r.fx, r.fy, r.fz), or per_node with the collision's force fc and the one
@@ -63,33 +80,48 @@ struct StepForce {
     int per_node;
 };
 
-// Launch list of one pair: step t with force f0, step t+1 with force f1.
+// Launch list of one pair: step t with force f0, step t+1 with force f1,
+// whole slabs (W = 0) or row windows of W rows.
 inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const StepForce &f0, const StepForce &f1,
-                                             uint32_t K, void *lat_in, void *lat_out, void *ring, void *head)
+                                             uint32_t K, uint32_t W, void *lat_in, void *lat_out, void *ring,
+                                             void *head)
 {
     const BlockGeom bg = block_geom(g, axis);
-    const uint32_t ns = bg.ns, slots = block_ring_slots(K);
-    const uint64_t slab = (uint64_t)g.lx * bg.nr;
+    const uint32_t ns = bg.ns, nr = bg.nr, slots = block_ring_slots(K);
+    const bool win = block_windowed(g, axis, W);
+    const uint32_t rr = block_ring_rows(g, axis, W);
+    const uint64_t rslab = (uint64_t)g.lx * rr; // nodes of a ring slab
     const PlaneStore full_in = {lat_in, g.N, ns, make_fastdiv(ns), bg.lat_slab_stride, bg.lat_row_stride, nullptr, 0,
                                 0, 0};
     const PlaneStore full_out = {lat_out, g.N, ns, make_fastdiv(ns), bg.lat_slab_stride, bg.lat_row_stride, nullptr, 0,
                                  0, 0};
-    const PlaneStore ring_only = {ring, slots * slab, slots, make_fastdiv(slots), slab, g.lx, nullptr, 0, 0, 0};
+    const PlaneStore ring_only = {ring, slots * rslab, slots, make_fastdiv(slots), rslab, g.lx, nullptr, 0, 0, 0};
     PlaneStore ring_head = ring_only;
     ring_head.head = head;
-    ring_head.head_stride = 2 * slab;
+    ring_head.head_stride = 2 * rslab;
     ring_head.head_lo = 0;
     ring_head.head_n = std::min<uint32_t>(2, ns);
 
+    // local-row frames of the two launch kinds (BlockArgs): step t reads the
+    // lattice and writes the ring, step t+1 reads the ring and writes the
+    // lattice
+    struct Rows {
+        uint32_t nrw, src_r0, src_nr, dst_r0, node_r0;
+    };
     std::vector<BlockArgs> out;
     auto add = [&](int32_t s0, uint32_t count, const PlaneStore &src, const PlaneStore &dst, int nt,
-                   const StepForce &f) {
+                   const StepForce &f, const Rows &rw) {
         BlockArgs a;
         a.g = g;
         a.ns = ns;
-        a.nr = bg.nr;
-        a.slab = (uint32_t)slab;
-        a.slabp = slab_padded((uint32_t)slab);
+        a.nr = nr;
+        a.nrw = rw.nrw;
+        a.src_r0 = rw.src_r0;
+        a.src_nr = rw.src_nr;
+        a.dst_r0 = rw.dst_r0;
+        a.node_r0 = rw.node_r0;
+        a.slab = g.lx * rw.nrw;
+        a.slabp = slab_padded(a.slab);
         a.dslabp = make_fastdiv(a.slabp);
         a.r = f.r;
         a.fc = f.fc;
@@ -102,15 +134,28 @@ inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const Step
         a.nt = nt;
         out.push_back(a);
     };
-    for (uint32_t s0 = 0; s0 < ns; s0 += K) {
-        const uint32_t n = std::min(K, ns - s0);
-        add((int32_t)s0, n, full_in, ring_head, 0, f0);
-        const uint32_t lo = std::max<uint32_t>(s0 == 0 ? 0 : s0 - 1, 1), hi = s0 + n - 1;
-        if (hi > lo)
-            add((int32_t)lo, hi - lo, ring_only, full_out, 1, f1);
+    const uint32_t step = win ? W : nr;
+    for (uint32_t r0 = 0; r0 < nr; r0 += step) {
+        const uint32_t w = std::min(step, nr - r0);
+        Rows ra, rb;
+        if (win) {
+            const uint32_t lo = r0 == 0 ? nr - 1 : r0 - 1; // lattice row of ring row 0
+            ra = Rows{w + 2, lo, nr, 0, lo};
+            rb = Rows{w, 1, w + 2, r0, r0};
+        } else {
+            ra = Rows{nr, 0, nr, 0, 0};
+            rb = ra;
+        }
+        for (uint32_t s0 = 0; s0 < ns; s0 += K) {
+            const uint32_t n = std::min(K, ns - s0);
+            add((int32_t)s0, n, full_in, ring_head, 0, f0, ra);
+            const uint32_t lo = std::max<uint32_t>(s0 == 0 ? 0 : s0 - 1, 1), hi = s0 + n - 1;
+            if (hi > lo)
+                add((int32_t)lo, hi - lo, ring_only, full_out, 1, f1, rb);
+        }
+        // slabs ns - 1 and 0 (one slab when ns == 1)
+        add((int32_t)ns - 1, std::min<uint32_t>(2, ns), ring_head, full_out, 1, f1, rb);
     }
-    // slabs ns - 1 and 0 (one slab when ns == 1)
-    add((int32_t)ns - 1, std::min<uint32_t>(2, ns), ring_head, full_out, 1, f1);
     return out;
 }
 
@@ -124,69 +169,109 @@ constexpr long BLOCK_MIN_SLABS = 4;
 constexpr long BLOCK_MAX_SLABS = 32;
 
 // The sweep lbm_create chooses for a lattice of elem-byte values: axis 2 (z)
-// or 1 (y) and K slabs per chunk (0: single steps only).  Along each axis K is
-// the number of slabs whose ring of K + 2 fits ring_bytes, at most
-// BLOCK_MAX_SLABS and the number of slabs.  The axis whose chunk holds more
-// nodes is taken (the larger slabs on a tie): the two extra ring slots are
-// then the smallest share of the ring.  A chunk of fewer than min_slabs
-// turns the sweep off.  axis_env / planes_env (may be null) are the values of
-// LBM_BLOCK_AXIS ("z" or "y") and LBM_BLOCK_PLANES (K, 0 turns the sweep
-// off), which override the choice; any other value of either is reported in
-// `bad`.  blocked = false (2-byte storage, whose step is bound by
-// arithmetic) makes the default K zero.
+// or 1 (y), K slabs per chunk (0: single steps only) and W rows per window
+// (0: whole slabs).  Along each axis K is the number of slabs whose ring of
+// K + 2 fits ring_bytes, at most BLOCK_MAX_SLABS and the number of slabs.
+// The axis whose chunk holds more nodes is taken (the larger slabs on a
+// tie): the two extra ring slots are then the smallest share of the ring.
+// When that chunk has fewer than min_slabs slabs, the slabs are cut into
+// row windows instead: as few windows, of equal height, as keep a window's
+// slab within ring_bytes / (BLOCK_WINDOW_K + 2), swept along the axis with
+// more slabs; with no such window the sweep is off.  axis_env / planes_env /
+// rows_env (may be null) are the values of LBM_BLOCK_AXIS ("z" or "y"),
+// LBM_BLOCK_PLANES (K, 0 turns the sweep off) and LBM_BLOCK_ROWS (W, 0 for
+// whole slabs), which override the choice (K then follows from W's slabs
+// unless LBM_BLOCK_PLANES is set); any other value is reported in `bad`.
+// blocked = false (2-byte storage, whose step is bound by arithmetic) makes
+// the default K zero.
+constexpr long BLOCK_WINDOW_K = 18;
+
 struct BlockChoice {
     int axis;
     uint32_t K;
+    uint32_t W;
     const char *bad; // null, or the variable whose value is invalid (the caller fails)
 };
 
 inline BlockChoice block_choice(const Geom &g, uint32_t elem, bool blocked, double ring_bytes, long min_slabs,
-                                const char *axis_env, const char *planes_env)
+                                const char *axis_env, const char *planes_env, const char *rows_env = nullptr)
 {
-    auto chunk = [&](int axis) -> long {
-        const BlockGeom bg = block_geom(g, axis);
-        const double slab_bytes = (double)g.lx * bg.nr * 19.0 * (double)elem;
+    // slabs of `rows` rows per chunk along `axis`
+    auto chunk_rows = [&](int axis, uint32_t rows) -> long {
+        const double slab_bytes = (double)g.lx * rows * 19.0 * (double)elem;
         const long k = (long)(ring_bytes / slab_bytes) - 2;
-        return std::max<long>(0, std::min<long>(std::min<long>(k, BLOCK_MAX_SLABS), (long)bg.ns));
+        return std::max<long>(0, std::min<long>(std::min<long>(k, BLOCK_MAX_SLABS), (long)block_geom(g, axis).ns));
     };
+    auto chunk = [&](int axis) -> long { return chunk_rows(axis, block_geom(g, axis).nr); };
     auto chunk_nodes = [&](int axis) -> uint64_t {
         return (uint64_t)chunk(axis) * g.lx * block_geom(g, axis).nr;
+    };
+    auto parse = [](const char *v, long &out) -> bool {
+        char *end = nullptr;
+        out = strtol(v, &end, 10);
+        return end != v && *end == 0 && out >= 0;
     };
     // z slabs (lx * ly nodes) are the larger ones when ly >= lz
     const int big = g.ly >= g.lz ? 2 : 1, small = 3 - big;
     int axis = chunk_nodes(small) > chunk_nodes(big) ? small : big;
+    if (blocked && chunk(axis) < min_slabs)
+        axis = g.ly >= g.lz ? 1 : 2; // row windows, along the axis with more slabs
     if (axis_env) {
         if (strcmp(axis_env, "y") == 0)
             axis = 1;
         else if (strcmp(axis_env, "z") == 0)
             axis = 2;
         else
-            return BlockChoice{axis, 0, "LBM_BLOCK_AXIS (y or z)"};
-    }
-    long kb = blocked ? chunk(axis) : 0;
-    if (kb < min_slabs)
-        kb = 0;
-    if (planes_env) {
-        char *end = nullptr;
-        const long v = strtol(planes_env, &end, 10);
-        if (end == planes_env || *end != 0 || v < 0)
-            return BlockChoice{axis, 0, "LBM_BLOCK_PLANES (an integer >= 0)"};
-        kb = v;
+            return BlockChoice{axis, 0, 0, "LBM_BLOCK_AXIS (y or z)"};
     }
     const BlockGeom bg = block_geom(g, axis);
-    kb = std::min<long>(std::min<long>(kb, BLOCK_MAX_SLABS), (long)bg.ns);
-    if (kb < 1 || (uint64_t)slab_padded(g.lx * bg.nr) * (uint64_t)kb >= (1ull << 31))
+    long kb = blocked ? chunk(axis) : 0;
+    long W = 0;
+    if (blocked && kb < min_slabs) {
+        const double row_bytes = (double)g.lx * 19.0 * (double)elem;
+        const long wmax = (long)(ring_bytes / (double)(BLOCK_WINDOW_K + 2) / row_bytes) - 2;
         kb = 0;
-    return BlockChoice{axis, (uint32_t)kb, nullptr};
+        if (wmax >= 1 && wmax < (long)bg.nr) {
+            const long nwin = ((long)bg.nr + wmax - 1) / wmax;
+            W = ((long)bg.nr + nwin - 1) / nwin;
+            kb = chunk_rows(axis, (uint32_t)W + 2);
+            if (kb < min_slabs)
+                kb = 0, W = 0;
+        }
+    }
+    if (rows_env) {
+        if (!parse(rows_env, W))
+            return BlockChoice{axis, 0, 0, "LBM_BLOCK_ROWS (an integer >= 0)"};
+        if (W >= (long)bg.nr)
+            W = 0;
+        if (W > 0) {
+            kb = chunk_rows(axis, (uint32_t)W + 2);
+        } else {
+            kb = blocked ? chunk(axis) : 0;
+            if (kb < min_slabs)
+                kb = 0;
+        }
+    }
+    if (planes_env) {
+        long v;
+        if (!parse(planes_env, v))
+            return BlockChoice{axis, 0, 0, "LBM_BLOCK_PLANES (an integer >= 0)"};
+        kb = v;
+    }
+    kb = std::min<long>(std::min<long>(kb, BLOCK_MAX_SLABS), (long)bg.ns);
+    const uint32_t rows = W > 0 ? (uint32_t)W + 2 : bg.nr;
+    if (kb < 1 || (uint64_t)slab_padded(g.lx * rows) * (uint64_t)kb >= (1ull << 31))
+        kb = 0;
+    return BlockChoice{axis, (uint32_t)kb, kb > 0 ? (uint32_t)W : 0u, nullptr};
 }
 
 // A pair under one uniform force (r.fx, r.fy, r.fz).
-inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const RatesD &r, uint32_t K, void *lat_in,
-                                             void *lat_out, void *ring, void *head)
+inline std::vector<BlockArgs> block_schedule(const Geom &g, int axis, const RatesD &r, uint32_t K, uint32_t W,
+                                             void *lat_in, void *lat_out, void *ring, void *head)
 {
     const ForceSpec u = {r.fx, r.fy, r.fz, nullptr, 0.0, 0.0, 0.0};
     const StepForce f = {r, u, u, 0};
-    return block_schedule(g, axis, f, f, K, lat_in, lat_out, ring, head);
+    return block_schedule(g, axis, f, f, K, W, lat_in, lat_out, ring, head);
 }
 
 } // namespace d3q19

@@ -187,10 +187,20 @@ __host__ __device__ inline PlaneRef plane_write_head(const PlaneStore &st, uint3
 // per_node: the body force is evaluated at every node as in step mode 16
 // (fc for the collision, fm for the fused fields u = j + F/2); otherwise it
 // is r.fx, r.fy, r.fz for both.
+// Rows: a launch computes rows 0 .. nrw - 1 of each slab in its own frame
+// (local rows).  Local row rl is row rl + src_r0 (modulo src_nr) of the
+// source store, row rl + dst_r0 of the destination store and lattice row
+// rl + node_r0 (modulo nr).  Over whole slabs all three frames are the
+// lattice's (offsets 0, src_nr = nrw = nr).  A row window of W rows starting
+// at lattice row r0 computes step t on rows r0 - 1 .. r0 + W (nrw = W + 2)
+// into a ring whose slabs hold those W + 2 rows, and step t+1 on rows
+// r0 .. r0 + W - 1 from ring rows 1 .. W.
 struct BlockArgs {
     Geom g;
-    uint32_t ns, nr;     // slabs along the sweep axis, rows per slab
-    uint32_t slab;       // nodes per slab, lx * nr
+    uint32_t ns, nr;     // slabs along the sweep axis, rows per lattice slab
+    uint32_t nrw;        // rows computed per slab (local rows)
+    uint32_t src_r0, src_nr, dst_r0, node_r0; // frames of the local rows
+    uint32_t slab;       // nodes computed per slab, lx * nrw
     uint32_t slabp;      // slab rounded up to a whole number of waves
     FastDiv dslabp;      // division by slabp
     int32_t s0;
@@ -246,6 +256,26 @@ __host__ __device__ inline BlockLane block_lane(uint32_t ix, uint32_t r, uint32_
     l.ix = ix;
     l.lx = lx;
     return l;
+}
+
+// Node w (< a.slab) of a launch's slab: its column, local row, row in the
+// source store and lattice row.  The schedule keeps rl + src_r0 < 2 src_nr
+// and rl + node_r0 < 2 nr (windows are narrower than the lattice slab), so
+// one conditional subtraction wraps each.
+struct BlockRow {
+    uint32_t ix, rl, rs, rg;
+};
+
+__host__ __device__ inline BlockRow block_row(const BlockArgs &a, uint32_t w)
+{
+    BlockRow b;
+    b.rl = fdiv(w, a.g.dlx);
+    b.ix = w - b.rl * a.g.lx;
+    uint32_t rs = b.rl + a.src_r0;
+    b.rs = rs >= a.src_nr ? rs - a.src_nr : rs;
+    uint32_t rg = b.rl + a.node_r0;
+    b.rg = rg >= a.nr ? rg - a.nr : rg;
+    return b;
 }
 
 // Buffer and element index that F_I of the node is gathered from: P(x - c_I)

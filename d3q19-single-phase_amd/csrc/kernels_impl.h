@@ -401,12 +401,12 @@ __global__ void __launch_bounds__((plane_block<P, NT>())) k_step_planes(BlockArg
     const uint32_t w = t - sl * a.slabp;
     if (sl >= a.count || w >= a.slab)
         return;
-    const uint32_t r = fdiv(w, a.g.dlx);
-    const uint32_t ix = w - r * a.g.lx;
+    const BlockRow br = block_row(a, w);
+    const uint32_t ix = br.ix;
     const uint32_t sb = block_plane(a, sl);
     PlaneRef pl[3];
     block_planes(a, sb, pl);
-    const BlockLane l = block_lane(ix, r, a.nr, (uint32_t)a.src.row_stride, a.g.lx);
+    const BlockLane l = block_lane(ix, br.rs, a.src_nr, (uint32_t)a.src.row_stride, a.g.lx);
 
     T f[19];
     block_gather<P, AXIS, NT == 0>(pl, l, f, std::make_integer_sequence<int, 19>{});
@@ -414,7 +414,7 @@ __global__ void __launch_bounds__((plane_block<P, NT>())) k_step_planes(BlockArg
     Rates<T> rt = rates_as<T>(a.r);
     T fmx = rt.fx, fmy = rt.fy, fmz = rt.fz; // force in u = j + F/2 of the fused fields
     if constexpr (FP != 0) {
-        const uint32_t iy = AXIS == 2 ? r : sb, iz = AXIS == 2 ? sb : r;
+        const uint32_t iy = AXIS == 2 ? br.rg : sb, iz = AXIS == 2 ? sb : br.rg;
         double x, y, z;
         node_force(a.fc, a.g, ix, iy, iz, x, y, z);
         rt.fx = (T)x;
@@ -445,7 +445,7 @@ __global__ void __launch_bounds__((plane_block<P, NT>())) k_step_planes(BlockArg
     }
 
     const PlaneRef d = plane_write(a.dst, sb);
-    const uint32_t e = r * (uint32_t)d.row_stride + ix; // head and ring rows share the stride
+    const uint32_t e = (br.rl + a.dst_r0) * (uint32_t)d.row_stride + ix; // head and ring rows share the stride
     block_store<P, NT>(d, e, f, std::make_integer_sequence<int, 19>{});
     const PlaneRef hd = plane_write_head(a.dst, sb);
     if (hd.buf != nullptr)

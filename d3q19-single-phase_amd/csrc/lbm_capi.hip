@@ -104,6 +104,7 @@ struct lbm_channel {
     int w32;            // step kernel, 2-byte storage: 1 = 32-bit pull offsets where N allows (default)
     uint32_t bK;        // two-step blocked sweep: slabs per chunk (0 = single steps only)
     int baxis;          // its sweep axis: 2 = z (slabs lx*ly), 1 = y (slabs lx*lz)
+    uint32_t bW;        // rows per window of its slabs (0 = whole slabs)
     int bforced;        // lbm_run_forcingp also pairs steps whose force varies over the nodes
     void *bring, *bhead; // its slab ring and head buffer (allocated on first use)
     bool ave_pending;   // lbm_run_particles: avedensity due once the next bounce-back has run
@@ -555,17 +556,20 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         // both FP64 (y: 20 650-20 700 MLUPS against 20 000-20 090 for z
         // slabs, 8 to a chunk) and FP32 (z: about 42 000).
         // LBM_BLOCK_AXIS=z|y sets the axis, LBM_BLOCK_PLANES=K the chunk (0:
-        // off).  Not by default for 2-byte storage, whose step is bound by
-        // arithmetic (FP16: measured 66 300 MLUPS blocked, 32 to a chunk,
-        // against 72 000 single-step).
-        const BlockChoice bc = block_choice(h->g, h->elem, h->elem != 2, BLOCK_RING_BYTES, BLOCK_MIN_SLABS,
-                                            getenv("LBM_BLOCK_AXIS"), getenv("LBM_BLOCK_PLANES"));
+        // off), LBM_BLOCK_ROWS=W row windows of W rows (0: whole slabs).  Not
+        // by default for 2-byte storage, whose step is bound by arithmetic
+        // (FP16: measured 66 300 MLUPS blocked, 32 to a chunk, against 72 000
+        // single-step).
+        const BlockChoice bc =
+            block_choice(h->g, h->elem, h->elem != 2, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, getenv("LBM_BLOCK_AXIS"),
+                         getenv("LBM_BLOCK_PLANES"), getenv("LBM_BLOCK_ROWS"));
         if (bc.bad) {
             delete h;
             return fail(LBM_ERR_ARG, "lbm_create: invalid value of %s", bc.bad);
         }
         h->baxis = bc.axis;
         h->bK = bc.K;
+        h->bW = bc.W;
         // LBM_BLOCK_FORCED=1: pairs under a force evaluated per node too.
         // Off by default: on the 192x768x384 channel such pairs took 3.33 ms
         // per FP64 step against 3.10 ms for single steps (the per-node force
@@ -1106,8 +1110,8 @@ static StepForce step_force_of(const lbm_channel *h, const ForceSpec &fc, const 
 static int run_pair_with(lbm_channel *h, const StepForce &f0, const StepForce &f1)
 {
     if (!h->bring) {
-        const size_t rb = block_ring_elems(h->g, h->baxis, h->bK) * h->elem,
-                     hb = block_head_elems(h->g, h->baxis) * h->elem;
+        const size_t rb = block_ring_elems(h->g, h->baxis, h->bK, h->bW) * h->elem,
+                     hb = block_head_elems(h->g, h->baxis, h->bW) * h->elem;
         hipError_t e = hipMalloc(&h->bring, rb);
         if (e == hipSuccess)
             e = hipMalloc(&h->bhead, hb);
@@ -1120,7 +1124,8 @@ static int run_pair_with(lbm_channel *h, const StepForce &f0, const StepForce &f
         }
     }
     const auto launches =
-        block_schedule(h->g, h->baxis, f0, f1, h->bK, h->lat[h->cur], h->lat[1 - h->cur], h->bring, h->bhead);
+        block_schedule(h->g, h->baxis, f0, f1, h->bK, h->bW, h->lat[h->cur], h->lat[1 - h->cur], h->bring,
+                       h->bhead);
     for (const BlockArgs &a : launches) {
         int rc = dispatch_err(PREC_CALL(h, step_planes, a, h->baxis, h->stream), "collision (blocked sweep)");
         if (rc)
@@ -1671,7 +1676,7 @@ int lbm_device_bytes(lbm_channel *h, size_t *bytes)
     if (h->stage)
         b += COPY_NODES * 19 * sizeof(double);
     if (h->bring)
-        b += (block_ring_elems(h->g, h->baxis, h->bK) + block_head_elems(h->g, h->baxis)) * h->elem;
+        b += (block_ring_elems(h->g, h->baxis, h->bK, h->bW) + block_head_elems(h->g, h->baxis, h->bW)) * h->elem;
     b += (2 * RED_BLOCKS + (size_t)PLANE_CHUNKS * NPLANE * h->g.lx + 2 + 2 * NPLANE * (size_t)h->g.lx) * sizeof(double) + 8;
     *bytes = b;
     return LBM_OK;
@@ -1684,6 +1689,15 @@ int lbm_block_info(lbm_channel *h, int *axis, int *slabs)
         *axis = h->baxis == 2 ? 'z' : 'y';
     if (slabs)
         *slabs = (int)h->bK;
+    return LBM_OK;
+}
+
+int lbm_block_rows(lbm_channel *h, int *rows)
+{
+    CHECK_H(h);
+    if (!rows)
+        return fail(LBM_ERR_ARG, "lbm_block_rows: null output");
+    *rows = (int)h->bW;
     return LBM_OK;
 }
 

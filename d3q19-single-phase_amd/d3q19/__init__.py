@@ -88,6 +88,7 @@ _SIGS = {
     "lbm_time_steps": [_H, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "lbm_device_bytes": [_H, ctypes.POINTER(ctypes.c_size_t)],
     "lbm_block_info": [_H, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "lbm_block_rows": [_H, ctypes.POINTER(ctypes.c_int)],
     "lbm_download_post": [_H, _D],
     "lbm_download_post_range": [_H, ctypes.c_longlong, ctypes.c_longlong, _D],
     "lbm_particles_create": [_H, _D, _D, _D, _D],
@@ -442,6 +443,12 @@ class Channel:
         a, k = ctypes.c_int(), ctypes.c_int()
         _call("lbm_block_info", self._h, ctypes.byref(a), ctypes.byref(k))
         return chr(a.value), k.value
+
+    def block_rows(self):
+        """Rows per window of the blocked sweep's slabs (0: whole slabs)."""
+        w = ctypes.c_int()
+        _call("lbm_block_rows", self._h, ctypes.byref(w))
+        return w.value
 
     def download_post(self):
         """Stored post-collision lattice, FP64, shape (19, N) (node = ix + lx*(iy + ly*iz))."""

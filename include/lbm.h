@@ -177,7 +177,7 @@ int lbm_avedensity(lbm_channel *h, double *rhomean);
  * the single-step path's bit for bit.  The environment variables
  * LBM_BLOCK_AXIS=z|y and LBM_BLOCK_PLANES (read by lbm_create) set the axis
  * and the slabs per chunk (at most 32), 0 turns the sweep off; lbm_block_info
- * reports the choice. */
+ * and lbm_block_rows report the choice. */
 int lbm_run(lbm_channel *h, int nsteps);
 
 /* Sum of all populations in FP64 (mass monitor; the sum is invariant under
@@ -260,6 +260,14 @@ int lbm_device_bytes(lbm_channel *h, size_t *bytes);
  * per chunk (0: single steps only).  Either pointer may be null.  No
  * reference counterpart (measurement and tests). */
 int lbm_block_info(lbm_channel *h, int *axis, int *slabs);
+
+/* Rows per window of that sweep's slabs: 0 when it sweeps whole slabs.  A
+ * lattice whose slabs are too large for a chunk of four in the ring (the
+ * 2048x1024x512 channel) is swept in row windows, each window on its own,
+ * the two rows at its edges computed twice; the environment variable
+ * LBM_BLOCK_ROWS=W sets the window.  No reference counterpart (measurement
+ * and tests). */
+int lbm_block_rows(lbm_channel *h, int *rows);
 
 /* The stored post-collision lattice as FP64 structure-of-arrays (19*N values,
  * population-major, node index ix + lx*(iy + ly*iz)), for checkpoint-free

@@ -5,9 +5,11 @@
 // is both read and written (its threads run concurrently); every gathered
 // value is the tag the single-step pull would read (P_s(x - c_i)_i, or
 // P_s(x)_opp(i) at a wall), so no ring slot is read after being overwritten;
-// every node and population of the output lattice holds its step-t+2 tag.
-// Also: the default sweep (block_choice) for the tuned workloads and under
-// the environment overrides.  Launches nothing on a GPU.
+// every node and population of the output lattice holds its step-t+2 tag and
+// is written exactly once.  Whole slabs and row windows (ragged last window,
+// windows of one row, windows one row short of the slab).  Also: the default
+// sweep (block_choice) for the tuned workloads and under the environment
+// overrides.  Launches nothing on a GPU.
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -103,7 +105,8 @@ static void gather_all(Ctx &c, const PlaneRef (&pl)[3], const BlockLane &l, uint
 }
 
 // the kernel's store addressing: element (uint32) r * row_stride + ix past the
-// slab offset, population stride d.stride
+// slab offset (r: the local row plus the launch's dst_r0), population stride
+// d.stride
 static void store(Ctx &c, const PlaneRef &d, uint32_t ix, uint32_t r, uint64_t n, int s)
 {
     Buf *B = c.find(d.buf);
@@ -120,7 +123,7 @@ static void store(Ctx &c, const PlaneRef &d, uint32_t ix, uint32_t r, uint64_t n
     }
 }
 
-static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
+static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis, uint32_t W = 0)
 {
     Ctx c;
     c.g = geom(lx, ly, lz);
@@ -128,8 +131,8 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
     Buf in, out, ring, head;
     in.v.resize(19 * g.N);
     out.v.assign(19 * g.N, 0);
-    ring.v.assign(block_ring_elems(g, axis, K), 0);
-    head.v.assign(block_head_elems(g, axis), 0);
+    ring.v.assign(block_ring_elems(g, axis, K, W), 0);
+    head.v.assign(block_head_elems(g, axis, W), 0);
     c.bufs[0] = &in;
     c.bufs[1] = &out;
     c.bufs[2] = &ring;
@@ -138,7 +141,9 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
         for (int i = 0; i < 19; i++)
             in.v[(uint64_t)i * g.N + n] = c.tag(0, n, i);
     RatesD r = {};
-    const auto launches = block_schedule(g, axis, r, K, in.v.data(), out.v.data(), ring.v.data(), head.v.data());
+    const auto launches =
+        block_schedule(g, axis, r, K, W, in.v.data(), out.v.data(), ring.v.data(), head.v.data());
+    uint64_t out_writes = 0;
     for (const BlockArgs &a : launches) {
       c.reads.clear();
       c.writes.clear();
@@ -151,6 +156,12 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
         }
         if (a.s0 < -(int32_t)a.ns || a.s0 >= (int32_t)a.ns || a.count > a.ns) {
             fprintf(stderr, "launch s0 %d count %u outside the range block_plane wraps once\n", a.s0, a.count);
+            return false;
+        }
+        if (a.slab != g.lx * a.nrw || a.src_r0 >= a.src_nr || a.node_r0 >= a.nr ||
+            (uint64_t)a.nrw + a.src_r0 > 2ull * a.src_nr || (uint64_t)a.nrw + a.node_r0 > 2ull * a.nr) {
+            fprintf(stderr, "launch rows: nrw %u src_r0 %u src_nr %u node_r0 %u nr %u outside one wrap\n", a.nrw,
+                    a.src_r0, a.src_nr, a.node_r0, a.nr);
             return false;
         }
         if (a.slabp % WAVE_LANES != 0 || a.slabp < a.slab || a.slabp - a.slab >= WAVE_LANES) {
@@ -169,12 +180,13 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
             const uint32_t w = t - sl * a.slabp;
             if (sl >= a.count || w >= a.slab)
                 continue;
-            const uint32_t rr = fdiv(w, g.dlx);
-            const uint32_t ix = w - rr * g.lx;
+            // the kernel's row mapping (block_row) and gather addressing
+            const BlockRow br = block_row(a, w);
+            const uint32_t ix = br.ix, rr = br.rg;
             const uint32_t sb = block_plane(a, sl);
             PlaneRef pl[3];
             block_planes(a, sb, pl);
-            const BlockLane l = block_lane(ix, rr, a.nr, (uint32_t)a.src.row_stride, g.lx);
+            const BlockLane l = block_lane(ix, br.rs, a.src_nr, (uint32_t)a.src.row_stride, g.lx);
             if (axis == 2)
                 gather_all<2>(c, pl, l, ix, rr, sb, s, std::make_integer_sequence<int, 19>{});
             else
@@ -182,10 +194,12 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
             if (!c.ok)
                 return false;
             const uint64_t n = node(g, axis, ix, rr, sb);
-            store(c, plane_write(a.dst, sb), ix, rr, n, s + 1);
+            store(c, plane_write(a.dst, sb), ix, br.rl + a.dst_r0, n, s + 1);
+            if (a.dst.ring == (void *)out.v.data())
+                out_writes += 19;
             const PlaneRef hd = plane_write_head(a.dst, sb);
             if (hd.buf != nullptr)
-                store(c, hd, ix, rr, n, s + 1);
+                store(c, hd, ix, br.rl + a.dst_r0, n, s + 1);
             if (!c.ok)
                 return false;
         }
@@ -202,6 +216,11 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
       for (const auto &w : c.pending)
           *w.first = w.second;
     }
+    if (out_writes != 19 * g.N) {
+        fprintf(stderr, "%ux%ux%u K=%u W=%u axis %d: %llu output stores for %llu values\n", lx, ly, lz, K, W, axis,
+                (unsigned long long)out_writes, (unsigned long long)(19 * g.N));
+        return false;
+    }
     for (uint64_t n = 0; n < g.N; n++)
         for (int i = 0; i < 19; i++)
             if (out.v[(uint64_t)i * g.N + n] != c.tag(2, n, i)) {
@@ -209,19 +228,22 @@ static bool check(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t K, int axis)
                         lz, K, axis, (unsigned long long)n, i);
                 return false;
             }
-    printf("ok %ux%ux%u K=%u axis %c (%zu launches)\n", lx, ly, lz, K, axis == 2 ? 'z' : 'y', launches.size());
+    printf("ok %ux%ux%u K=%u W=%u axis %c (%zu launches)\n", lx, ly, lz, K, W, axis == 2 ? 'z' : 'y',
+           launches.size());
     return true;
 }
 
 // lbm_create's default sweep (block_choice) for the workloads it was tuned on.
 static bool check_choice(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t elem, bool blocked, const char *axis_env,
-                         const char *planes_env, int want_axis, uint32_t want_K)
+                         const char *planes_env, int want_axis, uint32_t want_K, uint32_t want_W = 0,
+                         const char *rows_env = nullptr)
 {
     const Geom g = geom(lx, ly, lz);
-    const BlockChoice c = block_choice(g, elem, blocked, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, axis_env, planes_env);
-    if (c.axis != want_axis || c.K != want_K) {
-        fprintf(stderr, "%ux%ux%u elem %u: sweep axis %d K %u, want axis %d K %u\n", lx, ly, lz, elem, c.axis, c.K,
-                want_axis, want_K);
+    const BlockChoice c =
+        block_choice(g, elem, blocked, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, axis_env, planes_env, rows_env);
+    if (c.bad || c.axis != want_axis || c.K != want_K || c.W != want_W) {
+        fprintf(stderr, "%ux%ux%u elem %u: sweep axis %d K %u W %u, want axis %d K %u W %u\n", lx, ly, lz, elem,
+                c.axis, c.K, c.W, want_axis, want_K, want_W);
         return false;
     }
     return true;
@@ -240,8 +262,21 @@ int main()
                   check_choice(192, 768, 384, 2, false, "y", "12", 1, 12) &&
                   check_choice(192, 768, 384, 8, true, nullptr, "40", 1, 32) &&
                   // 2048x1024x512 in FP32: 80 MB z and 40 MB y slabs (three
-                  // to a ring), no sweep
-                  check_choice(512, 2048, 1024, 4, true, nullptr, nullptr, 1, 0) &&
+                  // to a ring): y slabs in four windows of 256 rows (10 MB
+                  // window slabs, 20 to a chunk); FP16 storage single steps;
+                  // the overrides
+                  check_choice(512, 2048, 1024, 4, true, nullptr, nullptr, 1, 20, 256) &&
+                  check_choice(512, 2048, 1024, 2, false, nullptr, nullptr, 1, 0, 0) &&
+                  check_choice(512, 2048, 1024, 4, true, "z", nullptr, 2, 20, 256) &&
+                  check_choice(512, 2048, 1024, 4, true, nullptr, nullptr, 1, 0, 0, "0") &&
+                  check_choice(512, 2048, 1024, 4, true, nullptr, "8", 1, 8, 128, "128") &&
+                  check_choice(512, 2048, 1024, 4, true, nullptr, "0", 1, 0, 0) &&
+                  // the 2048x1024x512 FP64 grid of half size (256x1024x512):
+                  // 19.9 MB y slabs, 9 to a chunk, whole slabs
+                  check_choice(256, 1024, 512, 8, true, nullptr, nullptr, 1, 9) &&
+                  // windows on a small grid by the override only
+                  check_choice(12, 10, 6, 8, true, nullptr, "3", 2, 3, 4, "4") &&
+                  check_choice(12, 10, 6, 8, true, nullptr, nullptr, 2, 6, 0, "10") &&
                   // 96x384x192 in FP32 (2.8 MB z slabs, 1.4 MB y slabs): 32
                   // z slabs hold more nodes than 32 y slabs
                   check_choice(96, 384, 192, 4, true, nullptr, nullptr, 2, 32) &&
@@ -261,11 +296,16 @@ int main()
             fprintf(stderr, "LBM_BLOCK_AXIS=\"%s\" accepted\n", bad_axis);
             return 1;
         }
-    for (const char *bad_k : {"", "abc", "4x", "-1", " "})
+    for (const char *bad_k : {"", "abc", "4x", "-1", " "}) {
         if (!block_choice(g0, 8, true, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, nullptr, bad_k).bad) {
             fprintf(stderr, "LBM_BLOCK_PLANES=\"%s\" accepted\n", bad_k);
             return 1;
         }
+        if (!block_choice(g0, 8, true, BLOCK_RING_BYTES, BLOCK_MIN_SLABS, nullptr, nullptr, bad_k).bad) {
+            fprintf(stderr, "LBM_BLOCK_ROWS=\"%s\" accepted\n", bad_k);
+            return 1;
+        }
+    }
     printf("ok sweep choice\n");
 
     const uint32_t cases[][4] = {{4, 3, 1, 2},  {4, 3, 2, 2},  {5, 4, 3, 2},  {6, 5, 7, 2},   {5, 3, 9, 4},
@@ -279,5 +319,19 @@ int main()
     if (!ok)
         return 1;
     printf("ok blocked sweep\n");
+    // row windows: {lx, ly, lz, K, W} (W rows of the slab's nr: ragged last
+    // windows, windows of one row, W = nr - 1, one and two slabs)
+    const uint32_t wcases[][5] = {{4, 7, 9, 3, 2},  {5, 8, 6, 2, 3},  {3, 10, 4, 4, 1},  {6, 5, 7, 2, 4},
+                                  {4, 6, 6, 5, 5},  {3, 9, 2, 2, 2},  {4, 12, 5, 1, 4},  {2, 3, 1, 2, 2},
+                                  {3, 4, 11, 16, 3}, {5, 13, 8, 6, 7}, {4, 2, 9, 3, 1},  {3, 11, 3, 32, 5}};
+    for (auto &k : wcases)
+        for (int axis : {2, 1}) {
+            const uint32_t nr = axis == 2 ? k[1] : k[2];
+            if (k[4] < nr)
+                ok = check(k[0], k[1], k[2], k[3], axis, k[4]) && ok;
+        }
+    if (!ok)
+        return 1;
+    printf("ok row windows\n");
     return 0;
 }

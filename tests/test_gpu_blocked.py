@@ -53,6 +53,68 @@ def test_blocked_sweep_fp64_is_bitwise_single_steps_and_oracle(monkeypatch, shap
 
 
 @pytest.mark.parametrize("axis", ["z", "y"])
+@pytest.mark.parametrize("shape,planes,rows", [
+    ((8, 9, 11), 3, 2), ((6, 10, 7), 2, 3), ((5, 12, 13), 4, 1), ((7, 11, 9), 5, 4), ((6, 13, 12), 32, 5),
+    ((4, 5, 6), 1, 4),
+])
+def test_row_windows_fp64_bitwise_single_steps_and_oracle(monkeypatch, shape, planes, rows, axis):
+    """Row windows (LBM_BLOCK_ROWS): each window of `rows` rows is swept on
+    its own, its two edge rows computed twice; the last window may be
+    narrower.  Populations and fields bitwise those of single steps and the
+    oracle over ten pairs and a single step."""
+    lx, ly, lz = shape
+    print(f"row windows {shape} planes={planes} rows={rows} axis={axis}", flush=True)
+    monkeypatch.setenv("LBM_BLOCK_ROWS", str(rows))
+    ref, (blk, one) = _pair(monkeypatch, lx, ly, lz, "fp64", planes, seed=54, axis=axis)
+    monkeypatch.delenv("LBM_BLOCK_ROWS")
+    with blk, one:
+        ns = lz if axis == "z" else ly
+        assert blk.block_info() == (axis, min(planes, ns)) and blk.block_rows() == rows
+        assert one.block_info()[1] == 0 and one.block_rows() == 0
+        ref.macrovar()
+        for c in (blk, one, ref):
+            c.run(21)
+        fb = blk.download_f()
+        assert_bitwise(fb, one.download_f(), f"row windows vs single steps {shape}")
+        assert_bitwise(fb, ref.f, f"row windows vs oracle {shape}")
+        for name, a, b in zip(("rho", "ux", "uy", "uz"), blk.macrovar(download=True), one.macrovar(download=True)):
+            assert_bitwise(a, b, name)
+
+
+@pytest.mark.parametrize("axis", ["z", "y"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_row_windows_with_forces_per_node(monkeypatch, prec, axis):
+    """Row windows under FORCINGP's force per node (LBM_BLOCK_FORCED=1), which
+    each node takes from its lattice row: the batched loop against FORCINGP,
+    collision_MRT and macrovar called one step at a time."""
+    lx, ly, lz = 10, 14, 12
+    print(f"row windows forcingp {prec} axis={axis}", flush=True)
+    _, _, f0 = seeded_pair(lx, ly, lz, seed=36, noise=1e-4)
+    for k, v in (("LBM_BLOCK_AXIS", axis), ("LBM_BLOCK_PLANES", "3"), ("LBM_BLOCK_ROWS", "5"),
+                 ("LBM_BLOCK_FORCED", "1")):
+        monkeypatch.setenv(k, v)
+    a = d3q19.Channel(lx, ly, lz, prec)
+    for k in ("LBM_BLOCK_PLANES", "LBM_BLOCK_AXIS", "LBM_BLOCK_ROWS", "LBM_BLOCK_FORCED"):
+        monkeypatch.delenv(k)
+    b = d3q19.Channel(lx, ly, lz, prec)
+    with a, b:
+        assert a.block_info() == (axis, 3) and a.block_rows() == 5
+        for c in (a, b):
+            c.upload_f(f0)
+            c.FORCING()
+            c.macrovar()
+        a.run_forcingp(20, 25, 35)
+        b.run_forcingp_stepwise(20, 25, 35)
+        pa, pb = a.download_post(), b.download_post()
+        if prec == "fp64":
+            assert_bitwise(pa, pb, "row windows: batched vs call-by-call FORCINGP loop")
+        else:
+            # the FP32 unit may contract the two kernels' FMAs differently (an
+            # ulp, 6e-8 relative); a wrong row in the force moves values by O(F)
+            assert np.max(np.abs(pa - pb)) <= 1e-6 * np.max(np.abs(pb))
+
+
+@pytest.mark.parametrize("axis", ["z", "y"])
 @pytest.mark.parametrize("prec", ["fp32", "fp16", "fp16c"])
 def test_blocked_sweep_reduced_precision_matches_single_steps(monkeypatch, prec, axis):
     lx, ly, lz = 24, 20, 18
@@ -91,15 +153,17 @@ def test_default_sweep_on_the_turbulent_channel(monkeypatch, prec, want):
     a chunk (a 224 MB ring), along y in FP64 and z in FP32; 2-byte storage
     runs single steps.  Only the handle is made (lattices allocated and
     cleared; no step runs)."""
-    monkeypatch.delenv("LBM_BLOCK_PLANES", raising=False)
-    monkeypatch.delenv("LBM_BLOCK_AXIS", raising=False)
+    for k in ("LBM_BLOCK_PLANES", "LBM_BLOCK_AXIS", "LBM_BLOCK_ROWS"):
+        monkeypatch.delenv(k, raising=False)
     print(f"default sweep {prec}", flush=True)
     with d3q19.Channel(192, 768, 384, prec) as ch:
         assert ch.block_info() == want
-    # and the sweep stays off for the 2048x1024x512 channel's 80 MB / 40 MB slabs in FP32
+        assert ch.block_rows() == 0
+    # the 2048x1024x512 channel's 80 MB / 40 MB slabs in FP32: y slabs in four
+    # row windows of 256 rows (10 MB window slabs, 20 to a chunk)
     if prec == "fp32":
         with d3q19.Channel(512, 2048, 1024, prec) as ch:
-            assert ch.block_info()[1] == 0
+            assert ch.block_info() == ("y", 20) and ch.block_rows() == 256
 
 
 @pytest.mark.parametrize("prec,want", [("fp64", ("y", 18)), ("fp32", ("z", 18))])

@@ -11,7 +11,10 @@ size-independent:
   rounding of each step allows;
 * a whole-plane roll of the start in y and z commutes with 10 steps: the
   rolled run's slabs equal the plain run's slabs rolled, bit for bit, and the
-  plane statistics and mass agree to FP64 summation order.
+  plane statistics and mass agree to FP64 summation order;
+* FP32 pairs of steps run the blocked sweep in row windows (the slabs are too
+  large for a whole-slab ring): two pairs equal four single steps bit for bit
+  on the z planes at every window edge.
 
 Node indices at this size exceed 2^30 and FP16 byte offsets reach 2.1 GB, so
 this exercises the 32-bit node arithmetic and the addressing at their limits.
@@ -94,7 +97,8 @@ def start():
 def test_fp32_full_size(start):
     ch = _channel("fp32", start)
     with ch:
-        assert ch.block_info()[1] == 0  # 80 MB / 40 MB slabs: single steps
+        # 80 MB / 40 MB slabs: y slabs in row windows of 256 z rows
+        assert ch.block_info() == ("y", 20) and ch.block_rows() == 256
         win = np.concatenate([_slab(ch, z) for z in (Z0 - 1, Z0, Z0 + 1)], axis=1)
         m0 = ch.mass()
         ch.run(1)
@@ -125,6 +129,36 @@ def test_fp32_full_size(start):
         for z, s in slabs.items():
             r = _slab(ch, (z + DZ) % LZ).reshape(19, LY, LX)
             assert np.array_equal(bits(np.roll(r, -DY, axis=1)), bits(s.reshape(19, LY, LX))), z
+
+
+# z planes at the edges of the four row windows (z rows 0-255, 256-511,
+# 512-767, 768-1023), where a window's step-t edge rows are computed twice
+WINDOW_EDGES = (0, 1, 255, 256, 511, 512, 767, 768, 1022, 1023)
+
+
+@pytest.mark.timeout(900)
+def test_fp32_row_windows_full_size_equal_single_steps(start, monkeypatch):
+    """Two blocked pairs of the default sweep (y slabs in row windows of 256 z
+    rows) against four single steps from the same start, bit for bit, on the z
+    planes at every window edge, and the FP64 mass.  (The FP32 unit allows FMA
+    contraction; the sweep and the single step inline the same node
+    arithmetic.)"""
+    ch = _channel("fp32", start)
+    with ch:
+        assert ch.block_rows() == 256
+        ch.run(4)
+        blocked = {z: _slab(ch, z) for z in WINDOW_EDGES}
+        m_blk = ch.mass()
+    print("fp32 single steps", flush=True)
+    monkeypatch.setenv("LBM_BLOCK_PLANES", "0")
+    ch = _channel("fp32", start)
+    monkeypatch.delenv("LBM_BLOCK_PLANES")
+    with ch:
+        assert ch.block_info()[1] == 0
+        ch.run(4)
+        for z, a in blocked.items():
+            assert np.array_equal(bits(_slab(ch, z)), bits(a)), f"z plane {z}: row windows differ from single steps"
+        assert ch.mass() == m_blk
 
 
 @pytest.mark.timeout(900)
