@@ -549,9 +549,11 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         h->w32 = (w && strcmp(w, "0") == 0) ? 0 : 1;
         // two-step blocked sweep (blocked.h, block_choice): chunks of K slabs
         // whose ring of K + 2 slabs stays within BLOCK_RING_BYTES, along the
-        // axis whose chunk holds more nodes; below BLOCK_MIN_SLABS the sweep
-        // is off (it measured slower than single steps: the 2048x1024x512
-        // channel's 40 MB y slabs allow two, 33 900 against 38 200 MLUPS).  On
+        // axis whose chunk holds more nodes.  Below BLOCK_MIN_SLABS whole
+        // slabs measured slower than single steps (the 2048x1024x512
+        // channel's 40 MB y slabs allow two: 33 900 against 38 200 MLUPS), so
+        // the slabs are then cut into row windows (there: four of 256 rows,
+        // 20 to a chunk, 42 100-42 200 MLUPS in FP32).  On
         // the 192x768x384 channel this picks 11.2 MB slabs, 18 to a chunk, in
         // both FP64 (y: 20 650-20 700 MLUPS against 20 000-20 090 for z
         // slabs, 8 to a chunk) and FP32 (z: about 42 000).
