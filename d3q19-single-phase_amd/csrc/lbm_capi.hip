@@ -98,6 +98,8 @@ struct lbm_channel {
     double *stage;      // COPY_NODES*19 doubles
     hipEvent_t ev[2];
     struct Part *part;  // rigid particles (nullptr without)
+    hipStream_t pstream; // lbm_run_particles: the force sums and beads_move beside the collision
+    hipEvent_t pev[2];   // its fork (after the bounce-back) and join (before beads_links)
     float *mlp;         // learned closure parameters on the device (nullptr: plain MRT)
     float mlp_inv_ystar, mlp_yscale;
     int runs;           // step kernel: 1 = vector runs where the shape allows, 0 = one node per thread
@@ -663,6 +665,11 @@ int lbm_destroy(lbm_channel *h)
         (void)hipFree(h->bring);
     if (h->bhead)
         (void)hipFree(h->bhead);
+    for (hipEvent_t e : h->pev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (h->pstream)
+        (void)hipStreamDestroy(h->pstream);
     if (h->ev[0])
         (void)hipEventDestroy(h->ev[0]);
     if (h->ev[1])
@@ -1956,9 +1963,25 @@ int lbm_run_particles(lbm_channel *h, int istep0, int nsteps)
     CHECK_P(h);
     if (nsteps < 0)
         return fail(LBM_ERR_ARG, "lbm_run_particles: nsteps %d < 0", nsteps);
-    const int nave = h->part->pg.nave;
+    // The per-particle force sums and beads_move need only the bounce-back's
+    // link contributions, and the collision touches no particle state, so
+    // they run on a second stream beside the collision; beads_links, which
+    // rewrites the node flags the collision reads, waits for both.  Streams
+    // of the default priority.
+    if (!h->pstream) {
+        HIP_TRY(hipStreamCreateWithFlags(&h->pstream, hipStreamNonBlocking));
+        for (hipEvent_t &e : h->pev)
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    Part *pt = h->part;
+    const int nave = pt->pg.nave;
     for (int s = 0; s < nsteps; s++) {
-        int rc = lbm_beads_collision(h);
+        // beads_collision: the bounce-back here, its sums below
+        hipError_t e = h->prec == LBM_FP64   ? PartLaunch<0>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream)
+                       : h->prec == LBM_FP32 ? PartLaunch<1>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream)
+                       : h->prec == LBM_FP16 ? PartLaunch<2>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream)
+                                             : PartLaunch<3>::ibb(pt->pg, pt->ps, h->lat[h->cur], h->stream);
+        int rc = dispatch_err(e, "beads_collision");
         if (rc)
             return rc;
         // avedensity after a step that is a multiple of nave (main.f90:163-167),
@@ -1970,6 +1993,18 @@ int lbm_run_particles(lbm_channel *h, int istep0, int nsteps)
                 return rc;
             h->ave_pending = false;
         }
+        HIP_TRY(hipEventRecord(h->pev[0], h->stream));
+        HIP_TRY(hipStreamWaitEvent(h->pstream, h->pev[0], 0));
+        rc = dispatch_err(part_sum(pt->pg, pt->ps, h->pstream), "beads_collision sum");
+        if (rc)
+            return rc;
+        pt->pg.fx = h->fc.fx; // beads_move's body force (pressure_force)
+        pt->pg.fy = h->fc.fy;
+        pt->pg.fz = h->fc.fz;
+        rc = dispatch_err(part_move(pt->pg, pt->ps, h->pstream), "beads_move");
+        if (rc)
+            return rc;
+        HIP_TRY(hipEventRecord(h->pev[1], h->pstream));
         rc = launch_step(h);
         if (rc)
             return rc;
@@ -1978,9 +2013,8 @@ int lbm_run_particles(lbm_channel *h, int istep0, int nsteps)
         const long long istep = (long long)istep0 + s + 1;
         if (nave > 0 && istep % nave == 0)
             h->ave_pending = true;
-        rc = lbm_beads_move(h);
-        if (!rc)
-            rc = lbm_beads_links(h);
+        HIP_TRY(hipStreamWaitEvent(h->stream, h->pev[1], 0));
+        rc = dispatch_err(part_classify(pt->pg, pt->ps, 0, h->stream), "beads_links");
         if (!rc)
             rc = lbm_beads_filling(h);
         if (rc)
