@@ -47,9 +47,10 @@ def test_particle_boxes_stay_inside_the_lattice():
 def test_blocked_two_step_sweep_schedule():
     """tests/host/block_check.hip: the launch list of the two-step blocked
     sweep, emulated with the kernels' addressing on tagged values, gathers
-    exactly what the single-step pull gathers and writes every output slot;
-    and lbm_create's default sweep (block_choice) is the measured optimum on
-    the workloads it was tuned on, with the environment overrides."""
+    exactly what the single-step pull gathers and writes every output slot
+    once, over whole slabs and in row windows; and lbm_create's default sweep
+    (block_choice) is the measured optimum on the workloads it was tuned on,
+    with the environment overrides."""
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     with tempfile.TemporaryDirectory() as d:
         exe = os.path.join(d, "block_check")
@@ -60,6 +61,7 @@ def test_blocked_two_step_sweep_schedule():
         assert r.returncode == 0, r.stderr + r.stdout
         assert "ok sweep choice" in r.stdout
         assert "ok blocked sweep" in r.stdout
+        assert "ok row windows" in r.stdout
 
 
 @pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
