@@ -125,6 +125,7 @@ def measure(prec, workload, steps, warmup, single_steps=False):
                 os.environ["LBM_BLOCK_PLANES"] = saved
     with ch:
         axis, slabs = ch.block_info()
+        rows = ch.block_rows()
         init_channel(ch, lx, ly, lz)
         m0 = ch.mass()
         ch.run(warmup)
@@ -157,8 +158,9 @@ def measure(prec, workload, steps, warmup, single_steps=False):
                      "traffic_point": "L2-to-fabric (upper bound on HBM bytes)"},
         # change of the mean density fluctuation (sum of shifted populations / N)
         "mass_drift_per_node": (m1 - m0) / n,
-        # lbm_run's pairing of steps: sweep axis and slabs per chunk (0: single steps)
-        "blocked_sweep": {"axis": axis, "slabs_per_chunk": slabs},
+        # lbm_run's pairing of steps: sweep axis, slabs per chunk (0: single
+        # steps) and rows per window of the slabs (0: whole slabs)
+        "blocked_sweep": {"axis": axis, "slabs_per_chunk": slabs, "rows_per_window": rows},
     }
 
 
