@@ -634,6 +634,8 @@ int lbm_destroy(lbm_channel *h)
     (void)hipSetDevice(h->cfg.device);
     if (h->stream)
         (void)hipStreamSynchronize(h->stream);
+    if (h->pstream) // particle work an error return left unjoined
+        (void)hipStreamSynchronize(h->pstream);
     for (int k = 0; k < 2; k++)
         if (h->lat[k])
             (void)hipFree(h->lat[k]);
