@@ -1998,18 +1998,21 @@ int lbm_run_particles(lbm_channel *h, int istep0, int nsteps)
         HIP_TRY(hipEventRecord(h->pev[0], h->stream));
         HIP_TRY(hipStreamWaitEvent(h->pstream, h->pev[0], 0));
         rc = dispatch_err(part_sum(pt->pg, pt->ps, h->pstream), "beads_collision sum");
-        if (rc)
-            return rc;
-        pt->pg.fx = h->fc.fx; // beads_move's body force (pressure_force)
-        pt->pg.fy = h->fc.fy;
-        pt->pg.fz = h->fc.fz;
-        rc = dispatch_err(part_move(pt->pg, pt->ps, h->pstream), "beads_move");
-        if (rc)
-            return rc;
+        if (!rc) {
+            pt->pg.fx = h->fc.fx; // beads_move's body force (pressure_force)
+            pt->pg.fy = h->fc.fy;
+            pt->pg.fz = h->fc.fz;
+            rc = dispatch_err(part_move(pt->pg, pt->ps, h->pstream), "beads_move");
+        }
+        // the join point; on every return below the main stream waits for it
+        // first, so no later call races the side stream's writes
         HIP_TRY(hipEventRecord(h->pev[1], h->pstream));
-        rc = launch_step(h);
-        if (rc)
+        if (!rc)
+            rc = launch_step(h);
+        if (rc) {
+            (void)hipStreamWaitEvent(h->stream, h->pev[1], 0);
             return rc;
+        }
         fields_from_cur(h); // macrovar
         h->rho_off = 0.0;
         const long long istep = (long long)istep0 + s + 1;
