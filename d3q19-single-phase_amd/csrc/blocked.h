@@ -17,7 +17,9 @@
 // into a head buffer, and one last launch computes those two slabs.  The
 // periodic wraps along the sweep axis are the slab indices taken modulo ns;
 // along the rows they wrap within a slab.  (Running A_i and B_{i-1} as one
-// launch over a ring of 2K + 2 slots measured slower.)
+// launch over a ring of 2K + 2 slots measured slower.)  When whole slabs are
+// too large for a useful chunk, the slabs are cut into row windows and the
+// whole list above runs once per window (block_windowed).
 #pragma once
 
 #include <string.h>
