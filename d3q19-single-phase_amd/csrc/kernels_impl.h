@@ -56,11 +56,13 @@ __device__ __forceinline__ const typename Prec<P>::S *elem(const typename Prec<P
 // F_I(x): streamed population of direction I at node x.  The bounce-back
 // choice is made with masks, not branches, so every load issues back to back.
 // Loads are plain (cached): the +-x shifted reads of neighbouring waves share
-// cache lines.  W32 (2-byte storage, N <= W32_MAX_N): a pull with c_x != 0
-// addresses both of its rows, I and OPP[I], from the lower of the two with a
-// 32-bit byte offset (|OPP[I] - I| <= 3, so below 4N elements, 8N bytes), and
-// the lane selects between two 32-bit offsets instead of two 64-bit row bases.
+// cache lines.  W32 (N <= w32_max_n(P)): a pull with c_x != 0 addresses both
+// of its rows, I and OPP[I], from the lower of the two with a 32-bit byte
+// offset (|OPP[I] - I| <= 3, so below 4N elements: 8N bytes in 2-byte
+// storage, 16N in FP32), and the lane selects between two 32-bit offsets
+// instead of two 64-bit row bases.
 constexpr uint64_t W32_MAX_N = 1ull << 29;
+constexpr uint64_t w32_max_n(int P) { return half_storage(P) ? W32_MAX_N : (P == 1 ? (1ull << 28) : 0); }
 
 template <int P, int I, bool W32 = false>
 __device__ __forceinline__ typename Prec<P>::T pull(const typename Prec<P>::S *__restrict__ L, uint64_t N,
@@ -77,7 +79,7 @@ __device__ __forceinline__ typename Prec<P>::T pull(const typename Prec<P>::S *_
             // memory-bound: this form measured 1.4 % faster than the one below
             return Prec<P>::template to_T<I>(L[(w ? (uint64_t)OPP[I] * N : (uint64_t)I * N) + node]);
         } else if constexpr (W32) {
-            static_assert(half_storage(P), "32-bit pull offsets are for 2-byte storage");
+            static_assert(half_storage(P) || P == 1, "32-bit pull offsets are for FP32 and 2-byte storage");
             constexpr int M = I < OPP[I] ? I : OPP[I];
             uint64_t row_m = (uint64_t)M * N;
             uint32_t d_i = (uint32_t)(I - M) * (uint32_t)N, d_o = (uint32_t)(OPP[I] - M) * (uint32_t)N;
@@ -86,7 +88,12 @@ __device__ __forceinline__ typename Prec<P>::T pull(const typename Prec<P>::S *_
             __asm__("" : "+s"(row_m));
             __asm__("" : "+s"(d_i));
             __asm__("" : "+s"(d_o));
-            return Prec<P>::template to_T<I>(*elem<P>(L + row_m, w ? b.n + d_o : src_of<I>(b) + d_i));
+            // element offset below 4N, its byte offset below 2^32 (w32_max_n):
+            // the load takes the scalar row base and a 32-bit lane offset
+            using S = typename Prec<P>::S;
+            const uint32_t off = w ? b.n + d_o : src_of<I>(b) + d_i;
+            return Prec<P>::template to_T<I>(
+                *(const S *)((const char *)(L + row_m) + off * (uint32_t)sizeof(S)));
         } else {
             uint64_t row_i = (uint64_t)I * N, row_o = (uint64_t)OPP[I] * N;
             // opaque to the optimiser, which would otherwise turn the select
@@ -1227,27 +1234,45 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
     }
     // one launch geometry per kernel's block size (step_block)
     auto geo = [&](int nb) { return std::make_pair(dim3((uint32_t)((a.g.N + nb - 1) / nb)), dim3(nb)); };
-    if constexpr (half_storage(P)) {
-        // the main loop's modes with 32-bit pull offsets where the lattice
-        // allows them (FP16 storage 1.8-2.5 % faster, FP16C 2.7-2.9 %, on the
-        // 192x768x384 channel)
-        if (a.w32 && a.g.N <= W32_MAX_N) {
+    if constexpr (half_storage(P) || P == 1) {
+        // 32-bit pull offsets where the lattice allows them: the main loop's
+        // modes in 2-byte storage (FP16 1.8-2.5 % faster, FP16C 2.7-2.9 %, on
+        // the 192x768x384 channel; FP32's memory-bound step measured no
+        // change), and the closure modes, which are issue-bound, in FP32 and
+        // 2-byte storage
+        if (a.w32 && a.g.N <= w32_max_n(P)) {
             switch (mode) {
             case 0:
-                hipLaunchKernelGGL((k_step<P, 32>), geo(step_block<P, 32>()).first, geo(step_block<P, 32>()).second, 0,
-                                   s, a);
-                return hipGetLastError();
             case 1:
-                hipLaunchKernelGGL((k_step<P, 33>), geo(step_block<P, 33>()).first, geo(step_block<P, 33>()).second, 0,
-                                   s, a);
-                return hipGetLastError();
             case 4:
-                hipLaunchKernelGGL((k_step<P, 36>), geo(step_block<P, 36>()).first, geo(step_block<P, 36>()).second, 0,
-                                   s, a);
-                return hipGetLastError();
             case 16:
-                hipLaunchKernelGGL((k_step<P, 48>), geo(step_block<P, 48>()).first, geo(step_block<P, 48>()).second, 0,
-                                   s, a);
+                if constexpr (half_storage(P)) {
+                    if (mode == 0)
+                        hipLaunchKernelGGL((k_step<P, 32>), geo(step_block<P, 32>()).first,
+                                           geo(step_block<P, 32>()).second, 0, s, a);
+                    else if (mode == 1)
+                        hipLaunchKernelGGL((k_step<P, 33>), geo(step_block<P, 33>()).first,
+                                           geo(step_block<P, 33>()).second, 0, s, a);
+                    else if (mode == 4)
+                        hipLaunchKernelGGL((k_step<P, 36>), geo(step_block<P, 36>()).first,
+                                           geo(step_block<P, 36>()).second, 0, s, a);
+                    else
+                        hipLaunchKernelGGL((k_step<P, 48>), geo(step_block<P, 48>()).first,
+                                           geo(step_block<P, 48>()).second, 0, s, a);
+                    return hipGetLastError();
+                }
+                break;
+            case 8:
+                hipLaunchKernelGGL((k_step_closure<P, 40>), geo(step_block<P, 40>()).first,
+                                   geo(step_block<P, 40>()).second, 0, s, a);
+                return hipGetLastError();
+            case 12:
+                hipLaunchKernelGGL((k_step_closure<P, 44>), geo(step_block<P, 44>()).first,
+                                   geo(step_block<P, 44>()).second, 0, s, a);
+                return hipGetLastError();
+            case 24:
+                hipLaunchKernelGGL((k_step_closure<P, 56>), geo(step_block<P, 56>()).first,
+                                   geo(step_block<P, 56>()).second, 0, s, a);
                 return hipGetLastError();
             default:
                 break;

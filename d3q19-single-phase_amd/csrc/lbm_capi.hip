@@ -544,9 +544,9 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         // the 192x768x384 channel: 64.6 % vs 69.2 % of HBM in FP64)
         const char *k = getenv("LBM_STEP_KERNEL");
         h->runs = (k && strcmp(k, "runs") == 0) ? 1 : 0;
-        // LBM_STEP_W32=0: 2-byte storage addresses its walled pulls from two
-        // 64-bit row bases even where 32-bit offsets would do (the same values;
-        // for comparing the two forms)
+        // LBM_STEP_W32=0: 2-byte storage and the FP32 closure address their
+        // walled pulls from two 64-bit row bases even where 32-bit offsets
+        // would do (the same values; for comparing the two forms)
         const char *w = getenv("LBM_STEP_W32");
         h->w32 = (w && strcmp(w, "0") == 0) ? 0 : 1;
         // two-step blocked sweep (blocked.h, block_choice): chunks of K slabs

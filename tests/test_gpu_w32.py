@@ -1,8 +1,9 @@
-"""2-byte storage addresses the pulls with c_x != 0 either from two 64-bit
-row bases or, where the lattice allows (N <= 2^29), from the lower row with a
-32-bit offset (step mode + 32, csrc/kernels_impl.h: pull).  The two forms load
-the same values, so every main-loop mode that takes the 32-bit form must give
-the same stored words as LBM_STEP_W32=0 does, bit for bit."""
+"""2-byte storage (and the FP32 closure kernel) addresses the pulls with
+c_x != 0 either from two 64-bit row bases or, where the lattice allows
+(N <= 2^29 in 2-byte storage, 2^28 in FP32), from the lower row with a 32-bit
+offset (step mode + 32, csrc/kernels_impl.h: pull).  The two forms load the
+same values, so every mode that takes the 32-bit form must give the same
+stored words as LBM_STEP_W32=0 does, bit for bit."""
 import numpy as np
 import pytest
 
@@ -75,3 +76,29 @@ def test_w32_pulls_with_particles(monkeypatch, prec):
     assert np.array_equal(bits(fa), bits(fb)), f"{prec}: lattice differs"
     for key in ("pos", "vel", "force"):
         assert np.array_equal(pa[key], pb[key]), f"{prec}: particle {key} differs"
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp16c"])
+@pytest.mark.parametrize("case", ["plain", "forcingp"])
+def test_w32_pulls_in_the_closure_kernel(monkeypatch, prec, case):
+    """The closure modes (8, and 24 under a force per node) with 32-bit pull
+    offsets against the row-base form."""
+    lx, ly, lz = 20, 24, 18
+    print(f"w32 closure {prec} {case}", flush=True)
+    _, _, f0 = seeded_pair(lx, ly, lz, seed=72, noise=1e-4)
+
+    def make():
+        with d3q19.Channel(lx, ly, lz, prec) as ch:
+            p = d3q19.closure_params(seed=5, nu=ch.p["visc"])
+            ch.upload_f(f0)
+            ch.FORCING()
+            ch.macrovar()
+            ch.closure_set(p, ystar=ch.p["ystar"])
+            if case == "plain":
+                ch.run(12)
+            else:
+                ch.run_forcingp(0, 12, 10**9)
+            return ch.download_post()
+
+    a, b = _twin(monkeypatch, make)
+    assert np.array_equal(bits(a), bits(b)), f"{prec} {case}: 32-bit and row-base closure pulls differ"
