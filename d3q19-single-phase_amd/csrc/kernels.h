@@ -67,7 +67,8 @@ struct StepArgs {
     float inv_ystar;   // u*/nu: y+ per lattice unit of wall distance
     float yscale;      // y+ feature divisor
     int runs;          // 1: modes 0 and 1 of FP64/FP32 take the vector-run kernel when lx allows
-    int w32;           // 1: 2-byte storage takes the 32-bit pull offsets when N allows (step mode + 32)
+    int w32;           // 1: 32-bit pull offsets when N allows (step mode + 32, or + 64 between adjacent
+                       // rows above 2^29 nodes in 2-byte storage); 2: + 64 wherever N <= 2^30
     ForceSpec fc;      // mode bit 16: collision force (r.fx, r.fy, r.fz are then unused)
     ForceSpec fm;      // mode bit 16: force that the fused fields u = j + F/2 are formed with
 };

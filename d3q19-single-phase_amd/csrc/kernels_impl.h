@@ -56,15 +56,18 @@ __device__ __forceinline__ const typename Prec<P>::S *elem(const typename Prec<P
 // F_I(x): streamed population of direction I at node x.  The bounce-back
 // choice is made with masks, not branches, so every load issues back to back.
 // Loads are plain (cached): the +-x shifted reads of neighbouring waves share
-// cache lines.  W32 (N <= w32_max_n(P)): a pull with c_x != 0 addresses both
-// of its rows, I and OPP[I], from the lower of the two with a 32-bit byte
-// offset (|OPP[I] - I| <= 3, so below 4N elements: 8N bytes in 2-byte
+// cache lines.  W32 = 1 (N <= w32_max_n(P)): a pull with c_x != 0 addresses
+// both of its rows, I and OPP[I], from the lower of the two with a 32-bit
+// byte offset (|OPP[I] - I| <= 3, so below 4N elements: 8N bytes in 2-byte
 // storage, 16N in FP32), and the lane selects between two 32-bit offsets
-// instead of two 64-bit row bases.
+// instead of two 64-bit row bases.  W32 = 2 (2-byte storage, N <= 2^30): only
+// the pulls whose rows are adjacent (|OPP[I] - I| = 1: I = 1, 2, 8, 9, 12,
+// 13; offsets below 2N elements, 4N bytes) take that form.
 constexpr uint64_t W32_MAX_N = 1ull << 29;
 constexpr uint64_t w32_max_n(int P) { return half_storage(P) ? W32_MAX_N : (P == 1 ? (1ull << 28) : 0); }
+constexpr uint64_t W32_ADJ_MAX_N = 1ull << 30;
 
-template <int P, int I, bool W32 = false>
+template <int P, int I, int W32 = 0>
 __device__ __forceinline__ typename Prec<P>::T pull(const typename Prec<P>::S *__restrict__ L, uint64_t N,
                                                     const Nbr &b)
 {
@@ -78,7 +81,7 @@ __device__ __forceinline__ typename Prec<P>::T pull(const typename Prec<P>::S *_
         if constexpr (P == 0) {
             // memory-bound: this form measured 1.4 % faster than the one below
             return Prec<P>::template to_T<I>(L[(w ? (uint64_t)OPP[I] * N : (uint64_t)I * N) + node]);
-        } else if constexpr (W32) {
+        } else if constexpr (W32 == 1 || (W32 == 2 && (OPP[I] - I == 1 || I - OPP[I] == 1))) {
             static_assert(half_storage(P) || P == 1, "32-bit pull offsets are for FP32 and 2-byte storage");
             constexpr int M = I < OPP[I] ? I : OPP[I];
             uint64_t row_m = (uint64_t)M * N;
@@ -110,14 +113,14 @@ __device__ __forceinline__ typename Prec<P>::T pull(const typename Prec<P>::S *_
     }
 }
 
-template <int P, bool W32, int... Is>
+template <int P, int W32, int... Is>
 __device__ __forceinline__ void pull_all_impl(const typename Prec<P>::S *__restrict__ L, uint64_t N, const Nbr &b,
                                               typename Prec<P>::T (&f)[19], std::integer_sequence<int, Is...>)
 {
     ((f[Is] = pull<P, Is, W32>(L, N, b)), ...);
 }
 
-template <int P, bool W32 = false>
+template <int P, int W32 = 0>
 __device__ __forceinline__ void pull_all(const typename Prec<P>::S *__restrict__ L, uint64_t N, const Nbr &b,
                                          typename Prec<P>::T (&f)[19])
 {
@@ -219,8 +222,10 @@ __device__ __forceinline__ uint32_t node_index(const Geom &g)
 // fields were formed with another force than the collision's (the force was
 // changed after the last macrovar): a.fc is the collision's force, a.fm the
 // fields'.
-// MODE + 32: 2-byte storage with N <= W32_MAX_N: the pulls with c_x != 0 take
-// 32-bit offsets (pull<P, I, true>); the same loads, fewer instructions.
+// MODE + 32: 2-byte storage with N <= W32_MAX_N (and the FP32 closure with
+// N <= 2^28): the pulls with c_x != 0 take 32-bit offsets (pull<P, I, 1>);
+// the same loads, fewer instructions.  MODE + 64: 2-byte storage with
+// N <= 2^30: the pulls between adjacent rows only (pull<P, I, 2>).
 template <int P, int MODE>
 __device__ __forceinline__ void step_node(const StepArgs &a)
 {
@@ -249,7 +254,7 @@ __device__ __forceinline__ void step_node(const StepArgs &a)
     // the 19 loads issue first; the closure's weights are staged while they
     // are in flight
     T f[19];
-    pull_all<P, (MODE & 32) != 0>(src, a.g.N, b, f);
+    pull_all<P, (MODE & 32) != 0 ? 1 : ((MODE & 64) != 0 ? 2 : 0)>(src, a.g.N, b, f);
     __shared__ __attribute__((aligned(16))) _Float16 wl[WAVE ? MLP_LDS : 8];
     if constexpr (WAVE) {
         mlp_stage(a.mlp, wl, threadIdx.x, NB);
@@ -1240,7 +1245,7 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
         // the 192x768x384 channel; FP32's memory-bound step measured no
         // change), and the closure modes, which are issue-bound, in FP32 and
         // 2-byte storage
-        if (a.w32 && a.g.N <= w32_max_n(P)) {
+        if (a.w32 == 1 && a.g.N <= w32_max_n(P)) {
             switch (mode) {
             case 0:
             case 1:
@@ -1273,6 +1278,32 @@ hipError_t Launch<P>::step(const StepArgs &a, int mode, hipStream_t s)
             case 24:
                 hipLaunchKernelGGL((k_step_closure<P, 56>), geo(step_block<P, 56>()).first,
                                    geo(step_block<P, 56>()).second, 0, s, a);
+                return hipGetLastError();
+            default:
+                break;
+            }
+        }
+    }
+    if constexpr (half_storage(P)) {
+        // above 2^29 nodes (the 2048x1024x512 channel), the pulls between
+        // adjacent rows only (step mode + 64)
+        if (a.w32 && (a.g.N > w32_max_n(P) || a.w32 == 2) && a.g.N <= W32_ADJ_MAX_N) {
+            switch (mode) {
+            case 0:
+                hipLaunchKernelGGL((k_step<P, 64>), geo(step_block<P, 64>()).first, geo(step_block<P, 64>()).second, 0,
+                                   s, a);
+                return hipGetLastError();
+            case 1:
+                hipLaunchKernelGGL((k_step<P, 65>), geo(step_block<P, 65>()).first, geo(step_block<P, 65>()).second, 0,
+                                   s, a);
+                return hipGetLastError();
+            case 4:
+                hipLaunchKernelGGL((k_step<P, 68>), geo(step_block<P, 68>()).first, geo(step_block<P, 68>()).second, 0,
+                                   s, a);
+                return hipGetLastError();
+            case 16:
+                hipLaunchKernelGGL((k_step<P, 80>), geo(step_block<P, 80>()).first, geo(step_block<P, 80>()).second, 0,
+                                   s, a);
                 return hipGetLastError();
             default:
                 break;

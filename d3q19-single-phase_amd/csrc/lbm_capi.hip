@@ -103,7 +103,7 @@ struct lbm_channel {
     float *mlp;         // learned closure parameters on the device (nullptr: plain MRT)
     float mlp_inv_ystar, mlp_yscale;
     int runs;           // step kernel: 1 = vector runs where the shape allows, 0 = one node per thread
-    int w32;            // step kernel, 2-byte storage: 1 = 32-bit pull offsets where N allows (default)
+    int w32;            // step kernel: 1 = 32-bit pull offsets where N allows (default), 2 = between adjacent rows only
     uint32_t bK;        // two-step blocked sweep: slabs per chunk (0 = single steps only)
     int baxis;          // its sweep axis: 2 = z (slabs lx*ly), 1 = y (slabs lx*lz)
     uint32_t bW;        // rows per window of its slabs (0 = whole slabs)
@@ -546,9 +546,11 @@ int lbm_create(const lbm_config *cfg, lbm_channel **out)
         h->runs = (k && strcmp(k, "runs") == 0) ? 1 : 0;
         // LBM_STEP_W32=0: 2-byte storage and the FP32 closure address their
         // walled pulls from two 64-bit row bases even where 32-bit offsets
-        // would do (the same values; for comparing the two forms)
+        // would do; LBM_STEP_W32=adjacent: 2-byte storage takes 32-bit
+        // offsets between adjacent rows only, the form lattices above 2^29
+        // nodes use (the same values; for comparing the forms)
         const char *w = getenv("LBM_STEP_W32");
-        h->w32 = (w && strcmp(w, "0") == 0) ? 0 : 1;
+        h->w32 = (w && strcmp(w, "0") == 0) ? 0 : (w && strcmp(w, "adjacent") == 0) ? 2 : 1;
         // two-step blocked sweep (blocked.h, block_choice): chunks of K slabs
         // whose ring of K + 2 slabs stays within BLOCK_RING_BYTES, along the
         // axis whose chunk holds more nodes.  Below BLOCK_MIN_SLABS whole

@@ -14,10 +14,11 @@ from test_gpu_parity import bits, seeded_pair
 pytestmark = pytest.mark.gpu
 
 
-def _twin(monkeypatch, make):
-    """make(w32_env) -> result for the default handle and one with LBM_STEP_W32=0."""
+def _twin(monkeypatch, make, first=None):
+    """make() -> result for a handle with LBM_STEP_W32=first (unset: the
+    default) and one with LBM_STEP_W32=0."""
     out = []
-    for env in (None, "0"):
+    for env in (first, "0"):
         if env is None:
             monkeypatch.delenv("LBM_STEP_W32", raising=False)
         else:
@@ -27,11 +28,14 @@ def _twin(monkeypatch, make):
     return out
 
 
+@pytest.mark.parametrize("form", [None, "adjacent"])
 @pytest.mark.parametrize("prec", ["fp16", "fp16c"])
 @pytest.mark.parametrize("case", ["plain", "rho_offset", "forcingp"])
-def test_w32_pulls_store_the_same_words(monkeypatch, prec, case):
+def test_w32_pulls_store_the_same_words(monkeypatch, prec, case, form):
+    """form None: all walled pulls with 32-bit offsets (N <= 2^29); "adjacent":
+    only those between adjacent rows, the form of lattices above 2^29 nodes."""
     lx, ly, lz = 24, 40, 30
-    print(f"w32 {prec} {case}", flush=True)
+    print(f"w32 {prec} {case} {form}", flush=True)
     _, _, f0 = seeded_pair(lx, ly, lz, seed=71, noise=1e-4)
 
     def make():
@@ -50,8 +54,8 @@ def test_w32_pulls_store_the_same_words(monkeypatch, prec, case):
                 ch.run_forcingp(0, 20, 7)
             return ch.download_post()
 
-    a, b = _twin(monkeypatch, make)
-    assert np.array_equal(bits(a), bits(b)), f"{prec} {case}: 32-bit and row-base pulls differ"
+    a, b = _twin(monkeypatch, make, form)
+    assert np.array_equal(bits(a), bits(b)), f"{prec} {case} {form}: 32-bit and row-base pulls differ"
 
 
 @pytest.mark.parametrize("prec", ["fp16", "fp16c"])
